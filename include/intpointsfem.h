@@ -1,0 +1,145 @@
+/*
+ * intpointsfem.h -- C ABI of the MI355X (gfx950) integration-point residual
+ * path for 2D plane-strain Mohr-Coulomb elastoplasticity.
+ *
+ * Two layers:
+ *   ipf_ctx_*  / ipf_* device entry points: the data and operations the
+ *     reference's GPU path keeps behind TPZCudaCalls / TPZVecGPU and the
+ *     TransferDataToGPU methods.  Plain pointers and sizes only.
+ *   ipf_cmesh_* / ipf_sm_* front-end entry points: the mesh stand-in and the
+ *     TPZElastoPlasticIntPointsStructMatrix drop-in built on the layer above.
+ *
+ * Conventions
+ *   - Every call returns int: IPF_OK (0) on success, a negative IPF_E* code
+ *     otherwise.  ipf_last_error(ctx) / ipf_global_last_error() give the text.
+ *     No C++ exception crosses this boundary.
+ *   - Arrays named *_dev are device pointers (HBM), *_host are host pointers.
+ *     Per-point arrays use the reference's AoS layout and point order:
+ *     strain/stress [3*npts] as [xx, xy, yy] (engineering shear strain),
+ *     tangent [9*npts] row-major 3x3, state [6*npts] in Voigt order
+ *     [XX, XY, XZ, YY, YZ, ZZ].
+ *   - Device entry points enqueue on the context's stream and return without
+ *     waiting; ipf_synchronize() waits and reports asynchronous errors.  The
+ *     *_host variants copy, run and wait.
+ *   - A context belongs to one host thread at a time.
+ */
+#ifndef INTPOINTSFEM_H
+#define INTPOINTSFEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    IPF_OK = 0,
+    IPF_EINVAL = -1,      /* bad argument or inconsistent input data */
+    IPF_EHIP = -2,        /* HIP runtime error */
+    IPF_ESTATE = -3,      /* call out of order (e.g. residual before setup) */
+    IPF_EUNSUPPORTED = -4 /* element shape without a compiled kernel */
+};
+
+/* Branch ids written next to the reference's mtype flag (0 elastic,
+ * 1 main plane or edge, -1 apex; FunctionsSigmaProjection.h:341,353,371). */
+enum { IPF_BR_ELASTIC = 0, IPF_BR_MAIN = 1, IPF_BR_RIGHT = 2, IPF_BR_LEFT = 3, IPF_BR_APEX = 4 };
+
+typedef struct ipf_ctx ipf_ctx;
+
+/* ------------------------------------------------------------------ */
+/* Device context                                                      */
+/* ------------------------------------------------------------------ */
+
+/* Creates a context on HIP device `device` with its own stream.
+ * Replaces the per-object TPZCudaCalls handles (TPZCudaCalls.h:15-50). */
+int ipf_ctx_create(int device, ipf_ctx **out);
+int ipf_ctx_destroy(ipf_ctx *ctx);
+const char *ipf_last_error(const ipf_ctx *ctx);
+const char *ipf_global_last_error(void);
+
+/* Use a caller-owned stream (hipStream_t passed as void*); NULL restores the
+ * context's own stream. */
+int ipf_set_stream(ipf_ctx *ctx, void *hip_stream);
+void *ipf_get_stream(const ipf_ctx *ctx);
+int ipf_synchronize(ipf_ctx *ctx);
+
+/* Uploads the integrator data and lays it out for the kernels.  Replaces
+ * TPZNumericalIntegrator::TransferDataToGPU (TPZNumericalIntegrator.cpp:218-233),
+ * TPZIrregularBlocksMatrix::TransferDataToGPU (TPZIrregularBlocksMatrix.cpp:144-174)
+ * and TPZConstitutiveLawProcessor::TransferDataToGPU (TPZConstitutiveLawProcessor.cpp:739-742).
+ *   blocks: IrregularBlocks fields (TPZIrregularBlocksMatrix.h:18-30) --
+ *     nblocks, fRowSizes, fColSizes, fMatrixPosition (int64 here: the
+ *     reference's int overflows beyond 2^31 stored values), fRowFirstIndex,
+ *     fColFirstIndex, fStorage (row-major R_e x C_e blocks);
+ *   dof_indexes [cols]   fDoFIndexes   (TPZNumericalIntegrator.h:88)
+ *   color_indexes [cols] fColorIndexes = dof + color*neq (TPZNumericalIntegrator.cpp:503)
+ *   ncolor, neq          fNColor, cmesh->NEquations()
+ *   weight [npts]        fWeight = w_gauss*|detJ| (TPZNumericalIntegrator.cpp:425)
+ * The state (plastic strain, alpha, mtype) is reset to zero
+ * (TPZConstitutiveLawProcessor::SetUpDataByIntPoints, .cpp:86-121).
+ * Fails with IPF_EINVAL when sizes are inconsistent, an index is out of range,
+ * an element's color indexes disagree, or two elements of one color share a
+ * DoF; with IPF_EUNSUPPORTED for a block shape without a kernel. */
+int ipf_transfer_data(ipf_ctx *ctx, int64_t nblocks, const int *row_sizes, const int *col_sizes,
+                      const int64_t *matrix_position, const int *row_first, const int *col_first,
+                      const double *storage, const int *dof_indexes, const int *color_indexes, int64_t ncolor,
+                      int64_t neq, const double *weight);
+
+/* Material of TPZConstitutiveLawProcessor::SetMaterial (.cpp:127-129); the
+ * reference reads lambda, mu, phi, psi, c from it per call (.cpp:152-158).
+ * Angles in radians. */
+int ipf_set_material(ipf_ctx *ctx, double lambda, double mu, double mc_phi, double mc_psi, double mc_cohesion);
+
+/* fRhsLinear / dRhsLinear (TPZElastoPlasticIntPointsStructMatrix.cpp:63-68,156-167), [neq] host array. */
+int ipf_set_rhs_linear(ipf_ctx *ctx, const double *f_lin_host);
+
+/* The UpdateMem switch of the material (TPZConstitutiveLawProcessor.cpp:197):
+ * when set, residual / compute_sigma calls commit plastic strain, alpha, mtype
+ * and branch per point. */
+int ipf_set_update_mem(ipf_ctx *ctx, int update_mem);
+
+/* Sizes: out[0..7] = nblocks, npts, rows, cols, neq, ncolor, storage values,
+ * device bytes held. */
+int ipf_info(const ipf_ctx *ctx, int64_t out[8]);
+
+/* ---- the hot path ---- */
+
+/* r = f_lin - sum_e B_e^T (w * sigma(B_e u_e)), device pointers [neq].
+ * TPZElastoPlasticIntPointsStructMatrix::Assemble(rhs)
+ * (TPZElastoPlasticIntPointsStructMatrix.cpp:137-154) ->
+ * TPZNumericalIntegrator::ResidualIntegration (TPZNumericalIntegrator.cpp:107-128).
+ * One fused kernel per element color: gather, B.u, return map, B^T (w sigma)
+ * and the color's atomic-free accumulation into r. */
+int ipf_residual(ipf_ctx *ctx, const double *u_dev, double *r_dev);
+int ipf_residual_host(ipf_ctx *ctx, const double *u_host, double *r_host);
+
+/* ---- the reference's separate steps (same results, unfused) ---- */
+
+/* y = B x (opt == 0; x [cols], y [rows]) or y = -B^T x (opt != 0; x [rows],
+ * y [cols]) without gather/scatter, device pointers.
+ * TPZIrregularBlocksMatrix::MultiplyVector (TPZIrregularBlocksMatrix.cpp:97-140). */
+int ipf_multiply_vector(ipf_ctx *ctx, const double *x_dev, double *y_dev, int opt);
+
+/* strain = B . u[dof]   TPZNumericalIntegrator::Multiply (TPZNumericalIntegrator.cpp:25-51) */
+int ipf_multiply(ipf_ctx *ctx, const double *u_dev, double *strain_dev);
+/* w*sigma per point     TPZConstitutiveLawProcessor::ComputeSigma (TPZConstitutiveLawProcessor.cpp:131-342, 588-659) */
+int ipf_compute_sigma(ipf_ctx *ctx, const double *strain_dev, double *wsigma_dev);
+/* w*sigma and w*Dep     TPZConstitutiveLawProcessor::ComputeSigmaDep (TPZConstitutiveLawProcessor.cpp:344-585, 663-735) */
+int ipf_compute_sigma_dep(ipf_ctx *ctx, const double *strain_dev, double *wsigma_dev, double *wdep_dev);
+/* res = -B^T wsigma, colored accumulation   TPZNumericalIntegrator::MultiplyTranspose (TPZNumericalIntegrator.cpp:54-105) */
+int ipf_multiply_transpose(ipf_ctx *ctx, const double *wsigma_dev, double *res_dev);
+
+/* ---- integration-point state (TPZConstitutiveLawProcessor.h:53-63) ---- */
+
+/* Copies the committed state to host arrays (any may be NULL):
+ * plastic_strain [6*npts] Voigt, alpha [npts], mtype [npts], branch [npts]. */
+int ipf_get_state(ipf_ctx *ctx, double *plastic_strain_host, double *alpha_host, int *mtype_host,
+                  int *branch_host);
+/* Sets the committed state (restart).  The XZ and YZ plastic components must
+ * be zero (plane strain); alpha may be NULL (zero). */
+int ipf_set_state(ipf_ctx *ctx, const double *plastic_strain_host, const double *alpha_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,70 @@
+/*
+ * intpointsfem_frontend.h -- C ABI of the host front-end: the mesh stand-in
+ * (TPZCompMesh) and the drop-in structural matrix
+ * TPZElastoPlasticIntPointsStructMatrix built on include/intpointsfem.h.
+ * Same conventions as intpointsfem.h (int status, ipf_global_last_error()).
+ */
+#ifndef INTPOINTSFEM_FRONTEND_H
+#define INTPOINTSFEM_FRONTEND_H
+
+#include <stdint.h>
+
+#include "intpointsfem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ipf_cmesh ipf_cmesh;
+typedef struct ipf_sm ipf_sm;
+
+/* Structured quarter annulus r in [r_in, r_out], theta in [0, pi/2]:
+ * nr radial cells growing geometrically by `ratio`, ntheta uniform angular
+ * cells, straight-sided Q_p Lagrange quadrilaterals, p in 1..3.
+ * Stands in for the gmsh + NeoPZ mesh of main.cpp:447-464, 542-607. */
+int ipf_cmesh_quarter_annulus(int nr, int ntheta, int porder, double r_in, double r_out, double ratio,
+                              ipf_cmesh **out);
+int ipf_cmesh_destroy(ipf_cmesh *m);
+/* Material (TElastoPlasticData + TPZElasticResponse): E, nu, cohesion,
+ * friction and dilation angles in radians. */
+int ipf_cmesh_set_material(ipf_cmesh *m, double E, double nu, double cohesion, double phi, double psi);
+/* Normal tractions sigma_n on the inner / outer arcs, t = sigma_n * n_out
+ * (BC type 6 of main.cpp:397-405 in perturbation form). */
+int ipf_cmesh_set_normal_stress(ipf_cmesh *m, double inner, double outer);
+/* out[0..7] = nelem, nnodes, neq, porder, nr, ntheta, npts_per_element, nshape */
+int ipf_cmesh_info(const ipf_cmesh *m, int64_t out[8]);
+/* out[0..4] = lambda, mu, phi, psi, cohesion */
+int ipf_cmesh_material(const ipf_cmesh *m, double out[5]);
+int ipf_cmesh_nodes(const ipf_cmesh *m, double *xy /* [2*nnodes] */);
+/* connect (node) lists per element: conn_ptr [nelem+1], conn [conn_ptr[nelem]] */
+int ipf_cmesh_connectivity(const ipf_cmesh *m, int64_t *conn_ptr, int64_t *conn);
+/* 1 for DoFs fixed by the symmetry conditions (u_y on theta=0, u_x on theta=pi/2) */
+int ipf_cmesh_dirichlet_mask(const ipf_cmesh *m, signed char *mask /* [neq] */);
+/* the material's UpdateMem switch (main.cpp:487-522) */
+int ipf_cmesh_set_update_mem(ipf_cmesh *m, int update_mem);
+
+/* TPZElastoPlasticIntPointsStructMatrix(cmesh) + SetUpDataStructure
+ * (TPZElastoPlasticIntPointsStructMatrix.cpp:15-25, :80-97).  With
+ * device >= 0 the data is also transferred to that GPU; with device < 0 only
+ * the host data is built (no GPU needed). */
+int ipf_sm_create(ipf_cmesh *m, int device, ipf_sm **out);
+int ipf_sm_destroy(ipf_sm *sm);
+/* Device context of the structural matrix (NULL before the transfer). */
+ipf_ctx *ipf_sm_context(ipf_sm *sm);
+/* out[0..7] = nblocks, npts, rows, cols, neq, ncolor, storage values, 0 */
+int ipf_sm_sizes(const ipf_sm *sm, int64_t out[8]);
+/* IrregularBlocks arrays (TPZIrregularBlocksMatrix.h:18-30), host copies */
+int ipf_sm_export_blocks(const ipf_sm *sm, int *row_sizes, int *col_sizes, int64_t *matrix_position,
+                         int *row_first, int *col_first, double *storage);
+/* fDoFIndexes, fColorIndexes, fElColorIndex [nblocks], fFirstColorIndex
+ * [ncolor+1], weights [npts], fRhsLinear [neq]; any may be NULL */
+int ipf_sm_export_indexes(const ipf_sm *sm, int *dof_indexes, int *color_indexes, int *el_color_index,
+                          int64_t *first_color_index, double *weight, double *rhs_linear);
+/* Assemble(rhs) for u (TPZElastoPlasticIntPointsStructMatrix.cpp:137-154),
+ * host arrays [neq]; commits the state when the mesh's UpdateMem is set. */
+int ipf_sm_assemble_rhs(ipf_sm *sm, const double *u_host, double *rhs_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,378 @@
+"""MI355X (gfx950) integration-point residual path for 2D plane-strain
+Mohr-Coulomb elastoplasticity, behind IntPointsFEM's operator API.
+
+The compute lives in ``libintpointsfem.so`` (host C++ + hand-written HIP
+kernels, C ABI in ``include/intpointsfem.h`` and
+``include/intpointsfem_frontend.h``).  This module is a thin ctypes binding:
+
+* :class:`CompMesh` -- the structured quarter-annulus mesh stand-in
+  (TPZCompMesh) with material and boundary tractions;
+* :class:`StructMatrix` -- TPZElastoPlasticIntPointsStructMatrix: builds the
+  integrator data, uploads it and evaluates residuals on the GPU;
+* :class:`DeviceContext` -- the raw device context (``ipf_ctx``): residual,
+  Multiply / ComputeSigma / MultiplyTranspose steps and state access on
+  device pointers (e.g. ``torch.Tensor.data_ptr()``).
+
+There is no CPU fallback: if the library is missing or no GPU is present the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["lib", "library_path", "CompMesh", "StructMatrix", "DeviceContext", "IPFError", "build"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libintpointsfem.so")
+
+IPF_OK = 0
+BRANCH_NAMES = ("elastic", "main", "right_edge", "left_edge", "apex")
+
+
+class IPFError(RuntimeError):
+    pass
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def build(verbose=False):
+    from . import build as _b
+    return _b.build(verbose=verbose)
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_int64)
+_bp = ctypes.POINTER(ctypes.c_byte)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_int = ctypes.c_int
+
+_lib = None
+
+
+def lib():
+    """Loads libintpointsfem.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise IPFError(f"{_LIB_PATH} is missing: build it with `python -m intpointsfem_amd.build` "
+                       "(there is no CPU fallback)")
+    L = ctypes.CDLL(_LIB_PATH)
+    sig = {
+        # device layer
+        "ipf_ctx_create": (_int, [_int, ctypes.POINTER(_vp)]),
+        "ipf_ctx_destroy": (_int, [_vp]),
+        "ipf_last_error": (ctypes.c_char_p, [_vp]),
+        "ipf_global_last_error": (ctypes.c_char_p, []),
+        "ipf_set_stream": (_int, [_vp, _vp]),
+        "ipf_get_stream": (_vp, [_vp]),
+        "ipf_synchronize": (_int, [_vp]),
+        "ipf_transfer_data": (_int, [_vp, _i64, _ip, _ip, _lp, _ip, _ip, _dp, _ip, _ip, _i64, _i64, _dp]),
+        "ipf_set_material": (_int, [_vp, _d, _d, _d, _d, _d]),
+        "ipf_set_rhs_linear": (_int, [_vp, _dp]),
+        "ipf_set_update_mem": (_int, [_vp, _int]),
+        "ipf_info": (_int, [_vp, _lp]),
+        "ipf_residual": (_int, [_vp, _vp, _vp]),
+        "ipf_residual_host": (_int, [_vp, _dp, _dp]),
+        "ipf_multiply_vector": (_int, [_vp, _vp, _vp, _int]),
+        "ipf_multiply": (_int, [_vp, _vp, _vp]),
+        "ipf_compute_sigma": (_int, [_vp, _vp, _vp]),
+        "ipf_compute_sigma_dep": (_int, [_vp, _vp, _vp, _vp]),
+        "ipf_multiply_transpose": (_int, [_vp, _vp, _vp]),
+        "ipf_get_state": (_int, [_vp, _dp, _dp, _ip, _ip]),
+        "ipf_set_state": (_int, [_vp, _dp, _dp]),
+        # front-end
+        "ipf_cmesh_quarter_annulus": (_int, [_int, _int, _int, _d, _d, _d, ctypes.POINTER(_vp)]),
+        "ipf_cmesh_destroy": (_int, [_vp]),
+        "ipf_cmesh_set_material": (_int, [_vp, _d, _d, _d, _d, _d]),
+        "ipf_cmesh_set_normal_stress": (_int, [_vp, _d, _d]),
+        "ipf_cmesh_info": (_int, [_vp, _lp]),
+        "ipf_cmesh_material": (_int, [_vp, _dp]),
+        "ipf_cmesh_nodes": (_int, [_vp, _dp]),
+        "ipf_cmesh_connectivity": (_int, [_vp, _lp, _lp]),
+        "ipf_cmesh_dirichlet_mask": (_int, [_vp, _bp]),
+        "ipf_cmesh_set_update_mem": (_int, [_vp, _int]),
+        "ipf_sm_create": (_int, [_vp, _int, ctypes.POINTER(_vp)]),
+        "ipf_sm_destroy": (_int, [_vp]),
+        "ipf_sm_context": (_vp, [_vp]),
+        "ipf_sm_sizes": (_int, [_vp, _lp]),
+        "ipf_sm_export_blocks": (_int, [_vp, _ip, _ip, _lp, _ip, _ip, _dp]),
+        "ipf_sm_export_indexes": (_int, [_vp, _ip, _ip, _ip, _lp, _dp, _dp]),
+        "ipf_sm_assemble_rhs": (_int, [_vp, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names the C ABI headers declare (checked against the library by the tests)."""
+    import re
+    names = []
+    inc = os.path.join(os.path.dirname(_HERE), "include")
+    for h in ("intpointsfem.h", "intpointsfem_frontend.h"):
+        with open(os.path.join(inc, h)) as f:
+            text = f.read()
+        names += re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*(ipf_[a-z0-9_]+)\s*\(", text, re.M)
+    return sorted(set(names))
+
+
+def _err(ctx=None):
+    L = lib()
+    msg = L.ipf_last_error(ctx) if ctx else L.ipf_global_last_error()
+    return (msg or b"").decode(errors="replace")
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != IPF_OK:
+        raise IPFError(f"{what} failed ({rc}): {_err(ctx)}")
+
+
+def _p(a, t):
+    if a is None:
+        return ctypes.cast(None, t)
+    return a.ctypes.data_as(t)
+
+
+def _ptr(x):
+    """Device pointer of a torch tensor or an int."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+class CompMesh:
+    """Structured quarter annulus with Q_p Lagrange elements (TPZCompMesh stand-in)."""
+
+    def __init__(self, nr, ntheta, porder, r_in=0.1, r_out=4.0, ratio=1.0):
+        L = lib()
+        h = _vp()
+        _check(L.ipf_cmesh_quarter_annulus(nr, ntheta, porder, r_in, r_out, ratio, ctypes.byref(h)),
+               what="ipf_cmesh_quarter_annulus")
+        self._h = h
+        self.nr, self.ntheta, self.porder = nr, ntheta, porder
+        self.r_in, self.r_out, self.ratio = r_in, r_out, ratio
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.ipf_cmesh_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_material(self, E, nu, cohesion, phi, psi):
+        _check(lib().ipf_cmesh_set_material(self._h, E, nu, cohesion, phi, psi), what="ipf_cmesh_set_material")
+
+    def set_normal_stress(self, inner, outer=0.0):
+        _check(lib().ipf_cmesh_set_normal_stress(self._h, inner, outer), what="ipf_cmesh_set_normal_stress")
+
+    def set_update_mem(self, flag):
+        _check(lib().ipf_cmesh_set_update_mem(self._h, 1 if flag else 0), what="ipf_cmesh_set_update_mem")
+
+    def info(self):
+        out = np.zeros(8, dtype=np.int64)
+        _check(lib().ipf_cmesh_info(self._h, _p(out, _lp)), what="ipf_cmesh_info")
+        keys = ("nelem", "nnodes", "neq", "porder", "nr", "ntheta", "npts_per_element", "nshape")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def material(self):
+        out = np.zeros(5)
+        _check(lib().ipf_cmesh_material(self._h, _p(out, _dp)), what="ipf_cmesh_material")
+        return dict(zip(("lambda", "mu", "phi", "psi", "cohesion"), (float(v) for v in out)))
+
+    def nodes(self):
+        n = self.info()["nnodes"]
+        xy = np.zeros(2 * n)
+        _check(lib().ipf_cmesh_nodes(self._h, _p(xy, _dp)), what="ipf_cmesh_nodes")
+        return xy.reshape(n, 2)
+
+    def connectivity(self):
+        inf = self.info()
+        ptr = np.zeros(inf["nelem"] + 1, dtype=np.int64)
+        conn = np.zeros(inf["nelem"] * inf["nshape"], dtype=np.int64)
+        _check(lib().ipf_cmesh_connectivity(self._h, _p(ptr, _lp), _p(conn, _lp)), what="ipf_cmesh_connectivity")
+        return ptr, conn
+
+    def dirichlet_mask(self):
+        m = np.zeros(self.info()["neq"], dtype=np.int8)
+        _check(lib().ipf_cmesh_dirichlet_mask(self._h, _p(m, _bp)), what="ipf_cmesh_dirichlet_mask")
+        return m.astype(bool)
+
+
+class DeviceContext:
+    """Non-owning view of an ``ipf_ctx`` (owned by a StructMatrix) or an owning one."""
+
+    def __init__(self, handle=None, device=0, owner=None):
+        L = lib()
+        self._own = handle is None
+        if handle is None:
+            h = _vp()
+            _check(L.ipf_ctx_create(device, ctypes.byref(h)), what="ipf_ctx_create")
+            handle = h
+        self._h = handle if isinstance(handle, _vp) else _vp(handle)
+        self._owner = owner
+
+    def __del__(self):
+        if getattr(self, "_own", False) and self._h and _lib is not None:
+            _lib.ipf_ctx_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _c(self, rc, what):
+        _check(rc, self._h, what)
+
+    def transfer_data(self, row_sizes, col_sizes, matrix_position, row_first, col_first, storage, dof_indexes,
+                      color_indexes, ncolor, neq, weight):
+        a = [np.ascontiguousarray(x, dtype=t) for x, t in
+             ((row_sizes, np.int32), (col_sizes, np.int32), (matrix_position, np.int64), (row_first, np.int32),
+              (col_first, np.int32), (storage, np.float64), (dof_indexes, np.int32), (color_indexes, np.int32),
+              (weight, np.float64))]
+        self._c(lib().ipf_transfer_data(self._h, a[0].size, _p(a[0], _ip), _p(a[1], _ip), _p(a[2], _lp),
+                                        _p(a[3], _ip), _p(a[4], _ip), _p(a[5], _dp), _p(a[6], _ip),
+                                        _p(a[7], _ip), int(ncolor), int(neq), _p(a[8], _dp)), "ipf_transfer_data")
+
+    def set_material(self, lam, mu, phi, psi, cohesion):
+        self._c(lib().ipf_set_material(self._h, lam, mu, phi, psi, cohesion), "ipf_set_material")
+
+    def set_rhs_linear(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        self._c(lib().ipf_set_rhs_linear(self._h, _p(f, _dp)), "ipf_set_rhs_linear")
+
+    def set_update_mem(self, flag):
+        self._c(lib().ipf_set_update_mem(self._h, 1 if flag else 0), "ipf_set_update_mem")
+
+    def set_stream(self, stream_ptr):
+        self._c(lib().ipf_set_stream(self._h, stream_ptr), "ipf_set_stream")
+
+    def stream(self):
+        return lib().ipf_get_stream(self._h)
+
+    def synchronize(self):
+        self._c(lib().ipf_synchronize(self._h), "ipf_synchronize")
+
+    def info(self):
+        out = np.zeros(8, dtype=np.int64)
+        self._c(lib().ipf_info(self._h, _p(out, _lp)), "ipf_info")
+        keys = ("nblocks", "npts", "rows", "cols", "neq", "ncolor", "nstorage", "device_bytes")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    # ---- compute on device pointers (torch tensors or raw ints)
+    def residual(self, u, r):
+        self._c(lib().ipf_residual(self._h, _ptr(u), _ptr(r)), "ipf_residual")
+
+    def residual_host(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.zeros_like(u)
+        self._c(lib().ipf_residual_host(self._h, _p(u, _dp), _p(r, _dp)), "ipf_residual_host")
+        return r
+
+    def multiply_vector(self, x, y, opt):
+        self._c(lib().ipf_multiply_vector(self._h, _ptr(x), _ptr(y), int(opt)), "ipf_multiply_vector")
+
+    def multiply(self, u, strain):
+        self._c(lib().ipf_multiply(self._h, _ptr(u), _ptr(strain)), "ipf_multiply")
+
+    def compute_sigma(self, strain, wsigma):
+        self._c(lib().ipf_compute_sigma(self._h, _ptr(strain), _ptr(wsigma)), "ipf_compute_sigma")
+
+    def compute_sigma_dep(self, strain, wsigma, wdep):
+        self._c(lib().ipf_compute_sigma_dep(self._h, _ptr(strain), _ptr(wsigma), _ptr(wdep)),
+                "ipf_compute_sigma_dep")
+
+    def multiply_transpose(self, wsigma, res):
+        self._c(lib().ipf_multiply_transpose(self._h, _ptr(wsigma), _ptr(res)), "ipf_multiply_transpose")
+
+    def get_state(self):
+        n = self.info()["npts"]
+        ep = np.zeros(6 * n)
+        al = np.zeros(n)
+        mt = np.zeros(n, dtype=np.int32)
+        br = np.zeros(n, dtype=np.int32)
+        self._c(lib().ipf_get_state(self._h, _p(ep, _dp), _p(al, _dp), _p(mt, _ip), _p(br, _ip)), "ipf_get_state")
+        return ep, al, mt, br
+
+    def set_state(self, plastic_strain6, alpha=None):
+        ep = np.ascontiguousarray(plastic_strain6, dtype=np.float64).reshape(-1)
+        al = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
+        self._c(lib().ipf_set_state(self._h, _p(ep, _dp), _p(al, _dp)), "ipf_set_state")
+
+
+class StructMatrix:
+    """TPZElastoPlasticIntPointsStructMatrix over a CompMesh.
+
+    ``device >= 0`` uploads to that GPU; ``device < 0`` builds the host data
+    only (exports work without a GPU)."""
+
+    def __init__(self, mesh, device=0):
+        h = _vp()
+        _check(lib().ipf_sm_create(mesh.handle, device, ctypes.byref(h)), what="ipf_sm_create")
+        self._h = h
+        self.mesh = mesh
+        self.device = device
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.ipf_sm_destroy(h)
+            self._h = None
+
+    def sizes(self):
+        out = np.zeros(8, dtype=np.int64)
+        _check(lib().ipf_sm_sizes(self._h, _p(out, _lp)), what="ipf_sm_sizes")
+        keys = ("nblocks", "npts", "rows", "cols", "neq", "ncolor", "nstorage")
+        return dict(zip(keys, (int(v) for v in out[:7])))
+
+    def context(self):
+        h = lib().ipf_sm_context(self._h)
+        if not h:
+            raise IPFError("structural matrix has no device context (created with device < 0)")
+        return DeviceContext(handle=h, owner=self)
+
+    def export_blocks(self):
+        s = self.sizes()
+        nb = s["nblocks"]
+        rs = np.zeros(nb, np.int32)
+        cs = np.zeros(nb, np.int32)
+        mp = np.zeros(nb + 1, np.int64)
+        rf = np.zeros(nb + 1, np.int32)
+        cf = np.zeros(nb + 1, np.int32)
+        st = np.zeros(s["nstorage"])
+        _check(lib().ipf_sm_export_blocks(self._h, _p(rs, _ip), _p(cs, _ip), _p(mp, _lp), _p(rf, _ip), _p(cf, _ip),
+                                          _p(st, _dp)), what="ipf_sm_export_blocks")
+        return dict(row_sizes=rs, col_sizes=cs, matrix_position=mp, row_first=rf, col_first=cf, storage=st)
+
+    def export_indexes(self):
+        s = self.sizes()
+        dof = np.zeros(s["cols"], np.int32)
+        col = np.zeros(s["cols"], np.int32)
+        elc = np.zeros(s["nblocks"], np.int32)
+        fci = np.zeros(s["ncolor"] + 1, np.int64)
+        w = np.zeros(s["npts"])
+        fl = np.zeros(s["neq"])
+        _check(lib().ipf_sm_export_indexes(self._h, _p(dof, _ip), _p(col, _ip), _p(elc, _ip), _p(fci, _lp),
+                                           _p(w, _dp), _p(fl, _dp)), what="ipf_sm_export_indexes")
+        return dict(dof_indexes=dof, color_indexes=col, el_color_index=elc, first_color_index=fci, weight=w,
+                    rhs_linear=fl)
+
+    def assemble_rhs(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.zeros_like(u)
+        _check(lib().ipf_sm_assemble_rhs(self._h, _p(u, _dp), _p(r, _dp)), what="ipf_sm_assemble_rhs")
+        return r

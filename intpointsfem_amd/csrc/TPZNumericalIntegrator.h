@@ -1,0 +1,69 @@
+// TPZNumericalIntegrator.h -- gather / B.u / return map / B^T(w sigma) /
+// colored assembly driver; API of TPZNumericalIntegrator.h:22-111 of the
+// reference.  Setup runs on the host from the mesh stand-in, the compute
+// methods run on the GPU through the C ABI of include/intpointsfem.h.
+#pragma once
+
+#include <memory>
+
+#include "TPZCompMesh.h"
+#include "TPZConstitutiveLawProcessor.h"
+#include "TPZIrregularBlocksMatrix.h"
+#include "tpz_types.h"
+
+class TPZNumericalIntegrator {
+public:
+    TPZNumericalIntegrator() = default;
+
+    // TPZNumericalIntegrator.cpp:25-37 / :54-78 / :117-128 (device vectors)
+    void Multiply(TPZVecGPU<REAL>& coef, TPZVecGPU<REAL>& delta_strain);
+    void MultiplyTranspose(TPZVecGPU<REAL>& sigma, TPZVecGPU<REAL>& res);
+    void ResidualIntegration(TPZFMatrix<REAL>& solution, TPZVecGPU<REAL>& rhs);
+    // Host-matrix overloads (TPZNumericalIntegrator.cpp:40-51, :81-114)
+    void Multiply(TPZFMatrix<REAL>& coef, TPZFMatrix<REAL>& delta_strain);
+    void MultiplyTranspose(TPZFMatrix<REAL>& sigma, TPZFMatrix<REAL>& res);
+    void ResidualIntegration(TPZFMatrix<REAL>& solution, TPZFMatrix<REAL>& rhs);
+
+    // K_e = sum_ip B_ip^T Dep_ip B_ip (host, TPZNumericalIntegrator.cpp:236-270)
+    void ComputeTangentMatrix(int64_t iel, TPZFMatrix<REAL>& Dep, TPZFMatrix<REAL>& K);
+
+    // Setup (TPZNumericalIntegrator.cpp:272-536)
+    void SetElementIndexes(TPZVec<int> element_indexes) { fElementIndex = element_indexes; }
+    void SetUpIrregularBlocksData(TPZCompMesh* cmesh);
+    int StressRateVectorSize(int dim);
+    void SetUpIndexes(TPZCompMesh* cmesh);
+    void SetUpColoredIndexes(TPZCompMesh* cmesh);
+    void FillLIndexes(TPZVec<int64_t>& IA, TPZVec<int64_t>& JA);
+    int64_t me(TPZVec<int64_t>& IA, TPZVec<int64_t>& JA, int64_t& i_dest, int64_t& j_dest);
+    bool isBuilt() { return fBlockMatrix.Rows() != 0; }
+
+    // Creates the device context (if needed) and uploads blocks, indexes,
+    // weights and material (TPZNumericalIntegrator.cpp:218-233).
+    void TransferDataToGPU(int device);
+    std::shared_ptr<ipf_ctx> Context() { return fCtx; }
+
+    TPZIrregularBlocksMatrix& BlockMatrix() { return fBlockMatrix; }
+    TPZConstitutiveLawProcessor& ConstitutiveLawProcessor() { return fConstitutiveLawProcessor; }
+    int64_t NColor() const { return fNColor; }
+    TPZVec<int>& DoFIndexes() { return fDoFIndexes; }
+    TPZVec<int>& ColorIndexes() { return fColorIndexes; }
+    TPZVec<int>& ElColorIndex() { return fElColorIndex; }
+    TPZVec<int64_t>& FirstColorIndex() { return fFirstColorIndex; }
+    TPZVec<int64_t>& ColorLSequence() { return fColorLSequence; }
+    TPZVec<int64_t>& FirstColorLIndex() { return fFirstColorLIndex; }
+
+private:
+    TPZVec<int> fElementIndex;
+    TPZIrregularBlocksMatrix fBlockMatrix;
+    int64_t fNColor = -1;
+    int64_t fNEquations = 0;
+    TPZVec<int> fDoFIndexes;
+    TPZVec<int> fColorIndexes;
+    TPZConstitutiveLawProcessor fConstitutiveLawProcessor;
+    TPZVec<int> fElColorIndex;
+    TPZVec<int64_t> fFirstColorIndex;
+    TPZVec<int64_t> fColorLSequence;
+    TPZVec<int64_t> fFirstColorLIndex;
+    TPZCompMesh* fMesh = nullptr;
+    std::shared_ptr<ipf_ctx> fCtx;
+};

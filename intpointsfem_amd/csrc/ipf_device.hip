@@ -1,0 +1,704 @@
+// ipf_device.hip -- device context behind the C ABI of include/intpointsfem.h.
+//
+// Owns every device buffer of one integrator instance (the reference spreads
+// them over TPZVecGPU members of TPZIrregularBlocksMatrix, TPZNumericalIntegrator
+// and TPZConstitutiveLawProcessor) and builds the tiled HBM layout described
+// in ipf_kernels.h from the reference's IrregularBlocks arrays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/intpointsfem.h"
+#include "ipf_kernels.h"
+
+namespace ipf {
+
+MatConst MatConst::make(double lambda, double mu, double mc_phi, double mc_psi, double c) {
+    MatConst m{};
+    const double G = mu;
+    const double K = lambda + 2 * mu / 3;  // TPZConstitutiveLawProcessor.cpp:158
+    m.lambda = lambda;
+    m.mu = mu;
+    m.G = G;
+    m.K = K;
+    m.lam2mu = lambda + 2 * mu;
+    m.cs_a = (lambda + mu) / (3 * lambda * mu + 2 * mu * mu);
+    m.cs_b = lambda / (6 * lambda * mu + 4 * mu * mu);
+    m.inv_mu = 1 / mu;
+    const double sinphi = sin(mc_phi);
+    const double cosphi = cos(mc_phi);
+    const double sinpsi = sin(mc_psi);
+    m.sinphi = sinphi;
+    m.cosphi = cosphi;
+    m.sinpsi = sinpsi;
+    m.phi_c = 2. * c * cosphi;
+    m.edge_c = 2. * cosphi * c;
+    m.constA = 4. * G * (1. + sinphi * sinpsi / 3.) + 4. * K * sinphi * sinpsi;
+    m.m0 = (2. * G * (1 + sinpsi / 3.) + 2. * K * sinpsi);
+    m.m1 = (4. * G / 3. - K * 2.) * sinpsi;
+    m.m2 = (2. * G * (1 - sinpsi / 3.) - 2. * K * sinpsi);
+    // edge Jacobians (constant for H = 0), FunctionsSigmaProjection.h:117-136, :217-238
+    auto edge = [](double ab0, double ab1, double* ji) {
+        const double j00 = -ab0, j10 = -ab1, j01 = -ab1, j11 = -ab0;
+        const double det = j00 * j11 - j01 * j10;
+        ji[0] = j11 / det;   // [0][0]
+        ji[1] = -j01 / det;  // [0][1]
+        ji[2] = -j10 / det;  // [1][0]
+        ji[3] = j00 / det;   // [1][1]
+    };
+    double ji[4];
+    m.r_ab0 = 4. * G * (1 + sinphi * sinpsi / 3.) + 4. * K * sinphi * sinpsi;
+    m.r_ab1 = 2. * G * (1. + sinphi + sinpsi - sinphi * sinpsi / 3.) + 4. * K * sinphi * sinpsi;
+    edge(m.r_ab0, m.r_ab1, ji);
+    m.r_ji00 = ji[0]; m.r_ji01 = ji[1]; m.r_ji10 = ji[2]; m.r_ji11 = ji[3];
+    m.l_ab0 = 4. * G * (1 + sinphi * sinpsi / 3.) + 4. * K * sinphi * sinpsi;
+    m.l_ab1 = 2. * G * (1. - sinphi - sinpsi - sinphi * sinpsi / 3.) + 4. * K * sinphi * sinpsi;
+    edge(m.l_ab0, m.l_ab1, ji);
+    m.l_ji00 = ji[0]; m.l_ji01 = ji[1]; m.l_ji10 = ji[2]; m.l_ji11 = ji[3];
+    m.omsp = 1 - sin(mc_psi);
+    m.opsp = 1 + sin(mc_psi);
+    const double cotphi = 1. / tan(mc_phi);
+    m.c_cot = c * cotphi;
+    m.apex_alpha = cos(mc_phi) / sin(mc_psi);
+    // principal-space tangents, FunctionsSigmaProjection.h:68-88, :163-188, :262-287
+    {
+        const double sin_phi = sinphi, sin_psi = sinpsi;
+        double* t = m.tan_plane;
+        const double den = 6.0 * G + 2.0 * (G + 3.0 * K) * sin_phi * sin_psi;
+        t[0] = (sin_phi - 1.0) * (-3.0 * G + (G + 3.0 * K) * sin_psi) / den;
+        t[3] = (2.0 * G - 3.0 * K) * (sin_phi + 1.0) * sin_psi / den;
+        t[6] = -(sin_phi + 1.0) * (-3.0 * G + (G + 3.0 * K) * sin_psi) / den;
+        t[1] = 0.0;
+        t[4] = 1.0;
+        t[7] = 0.0;
+        t[2] = -(sin_phi - 1.0) * (3.0 * G + (G + 3.0 * K) * sin_psi) / den;
+        t[5] = (2.0 * G - 3.0 * K) * (sin_phi - 1.0) * sin_psi / den;
+        t[8] = (sin_phi + 1.0) * (3.0 * G + (G + 3.0 * K) * sin_psi) / den;
+    }
+    {
+        const double sin_phi = sinphi, sin_psi = sinpsi;
+        double* t = m.tan_right;
+        const double a = 4.0 * G * (1.0 + (1.0 / 3.0) * sin_phi * sin_psi) + 4.0 * K * sin_phi * sin_psi;
+        const double b = 2.0 * G * (1.0 + sin_phi + sin_psi - (1.0 / 3.0) * sin_phi * sin_psi) + 4.0 * K * sin_phi * sin_psi;
+        t[0] = (3.0 * a + 3.0 * b - 4.0 * (1.0 + sin_phi) * (3.0 * K * sin_psi + G * (3.0 + sin_psi))) / (3. * (a + b));
+        t[3] = (2.0 * (1.0 + sin_phi) * (-6.0 * K * sin_psi + G * (3.0 + sin_psi))) / (3. * (a + b));
+        t[6] = (2.0 * (1.0 + sin_phi) * (-6.0 * K * sin_psi + G * (3.0 + sin_psi))) / (3. * (a + b));
+        t[1] = (-2 * (-1 + sin_phi) * (3 * K * sin_psi + G * (3 + sin_psi))) / (3. * (a + b));
+        t[4] = (-3 * b * b + 3 * a * (a + 2 * G * (-1 + sin_phi)) -
+                2 * (a * G + 2 * b * G + 3 * a * K - 3 * b * K) * (-1 + sin_phi) * sin_psi) / (3. * (a - b) * (a + b));
+        t[7] = (2 * (-1 + sin_phi) * (b * G * (-3 + sin_psi) + a * (2 * G - 3 * K) * sin_psi + 3 * b * K * sin_psi)) /
+               (3. * (a * a - b * b));
+        t[2] = (-2 * (-1 + sin_phi) * (3 * K * sin_psi + G * (3 + sin_psi))) / (3. * (a + b));
+        t[5] = (2 * (-1 + sin_phi) * (b * G * (-3 + sin_psi) + a * (2 * G - 3 * K) * sin_psi + 3 * b * K * sin_psi)) /
+               (3. * (a * a - b * b));
+        t[8] = (-3 * b * b + 3 * a * (a + 2 * G * (-1 + sin_phi)) -
+                2 * (a * G + 2 * b * G + 3 * a * K - 3 * b * K) * (-1 + sin_phi) * sin_psi) / (3. * (a - b) * (a + b));
+    }
+    {
+        const double sin_phi = sinphi, sin_psi = sinpsi;
+        double* t = m.tan_left;
+        const double a = 4.0 * G * (1.0 + (1.0 / 3.0) * sin_phi * sin_psi) + 4.0 * K * sin_phi * sin_psi;
+        const double b = 2.0 * G * (1.0 - sin_phi - sin_psi - (1.0 / 3.0) * sin_phi * sin_psi) + 4.0 * K * sin_phi * sin_psi;
+        t[0] = (-3 * b * b + 3 * a * (a - 2 * G * (1 + sin_phi)) -
+                2 * (a * G + 2 * b * G + 3 * a * K - 3 * b * K) * (1 + sin_phi) * sin_psi) / (3. * (a - b) * (a + b));
+        t[3] = (2 * (1 + sin_phi) * (a * (2 * G - 3 * K) * sin_psi + 3 * b * K * sin_psi + b * G * (3 + sin_psi))) /
+               (3. * (a * a - b * b));
+        t[6] = (-2 * (1 + sin_phi) * (G * (-3 + sin_psi) + 3 * K * sin_psi)) / (3. * (a + b));
+        t[1] = (2 * (1 + sin_phi) * (a * (2 * G - 3 * K) * sin_psi + 3 * b * K * sin_psi + b * G * (3 + sin_psi))) /
+               (3. * (a * a - b * b));
+        t[4] = (-3 * b * b + 3 * a * (a - 2 * G * (1 + sin_phi)) -
+                2 * (a * G + 2 * b * G + 3 * a * K - 3 * b * K) * (1 + sin_phi) * sin_psi) / (3. * (a - b) * (a + b));
+        t[7] = (-2 * (1 + sin_phi) * (G * (-3 + sin_psi) + 3 * K * sin_psi)) / (3. * (a + b));
+        t[2] = (2 * (-1 + sin_phi) * (G * (-3 + sin_psi) - 6 * K * sin_psi)) / (3. * (a + b));
+        t[5] = (2 * (-1 + sin_phi) * (G * (-3 + sin_psi) - 6 * K * sin_psi)) / (3. * (a + b));
+        t[8] = (3 * a + 3 * b - 4 * (-1 + sin_phi) * (G * (-3 + sin_psi) + 3 * K * sin_psi)) / (3. * (a + b));
+    }
+    return m;
+}
+
+}  // namespace ipf
+
+using namespace ipf;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Group {
+    int color;
+    int shape;
+    std::vector<int64_t> elems;  // reference block indexes, ascending
+    int64_t ntiles = 0;
+    int64_t pt0 = 0;
+    int64_t b_off = 0;
+    int64_t dof_off = 0;
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    int64_t n = 0;
+};
+
+}  // namespace
+
+struct ipf_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // sizes
+    int64_t nblocks = 0, npts = 0, rows = 0, cols = 0, neq = 0, ncolor = 0, nstorage = 0, npad = 0;
+    bool have_data = false, have_material = false;
+    bool update_mem = false;
+    MatConst mat{};
+
+    std::vector<Group> groups;
+
+    // device buffers
+    double* dB = nullptr;
+    int* dDof = nullptr;
+    int* dCol = nullptr;        // column index col_first[e] + j, same layout as dDof
+    double* dW = nullptr;
+    double* dEp[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* dAlpha = nullptr;
+    signed char* dMtype = nullptr;
+    signed char* dBranch = nullptr;
+    int64_t* dOrig = nullptr;   // orig point of each new slot (-1 for padding)
+    double* dFlin = nullptr;
+    double* dU = nullptr;       // staging for host-pointer calls
+    double* dR = nullptr;
+    int64_t bytes = 0;
+};
+
+namespace {
+
+int fail(ipf_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    g_last_error = buf;
+    return code;
+}
+
+#define IPF_HIP(ctx, expr)                                                                       \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail((ctx), IPF_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));         \
+    } while (0)
+
+template <typename T>
+int dev_alloc(ipf_ctx* ctx, T** p, int64_t n) {
+    *p = nullptr;
+    if (n <= 0) return IPF_OK;
+    IPF_HIP(ctx, hipMalloc((void**)p, sizeof(T) * (size_t)n));
+    ctx->bytes += (int64_t)sizeof(T) * n;
+    return IPF_OK;
+}
+
+template <typename T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void release_data(ipf_ctx* ctx) {
+    dev_free(ctx->dB);
+    dev_free(ctx->dDof);
+    dev_free(ctx->dCol);
+    dev_free(ctx->dW);
+    for (auto& p : ctx->dEp) dev_free(p);
+    dev_free(ctx->dAlpha);
+    dev_free(ctx->dMtype);
+    dev_free(ctx->dBranch);
+    dev_free(ctx->dOrig);
+    dev_free(ctx->dFlin);
+    dev_free(ctx->dU);
+    dev_free(ctx->dR);
+    ctx->groups.clear();
+    ctx->have_data = false;
+    ctx->bytes = 0;
+}
+
+int shape_of(int nip, int ce) {
+    for (int s = 0; s < SHAPE_COUNT; s++)
+        if (shape_nip(s) == nip && shape_ce(s) == ce) return s;
+    return -1;
+}
+
+GroupDev group_dev(const ipf_ctx* ctx, const Group& g, bool columns = false) {
+    GroupDev d;
+    d.B = ctx->dB + g.b_off;
+    d.dof = (columns ? ctx->dCol : ctx->dDof) + g.dof_off;
+    d.nelem = (int64_t)g.elems.size();
+    d.ntiles = g.ntiles;
+    d.pt0 = g.pt0;
+    return d;
+}
+
+PointState point_state(ipf_ctx* ctx) {
+    PointState s;
+    s.w = ctx->dW;
+    s.ep0 = ctx->dEp[0];
+    s.ep1 = ctx->dEp[1];
+    s.ep2 = ctx->dEp[2];
+    s.ep3 = ctx->dEp[3];
+    s.alpha = ctx->dAlpha;
+    s.mtype = ctx->dMtype;
+    s.branch = ctx->dBranch;
+    return s;
+}
+
+int check_ready(ipf_ctx* ctx, bool need_material) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    if (!ctx->have_data) return fail(ctx, IPF_ESTATE, "integrator data not transferred (call ipf_transfer_data)");
+    if (need_material && !ctx->have_material) return fail(ctx, IPF_ESTATE, "material not set (call ipf_set_material)");
+    return IPF_OK;
+}
+
+int launch_status(ipf_ctx* ctx, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, IPF_EHIP, "%s launch failed: %s", what, hipGetErrorString(e));
+    return IPF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ipf_last_error(const ipf_ctx* ctx) { return ctx ? ctx->err.c_str() : g_last_error.c_str(); }
+const char* ipf_global_last_error(void) { return g_last_error.c_str(); }
+int ipf_internal_set_error(const char* msg) {
+    g_last_error = msg ? msg : "";
+    return IPF_OK;
+}
+
+int ipf_ctx_create(int device, ipf_ctx** out) {
+    if (!out) return fail(nullptr, IPF_EINVAL, "ipf_ctx_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0)
+        return fail(nullptr, IPF_EHIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= ndev) return fail(nullptr, IPF_EINVAL, "device %d out of range [0,%d)", device, ndev);
+    ipf_ctx* ctx = new (std::nothrow) ipf_ctx();
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "out of host memory");
+    ctx->device = device;
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreate(&ctx->own_stream);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, IPF_EHIP, "context setup failed: %s", hipGetErrorString(e));
+    }
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return IPF_OK;
+}
+
+int ipf_ctx_destroy(ipf_ctx* ctx) {
+    if (!ctx) return IPF_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    release_data(ctx);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return IPF_OK;
+}
+
+int ipf_set_stream(ipf_ctx* ctx, void* s) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    return IPF_OK;
+}
+
+void* ipf_get_stream(const ipf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int ipf_synchronize(ipf_ctx* ctx) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    IPF_HIP(ctx, hipGetLastError());
+    return IPF_OK;
+}
+
+int ipf_info(const ipf_ctx* ctx, int64_t out[8]) {
+    if (!ctx || !out) return fail(nullptr, IPF_EINVAL, "ipf_info: null argument");
+    out[0] = ctx->nblocks;
+    out[1] = ctx->npts;
+    out[2] = ctx->rows;
+    out[3] = ctx->cols;
+    out[4] = ctx->neq;
+    out[5] = ctx->ncolor;
+    out[6] = ctx->nstorage;
+    out[7] = ctx->bytes;
+    return IPF_OK;
+}
+
+int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const int* col_sizes,
+                      const int64_t* matrix_position, const int* row_first, const int* col_first,
+                      const double* storage, const int* dof_indexes, const int* color_indexes, int64_t ncolor,
+                      int64_t neq, const double* weight) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    if (nblocks < 0 || ncolor < 1 || neq < 1)
+        return fail(ctx, IPF_EINVAL, "bad sizes: nblocks=%lld ncolor=%lld neq=%lld", (long long)nblocks,
+                    (long long)ncolor, (long long)neq);
+    if (nblocks > 0 && (!row_sizes || !col_sizes || !matrix_position || !row_first || !col_first || !storage ||
+                        !dof_indexes || !color_indexes || !weight))
+        return fail(ctx, IPF_EINVAL, "ipf_transfer_data: null array");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    release_data(ctx);
+
+    // --- validate the IrregularBlocks prefix sums (TPZNumericalIntegrator.cpp:294-317)
+    if (nblocks > 0 && (row_first[0] != 0 || col_first[0] != 0 || matrix_position[0] != 0))
+        return fail(ctx, IPF_EINVAL, "first row/col/matrix position must be 0");
+    std::vector<int> elem_shape(nblocks);
+    for (int64_t e = 0; e < nblocks; e++) {
+        const int R = row_sizes[e], C = col_sizes[e];
+        if (R <= 0 || C <= 0 || R % 3 != 0 || C % 2 != 0)
+            return fail(ctx, IPF_EINVAL, "block %lld: row size %d / col size %d not 3*n_ip / 2*n_shape", (long long)e,
+                        R, C);
+        if ((int64_t)row_first[e + 1] != (int64_t)row_first[e] + R || (int64_t)col_first[e + 1] != (int64_t)col_first[e] + C ||
+            matrix_position[e + 1] != matrix_position[e] + (int64_t)R * C)
+            return fail(ctx, IPF_EINVAL, "block %lld: inconsistent first-index / position prefix sums", (long long)e);
+        const int s = shape_of(R / 3, C);
+        if (s < 0)
+            return fail(ctx, IPF_EUNSUPPORTED, "block %lld: shape (n_ip=%d, C_e=%d) has no kernel (supported: Q1/Q2/Q3)",
+                        (long long)e, R / 3, C);
+        elem_shape[e] = s;
+    }
+    const int64_t rows = nblocks ? row_first[nblocks] : 0;
+    const int64_t cols = nblocks ? col_first[nblocks] : 0;
+    const int64_t npts = rows / 3;
+
+    // --- element colors from fColorIndexes = fDoFIndexes + color*neq (TPZNumericalIntegrator.cpp:503)
+    std::vector<int> elem_color(nblocks);
+    for (int64_t e = 0; e < nblocks; e++) {
+        int color = -1;
+        for (int64_t i = col_first[e]; i < col_first[e + 1]; i++) {
+            const int64_t d = dof_indexes[i];
+            if (d < 0 || d >= neq)
+                return fail(ctx, IPF_EINVAL, "dof index %lld of block %lld out of range [0,%lld)", (long long)d,
+                            (long long)e, (long long)neq);
+            const int64_t diff = (int64_t)color_indexes[i] - d;
+            if (diff < 0 || diff % neq != 0 || diff / neq >= ncolor)
+                return fail(ctx, IPF_EINVAL, "color index %lld of block %lld is not dof + color*neq",
+                            (long long)color_indexes[i], (long long)e);
+            const int c = (int)(diff / neq);
+            if (color >= 0 && c != color)
+                return fail(ctx, IPF_EINVAL, "block %lld has entries of colors %d and %d", (long long)e, color, c);
+            color = c;
+        }
+        elem_color[e] = color;
+    }
+    // --- coloring invariant: no DoF twice within one color (race-free accumulation)
+    {
+        std::vector<int> owner(neq, -1);
+        for (int c = 0; c < ncolor; c++) {
+            for (int64_t e = 0; e < nblocks; e++) {
+                if (elem_color[e] != c) continue;
+                for (int64_t i = col_first[e]; i < col_first[e + 1]; i++) {
+                    const int d = dof_indexes[i];
+                    if (owner[d] == c)
+                        return fail(ctx, IPF_EINVAL, "color %d: DoF %d is shared by two elements (block %lld)", c, d,
+                                    (long long)e);
+                    owner[d] = c;
+                }
+            }
+        }
+    }
+
+    // --- groups: color-major, then shape, elements ascending
+    std::vector<Group> groups;
+    for (int c = 0; c < ncolor; c++)
+        for (int s = 0; s < SHAPE_COUNT; s++) {
+            Group g;
+            g.color = c;
+            g.shape = s;
+            for (int64_t e = 0; e < nblocks; e++)
+                if (elem_color[e] == c && elem_shape[e] == s) g.elems.push_back(e);
+            if (!g.elems.empty()) groups.push_back(std::move(g));
+        }
+    int64_t npad = 0, nb = 0, nd = 0;
+    for (auto& g : groups) {
+        const int ET = shape_et(g.shape), TP = shape_tp(g.shape), CE = shape_ce(g.shape);
+        g.ntiles = ((int64_t)g.elems.size() + ET - 1) / ET;
+        g.pt0 = npad;
+        g.b_off = nb;
+        g.dof_off = nd;
+        npad += g.ntiles * TP;
+        nb += g.ntiles * (int64_t)(3 * CE) * TP;
+        nd += (int64_t)g.elems.size() * CE;
+    }
+
+    // --- host staging in the tiled layout
+    std::vector<double> hB;
+    std::vector<int> hDof, hCol;
+    std::vector<double> hW(npad, 0.0);
+    std::vector<int64_t> hOrig(npad, -1);
+    try {
+        hB.assign(nb, 0.0);
+        hDof.resize(nd);
+        hCol.resize(nd);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, IPF_EINVAL, "out of host memory staging %lld B values", (long long)nb);
+    }
+    for (const auto& g : groups) {
+        const int NIP = shape_nip(g.shape), CE = shape_ce(g.shape), ET = shape_et(g.shape), TP = shape_tp(g.shape);
+        const int NB = 3 * CE;
+        const int64_t ne = (int64_t)g.elems.size();
+        for (int64_t k = 0; k < ne; k++) {
+            const int64_t e = g.elems[k];
+            std::memcpy(&hDof[g.dof_off + k * CE], &dof_indexes[col_first[e]], sizeof(int) * CE);
+            for (int j = 0; j < CE; j++) hCol[g.dof_off + k * CE + j] = col_first[e] + j;
+            const int64_t t = k / ET;
+            const int el = (int)(k % ET);
+            double* tileB = hB.data() + g.b_off + t * (int64_t)NB * TP;
+            const double* src = storage + matrix_position[e];
+            for (int ip = 0; ip < NIP; ip++) {
+                const int q = el * NIP + ip;
+                const int64_t P = g.pt0 + t * TP + q;
+                const int64_t o = row_first[e] / 3 + ip;
+                hOrig[P] = o;
+                hW[P] = weight[o];
+                for (int m = 0; m < NB; m++) {
+                    // m = k*CE + j <-> row 3*ip + k, column j of the row-major block
+                    tileB[((int64_t)(m >> 1) * TP + q) * 2 + (m & 1)] = src[(int64_t)(3 * ip) * CE + m];
+                }
+            }
+        }
+    }
+
+    // --- device buffers
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->dB, nb)) || (rc = dev_alloc(ctx, &ctx->dDof, nd)) ||
+        (rc = dev_alloc(ctx, &ctx->dCol, nd)) ||
+        (rc = dev_alloc(ctx, &ctx->dW, npad)) || (rc = dev_alloc(ctx, &ctx->dEp[0], npad)) ||
+        (rc = dev_alloc(ctx, &ctx->dEp[1], npad)) || (rc = dev_alloc(ctx, &ctx->dEp[2], npad)) ||
+        (rc = dev_alloc(ctx, &ctx->dEp[3], npad)) || (rc = dev_alloc(ctx, &ctx->dAlpha, npad)) ||
+        (rc = dev_alloc(ctx, &ctx->dMtype, npad)) || (rc = dev_alloc(ctx, &ctx->dBranch, npad)) ||
+        (rc = dev_alloc(ctx, &ctx->dOrig, npad)) || (rc = dev_alloc(ctx, &ctx->dFlin, neq)) ||
+        (rc = dev_alloc(ctx, &ctx->dU, neq)) || (rc = dev_alloc(ctx, &ctx->dR, neq))) {
+        release_data(ctx);
+        return rc;
+    }
+    auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (bytes == 0) return IPF_OK;
+        IPF_HIP(ctx, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return IPF_OK;
+    };
+    auto zero = [&](void* dst, size_t bytes) -> int {
+        if (bytes == 0) return IPF_OK;
+        IPF_HIP(ctx, hipMemset(dst, 0, bytes));
+        return IPF_OK;
+    };
+    if ((rc = up(ctx->dB, hB.data(), sizeof(double) * nb)) || (rc = up(ctx->dDof, hDof.data(), sizeof(int) * nd)) ||
+        (rc = up(ctx->dCol, hCol.data(), sizeof(int) * nd)) ||
+        (rc = up(ctx->dW, hW.data(), sizeof(double) * npad)) ||
+        (rc = up(ctx->dOrig, hOrig.data(), sizeof(int64_t) * npad)) ||
+        (rc = zero(ctx->dEp[0], sizeof(double) * npad)) || (rc = zero(ctx->dEp[1], sizeof(double) * npad)) ||
+        (rc = zero(ctx->dEp[2], sizeof(double) * npad)) || (rc = zero(ctx->dEp[3], sizeof(double) * npad)) ||
+        (rc = zero(ctx->dAlpha, sizeof(double) * npad)) || (rc = zero(ctx->dMtype, npad)) ||
+        (rc = zero(ctx->dBranch, npad)) || (rc = zero(ctx->dFlin, sizeof(double) * neq))) {
+        release_data(ctx);
+        return rc;
+    }
+    ctx->nblocks = nblocks;
+    ctx->npts = npts;
+    ctx->rows = rows;
+    ctx->cols = cols;
+    ctx->neq = neq;
+    ctx->ncolor = ncolor;
+    ctx->nstorage = nblocks ? matrix_position[nblocks] : 0;
+    ctx->npad = npad;
+    ctx->groups = std::move(groups);
+    ctx->have_data = true;
+    return IPF_OK;
+}
+
+int ipf_set_material(ipf_ctx* ctx, double lambda, double mu, double mc_phi, double mc_psi, double mc_cohesion) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    if (!(mu > 0) || !std::isfinite(lambda) || !std::isfinite(mc_phi) || !std::isfinite(mc_psi) ||
+        !std::isfinite(mc_cohesion))
+        return fail(ctx, IPF_EINVAL, "bad material: lambda=%g mu=%g phi=%g psi=%g c=%g", lambda, mu, mc_phi, mc_psi,
+                    mc_cohesion);
+    ctx->mat = MatConst::make(lambda, mu, mc_phi, mc_psi, mc_cohesion);
+    ctx->have_material = true;
+    return IPF_OK;
+}
+
+int ipf_set_rhs_linear(ipf_ctx* ctx, const double* f_lin_host) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!f_lin_host) return fail(ctx, IPF_EINVAL, "ipf_set_rhs_linear: NULL");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipMemcpyAsync(ctx->dFlin, f_lin_host, sizeof(double) * ctx->neq, hipMemcpyHostToDevice, ctx->stream));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return IPF_OK;
+}
+
+int ipf_set_update_mem(ipf_ctx* ctx, int update_mem) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    ctx->update_mem = update_mem != 0;
+    return IPF_OK;
+}
+
+int ipf_residual(ipf_ctx* ctx, const double* u_dev, double* r_dev) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!u_dev || !r_dev) return fail(ctx, IPF_EINVAL, "ipf_residual: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    // r = f_lin (TPZElastoPlasticIntPointsStructMatrix.cpp:147), then each color adds -B^T(w sigma)
+    launch_copy(ctx->neq, ctx->dFlin, r_dev, ctx->stream);
+    const PointState st = point_state(ctx);
+    for (const auto& g : ctx->groups)
+        launch_residual(g.shape, ctx->update_mem, group_dev(ctx, g), u_dev, r_dev, st, ctx->mat, ctx->stream);
+    return launch_status(ctx, "residual");
+}
+
+int ipf_residual_host(ipf_ctx* ctx, const double* u_host, double* r_host) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!u_host || !r_host) return fail(ctx, IPF_EINVAL, "ipf_residual_host: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipMemcpyAsync(ctx->dU, u_host, sizeof(double) * ctx->neq, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = ipf_residual(ctx, ctx->dU, ctx->dR))) return rc;
+    IPF_HIP(ctx, hipMemcpyAsync(r_host, ctx->dR, sizeof(double) * ctx->neq, hipMemcpyDeviceToHost, ctx->stream));
+    return ipf_synchronize(ctx);
+}
+
+int ipf_multiply(ipf_ctx* ctx, const double* u_dev, double* strain_dev) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!u_dev || !strain_dev) return fail(ctx, IPF_EINVAL, "ipf_multiply: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    for (const auto& g : ctx->groups)
+        launch_multiply(g.shape, group_dev(ctx, g), u_dev, strain_dev, ctx->dOrig, ctx->stream);
+    return launch_status(ctx, "multiply");
+}
+
+int ipf_multiply_vector(ipf_ctx* ctx, const double* x_dev, double* y_dev, int opt) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!x_dev || !y_dev) return fail(ctx, IPF_EINVAL, "ipf_multiply_vector: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if (opt == 0) {
+        for (const auto& g : ctx->groups)
+            launch_multiply(g.shape, group_dev(ctx, g, true), x_dev, y_dev, ctx->dOrig, ctx->stream);
+    } else {
+        launch_fill(ctx->cols, y_dev, 0.0, ctx->stream);
+        for (const auto& g : ctx->groups)
+            launch_transpose(g.shape, group_dev(ctx, g, true), x_dev, y_dev, ctx->dOrig, ctx->stream);
+    }
+    return launch_status(ctx, "multiply_vector");
+}
+
+int ipf_compute_sigma(ipf_ctx* ctx, const double* strain_dev, double* wsigma_dev) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!strain_dev || !wsigma_dev) return fail(ctx, IPF_EINVAL, "ipf_compute_sigma: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    launch_sigma(false, ctx->update_mem, ctx->npad, strain_dev, wsigma_dev, nullptr, ctx->dOrig, point_state(ctx),
+                 ctx->mat, ctx->stream);
+    return launch_status(ctx, "compute_sigma");
+}
+
+int ipf_compute_sigma_dep(ipf_ctx* ctx, const double* strain_dev, double* wsigma_dev, double* wdep_dev) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!strain_dev || !wsigma_dev || !wdep_dev) return fail(ctx, IPF_EINVAL, "ipf_compute_sigma_dep: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    launch_sigma(true, ctx->update_mem, ctx->npad, strain_dev, wsigma_dev, wdep_dev, ctx->dOrig, point_state(ctx),
+                 ctx->mat, ctx->stream);
+    return launch_status(ctx, "compute_sigma_dep");
+}
+
+int ipf_multiply_transpose(ipf_ctx* ctx, const double* wsigma_dev, double* res_dev) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!wsigma_dev || !res_dev) return fail(ctx, IPF_EINVAL, "ipf_multiply_transpose: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    launch_fill(ctx->neq, res_dev, 0.0, ctx->stream);
+    for (const auto& g : ctx->groups)
+        launch_transpose(g.shape, group_dev(ctx, g), wsigma_dev, res_dev, ctx->dOrig, ctx->stream);
+    return launch_status(ctx, "multiply_transpose");
+}
+
+int ipf_get_state(ipf_ctx* ctx, double* plastic_strain_host, double* alpha_host, int* mtype_host, int* branch_host) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t n = ctx->npad;
+    std::vector<int64_t> orig(n);
+    IPF_HIP(ctx, hipMemcpy(orig.data(), ctx->dOrig, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+    if (plastic_strain_host) {
+        std::vector<double> ep(n);
+        const int comp[4] = {XX, XY, YY, ZZ};
+        for (int64_t o = 0; o < ctx->npts; o++) {
+            plastic_strain_host[6 * o + XZ] = 0.0;
+            plastic_strain_host[6 * o + YZ] = 0.0;
+        }
+        for (int c = 0; c < 4; c++) {
+            IPF_HIP(ctx, hipMemcpy(ep.data(), ctx->dEp[c], sizeof(double) * n, hipMemcpyDeviceToHost));
+            for (int64_t P = 0; P < n; P++)
+                if (orig[P] >= 0) plastic_strain_host[6 * orig[P] + comp[c]] = ep[P];
+        }
+    }
+    if (alpha_host) {
+        std::vector<double> a(n);
+        IPF_HIP(ctx, hipMemcpy(a.data(), ctx->dAlpha, sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (int64_t P = 0; P < n; P++)
+            if (orig[P] >= 0) alpha_host[orig[P]] = a[P];
+    }
+    std::vector<signed char> b(n);
+    if (mtype_host) {
+        IPF_HIP(ctx, hipMemcpy(b.data(), ctx->dMtype, n, hipMemcpyDeviceToHost));
+        for (int64_t P = 0; P < n; P++)
+            if (orig[P] >= 0) mtype_host[orig[P]] = b[P];
+    }
+    if (branch_host) {
+        IPF_HIP(ctx, hipMemcpy(b.data(), ctx->dBranch, n, hipMemcpyDeviceToHost));
+        for (int64_t P = 0; P < n; P++)
+            if (orig[P] >= 0) branch_host[orig[P]] = b[P];
+    }
+    return IPF_OK;
+}
+
+int ipf_set_state(ipf_ctx* ctx, const double* plastic_strain_host, const double* alpha_host) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!plastic_strain_host) return fail(ctx, IPF_EINVAL, "ipf_set_state: plastic strain is NULL");
+    for (int64_t o = 0; o < ctx->npts; o++)
+        if (plastic_strain_host[6 * o + XZ] != 0.0 || plastic_strain_host[6 * o + YZ] != 0.0)
+            return fail(ctx, IPF_EINVAL, "point %lld: plastic strain XZ/YZ must be zero in plane strain",
+                        (long long)o);
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t n = ctx->npad;
+    std::vector<int64_t> orig(n);
+    IPF_HIP(ctx, hipMemcpy(orig.data(), ctx->dOrig, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+    std::vector<double> v(n);
+    const int comp[4] = {XX, XY, YY, ZZ};
+    for (int c = 0; c < 4; c++) {
+        for (int64_t P = 0; P < n; P++) v[P] = orig[P] >= 0 ? plastic_strain_host[6 * orig[P] + comp[c]] : 0.0;
+        IPF_HIP(ctx, hipMemcpy(ctx->dEp[c], v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    for (int64_t P = 0; P < n; P++) v[P] = (alpha_host && orig[P] >= 0) ? alpha_host[orig[P]] : 0.0;
+    IPF_HIP(ctx, hipMemcpy(ctx->dAlpha, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    return IPF_OK;
+}
+
+}  // extern "C"

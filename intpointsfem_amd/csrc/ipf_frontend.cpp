@@ -1,0 +1,222 @@
+// ipf_frontend.cpp -- C ABI of include/intpointsfem_frontend.h over the C++
+// classes TPZCompMesh and TPZElastoPlasticIntPointsStructMatrix.  Exceptions
+// are caught here and turned into status codes.
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/intpointsfem_frontend.h"
+#include "TPZCompMesh.h"
+#include "TPZElastoPlasticIntPointsStructMatrix.h"
+
+// Last-error text shared with the device layer (ipf_device.hip).
+extern "C" int ipf_internal_set_error(const char* msg);
+
+struct ipf_cmesh {
+    std::unique_ptr<TPZCompMesh> mesh;
+};
+
+struct ipf_sm {
+    ipf_cmesh* mesh;
+    std::unique_ptr<TPZElastoPlasticIntPointsStructMatrix> sm;
+};
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        ipf_internal_set_error("out of host memory");
+        return IPF_EINVAL;
+    } catch (const std::invalid_argument& e) {
+        ipf_internal_set_error(e.what());
+        return IPF_EINVAL;
+    } catch (const std::exception& e) {
+        ipf_internal_set_error(e.what());
+        return IPF_EHIP;
+    }
+}
+
+int null_arg(const char* fn) {
+    ipf_internal_set_error((std::string(fn) + ": NULL argument").c_str());
+    return IPF_EINVAL;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ipf_cmesh_quarter_annulus(int nr, int ntheta, int porder, double r_in, double r_out, double ratio,
+                              ipf_cmesh** out) {
+    if (!out) return null_arg("ipf_cmesh_quarter_annulus");
+    *out = nullptr;
+    return guarded([&] {
+        auto m = std::make_unique<ipf_cmesh>();
+        m->mesh = std::make_unique<TPZCompMesh>(nr, ntheta, porder, r_in, r_out, ratio);
+        *out = m.release();
+        return IPF_OK;
+    });
+}
+
+int ipf_cmesh_destroy(ipf_cmesh* m) {
+    delete m;
+    return IPF_OK;
+}
+
+int ipf_cmesh_set_material(ipf_cmesh* m, double E, double nu, double c, double phi, double psi) {
+    if (!m) return null_arg("ipf_cmesh_set_material");
+    return guarded([&] {
+        m->mesh->SetMaterial(E, nu, c, phi, psi);
+        return IPF_OK;
+    });
+}
+
+int ipf_cmesh_set_normal_stress(ipf_cmesh* m, double inner, double outer) {
+    if (!m) return null_arg("ipf_cmesh_set_normal_stress");
+    m->mesh->SetNormalStress(inner, outer);
+    return IPF_OK;
+}
+
+int ipf_cmesh_info(const ipf_cmesh* m, int64_t out[8]) {
+    if (!m || !out) return null_arg("ipf_cmesh_info");
+    const TPZCompMesh& c = *m->mesh;
+    out[0] = c.NElements();
+    out[1] = c.NNodes();
+    out[2] = c.NEquations();
+    out[3] = c.POrder();
+    out[4] = c.NRadial();
+    out[5] = c.NTheta();
+    out[6] = c.NIntPoints();
+    out[7] = c.NShapeF();
+    return IPF_OK;
+}
+
+int ipf_cmesh_material(const ipf_cmesh* m, double out[5]) {
+    if (!m || !out) return null_arg("ipf_cmesh_material");
+    const TPZCompMesh& c = *m->mesh;
+    out[0] = c.Lambda();
+    out[1] = c.Mu();
+    out[2] = c.Phi();
+    out[3] = c.Psi();
+    out[4] = c.Cohesion();
+    return IPF_OK;
+}
+
+int ipf_cmesh_nodes(const ipf_cmesh* m, double* xy) {
+    if (!m || !xy) return null_arg("ipf_cmesh_nodes");
+    for (int64_t n = 0; n < m->mesh->NNodes(); n++) m->mesh->NodeCoords(n, xy[2 * n], xy[2 * n + 1]);
+    return IPF_OK;
+}
+
+int ipf_cmesh_connectivity(const ipf_cmesh* m, int64_t* conn_ptr, int64_t* conn) {
+    if (!m || !conn_ptr || !conn) return null_arg("ipf_cmesh_connectivity");
+    std::vector<int64_t> nodes;
+    conn_ptr[0] = 0;
+    for (int64_t e = 0; e < m->mesh->NElements(); e++) {
+        m->mesh->ElementConnects(e, nodes);
+        std::memcpy(conn + conn_ptr[e], nodes.data(), sizeof(int64_t) * nodes.size());
+        conn_ptr[e + 1] = conn_ptr[e] + (int64_t)nodes.size();
+    }
+    return IPF_OK;
+}
+
+int ipf_cmesh_dirichlet_mask(const ipf_cmesh* m, signed char* mask) {
+    if (!m || !mask) return null_arg("ipf_cmesh_dirichlet_mask");
+    std::vector<signed char> v;
+    m->mesh->DirichletMask(v);
+    std::memcpy(mask, v.data(), v.size());
+    return IPF_OK;
+}
+
+int ipf_cmesh_set_update_mem(ipf_cmesh* m, int update_mem) {
+    if (!m) return null_arg("ipf_cmesh_set_update_mem");
+    m->mesh->SetUpdateMem(update_mem != 0);
+    return IPF_OK;
+}
+
+int ipf_sm_create(ipf_cmesh* m, int device, ipf_sm** out) {
+    if (!m || !out) return null_arg("ipf_sm_create");
+    *out = nullptr;
+    return guarded([&] {
+        auto s = std::make_unique<ipf_sm>();
+        s->mesh = m;
+        s->sm = std::make_unique<TPZElastoPlasticIntPointsStructMatrix>(m->mesh.get(), device < 0 ? 0 : device);
+        s->sm->SetUpDataStructure();
+        if (device >= 0) s->sm->TransferDataToGPU();
+        *out = s.release();
+        return IPF_OK;
+    });
+}
+
+int ipf_sm_destroy(ipf_sm* sm) {
+    delete sm;
+    return IPF_OK;
+}
+
+ipf_ctx* ipf_sm_context(ipf_sm* sm) { return sm ? sm->sm->Integrator().Context().get() : nullptr; }
+
+int ipf_sm_sizes(const ipf_sm* sm, int64_t out[8]) {
+    if (!sm || !out) return null_arg("ipf_sm_sizes");
+    auto& in = sm->sm->Integrator();
+    auto& B = in.BlockMatrix().Blocks();
+    out[0] = B.fNumBlocks;
+    out[1] = in.BlockMatrix().Rows() / 3;
+    out[2] = in.BlockMatrix().Rows();
+    out[3] = in.BlockMatrix().Cols();
+    out[4] = sm->mesh->mesh->NEquations();
+    out[5] = in.NColor();
+    out[6] = B.fNumBlocks ? B.fMatrixPosition[B.fNumBlocks] : 0;
+    out[7] = 0;
+    return IPF_OK;
+}
+
+int ipf_sm_export_blocks(const ipf_sm* sm, int* row_sizes, int* col_sizes, int64_t* matrix_position,
+                         int* row_first, int* col_first, double* storage) {
+    if (!sm) return null_arg("ipf_sm_export_blocks");
+    auto& B = sm->sm->Integrator().BlockMatrix().Blocks();
+    const int64_t n = B.fNumBlocks;
+    if (row_sizes) std::memcpy(row_sizes, B.fRowSizes.data(), sizeof(int) * n);
+    if (col_sizes) std::memcpy(col_sizes, B.fColSizes.data(), sizeof(int) * n);
+    if (matrix_position) std::memcpy(matrix_position, B.fMatrixPosition.data(), sizeof(int64_t) * (n + 1));
+    if (row_first) std::memcpy(row_first, B.fRowFirstIndex.data(), sizeof(int) * (n + 1));
+    if (col_first) std::memcpy(col_first, B.fColFirstIndex.data(), sizeof(int) * (n + 1));
+    if (storage) std::memcpy(storage, B.fStorage.data(), sizeof(double) * B.fStorage.size());
+    return IPF_OK;
+}
+
+int ipf_sm_export_indexes(const ipf_sm* sm, int* dof_indexes, int* color_indexes, int* el_color_index,
+                          int64_t* first_color_index, double* weight, double* rhs_linear) {
+    if (!sm) return null_arg("ipf_sm_export_indexes");
+    auto& in = sm->sm->Integrator();
+    if (dof_indexes) std::memcpy(dof_indexes, in.DoFIndexes().data(), sizeof(int) * in.DoFIndexes().size());
+    if (color_indexes) std::memcpy(color_indexes, in.ColorIndexes().data(), sizeof(int) * in.ColorIndexes().size());
+    if (el_color_index)
+        std::memcpy(el_color_index, in.ElColorIndex().data(), sizeof(int) * in.BlockMatrix().Blocks().fNumBlocks);
+    if (first_color_index)
+        std::memcpy(first_color_index, in.FirstColorIndex().data(), sizeof(int64_t) * in.FirstColorIndex().size());
+    if (weight)
+        std::memcpy(weight, in.ConstitutiveLawProcessor().Weight().data(),
+                    sizeof(double) * in.ConstitutiveLawProcessor().Weight().size());
+    if (rhs_linear)
+        std::memcpy(rhs_linear, sm->sm->RhsLinear().data(), sizeof(double) * sm->sm->RhsLinear().Rows());
+    return IPF_OK;
+}
+
+int ipf_sm_assemble_rhs(ipf_sm* sm, const double* u_host, double* rhs_host) {
+    if (!sm || !u_host || !rhs_host) return null_arg("ipf_sm_assemble_rhs");
+    return guarded([&] {
+        TPZCompMesh* m = sm->mesh->mesh.get();
+        std::memcpy(m->Solution().data(), u_host, sizeof(double) * m->NEquations());
+        TPZFMatrix<STATE> rhs;
+        sm->sm->Assemble(rhs);
+        std::memcpy(rhs_host, rhs.data(), sizeof(double) * m->NEquations());
+        return IPF_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// ipf_kernels.h -- device data layout shared by the gfx950 kernels and the
+// host code that builds it.
+//
+// HBM layout (built once by ipf_transfer_data):
+//   Elements are grouped by color, then by block shape (n_ip, C_e).  Within a
+//   group, elements keep the reference's ascending order and are packed into
+//   tiles of ET = floor(64 / n_ip) elements; one 64-lane wavefront owns a tile
+//   and lane q < TP = ET * n_ip owns integration point (q / n_ip, q % n_ip).
+//   B of a tile is stored point-interleaved (structure of arrays): value m of
+//   point q (m = 3 * k + ... row k in {xx, xy, yy} times column j) sits at
+//   double2 index (m / 2) * TP + q, component m % 2, so every 16-byte load of
+//   the wavefront reads 64 consecutive double2 (1 KiB contiguous).
+//   Per-point arrays (weight, plastic strain XX/XY/YY/ZZ, alpha, mtype,
+//   branch) are SoA in the same tile order ("new" point index).
+#pragma once
+
+#include <cstdint>
+
+#include "ipf_pointmath.h"
+
+namespace ipf {
+
+struct GroupDev {
+    const double* B;      // tile-major, point-interleaved B values of the group
+    const int* dof;       // [nelem * CE] DoF indexes, element order of the group
+    int64_t nelem;        // elements in the group
+    int64_t ntiles;       // ceil(nelem / ET)
+    int64_t pt0;          // new-point index of the group's first slot
+};
+
+struct PointState {
+    const double* w;
+    double* ep0;          // plastic strain XX
+    double* ep1;          // plastic strain XY (engineering)
+    double* ep2;          // plastic strain YY
+    double* ep3;          // plastic strain ZZ
+    double* alpha;
+    signed char* mtype;
+    signed char* branch;
+};
+
+// Supported block shapes (n_ip, C_e): Q1, Q2, Q3 quadrilaterals with
+// (p+1)^2 Gauss points.
+enum Shape { SHAPE_Q1 = 0, SHAPE_Q2 = 1, SHAPE_Q3 = 2, SHAPE_COUNT = 3 };
+
+constexpr int shape_nip(int s) { return s == SHAPE_Q1 ? 4 : s == SHAPE_Q2 ? 9 : 16; }
+constexpr int shape_ce(int s) { return s == SHAPE_Q1 ? 8 : s == SHAPE_Q2 ? 18 : 32; }
+constexpr int shape_et(int s) { return 64 / shape_nip(s); }
+constexpr int shape_tp(int s) { return shape_et(s) * shape_nip(s); }
+
+// Kernel launchers (ipf_kernels.hip).  All enqueue on `stream`.
+void launch_residual(int shape, bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
+                     const MatConst& m, hipStream_t stream);
+void launch_multiply(int shape, const GroupDev& g, const double* u, double* strain, const int64_t* orig_of_new,
+                     hipStream_t stream);
+void launch_transpose(int shape, const GroupDev& g, const double* wsigma, double* r, const int64_t* orig_of_new,
+                      hipStream_t stream);
+void launch_sigma(bool dep, bool commit, int64_t npad, const double* strain, double* wsigma, double* wdep,
+                  const int64_t* orig_of_new, const PointState& st, const MatConst& m, hipStream_t stream);
+void launch_copy(int64_t n, const double* src, double* dst, hipStream_t stream);
+void launch_fill(int64_t n, double* dst, double value, hipStream_t stream);
+
+}  // namespace ipf

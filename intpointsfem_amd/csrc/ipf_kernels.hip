@@ -1,0 +1,334 @@
+// ipf_kernels.hip -- gfx950 kernels of the integration-point residual path.
+//
+// k_residual is the hot path: one launch per element color, one 64-lane
+// wavefront per tile of elements (ipf_kernels.h).  Per tile it
+//   1. gathers u at the elements' DoFs (TPZNumericalIntegrator.cpp:32,47),
+//   2. streams the tile's B values with coalesced 16-byte loads and forms the
+//      strain of each point, summing columns j = 0..C_e-1 in order
+//      (TPZIrregularBlocksMatrix.cpp:97-140),
+//   3. runs the Mohr-Coulomb return map per point (ipf_pointmath.h),
+//   4. forms each point's contribution to -B^T (w sigma) from the B values
+//      still in registers, reduces the element's points through LDS, and
+//   5. adds the element forces into r.  Elements of one color share no DoF
+//      (TPZNumericalIntegrator.cpp:452-536), so the read-modify-write of r
+//      needs no atomics; colors run as consecutive launches on one stream.
+#include "ipf_kernels.h"
+
+namespace ipf {
+
+namespace {
+
+constexpr int kWave = 64;
+
+template <int SHAPE>
+struct Cfg {
+    static constexpr int NIP = shape_nip(SHAPE);
+    static constexpr int CE = shape_ce(SHAPE);
+    static constexpr int ET = shape_et(SHAPE);
+    static constexpr int TP = shape_tp(SHAPE);
+    static constexpr int NB = 3 * CE;            // B values per point
+    static constexpr bool KEEP_B = (NB <= 64);   // keep B in registers across the return map
+    static constexpr int WPB = (CE <= 18) ? 4 : 2;  // wavefronts per workgroup
+};
+
+// Loads the NB values of point `lane` of tile `tile` (pairs of doubles).
+template <int SHAPE>
+__device__ __forceinline__ void load_point_B(const double* __restrict__ Bg, int64_t tile, int lane, double* b) {
+    using C = Cfg<SHAPE>;
+    const double2* Bt = reinterpret_cast<const double2*>(Bg + tile * (int64_t)(C::NB * C::TP));
+#pragma unroll
+    for (int m2 = 0; m2 < C::NB / 2; m2++) {
+        const double2 v = Bt[m2 * C::TP + lane];
+        b[2 * m2] = v.x;
+        b[2 * m2 + 1] = v.y;
+    }
+}
+
+// strain_k = sum_j B[k][j] u_e[j], j ascending, starting from 0.0
+template <int SHAPE>
+__device__ __forceinline__ void strain_of(const double* b, const double* ue, double* e3) {
+    using C = Cfg<SHAPE>;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) acc = acc + b[k * C::CE + j] * ue[j];
+        e3[k] = acc;
+    }
+}
+
+// Point partial of B^T x: g_j = B[0][j] x0 + B[1][j] x1 + B[2][j] x2 -> LDS row.
+template <int SHAPE>
+__device__ __forceinline__ void bt_partial(const double* b, const double* x, double* row) {
+    using C = Cfg<SHAPE>;
+#pragma unroll
+    for (int j = 0; j < C::CE; j++) {
+        double g = b[j] * x[0];
+        g = g + b[C::CE + j] * x[1];
+        g = g + b[2 * C::CE + j] * x[2];
+        row[j] = g;
+    }
+}
+
+// Sums the element's point partials (ip ascending), negates and adds into r.
+template <int SHAPE>
+__device__ __forceinline__ void reduce_scatter(const double* L, const GroupDev& g, int64_t tile, int lane,
+                                               double* __restrict__ r) {
+    using C = Cfg<SHAPE>;
+    for (int task = lane; task < C::ET * C::CE; task += kWave) {
+        const int e2 = task / C::CE;
+        const int j = task - e2 * C::CE;
+        const int64_t elem = tile * C::ET + e2;
+        if (elem < g.nelem) {
+            double f = L[(e2 * C::NIP) * C::CE + j];
+#pragma unroll
+            for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e2 * C::NIP + ip) * C::CE + j];
+            const int d = g.dof[elem * C::CE + j];
+            r[d] = r[d] + (-f);
+        }
+    }
+}
+
+template <int SHAPE, bool COMMIT>
+__global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave)
+k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, PointState st, MatConst m) {
+    using C = Cfg<SHAPE>;
+    __shared__ double lds[C::WPB][C::TP * C::CE];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int64_t tile = (int64_t)blockIdx.x * C::WPB + wave;
+    const bool tile_ok = tile < g.ntiles;
+    const int el = lane / C::NIP;
+    const int64_t elem = tile * C::ET + el;
+    const bool slot = tile_ok && lane < C::TP;
+    const bool valid = slot && elem < g.nelem;
+
+    double ue[C::CE];
+    if (valid) {
+        const int* d = g.dof + elem * C::CE;
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) ue[j] = u[d[j]];
+    } else {
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) ue[j] = 0.0;
+    }
+    double b[C::NB];
+    if (slot) {
+        load_point_B<SHAPE>(g.B, tile, lane, b);
+    } else {
+#pragma unroll
+        for (int k = 0; k < C::NB; k++) b[k] = 0.0;
+    }
+    double e3[3];
+    strain_of<SHAPE>(b, ue, e3);
+
+    double ws[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+        const int64_t P = g.pt0 + tile * C::TP + lane;
+        double ep[4] = {st.ep0[P], st.ep1[P], st.ep2[P], st.ep3[P]};
+        double alpha = st.alpha[P];
+        int mt, br;
+        point_sigma<false>(e3, ep, alpha, mt, br, m, st.w[P], ws, nullptr);
+        if (COMMIT) {
+            st.ep0[P] = ep[0];
+            st.ep1[P] = ep[1];
+            st.ep2[P] = ep[2];
+            st.ep3[P] = ep[3];
+            st.alpha[P] = alpha;
+            st.mtype[P] = (signed char)mt;
+            st.branch[P] = (signed char)br;
+        }
+    }
+    if (!C::KEEP_B && slot) load_point_B<SHAPE>(g.B, tile, lane, b);
+    double* L = lds[wave];
+    if (slot) bt_partial<SHAPE>(b, ws, L + lane * C::CE);
+    __syncthreads();
+    if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, r);
+}
+
+template <int SHAPE>
+__global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave)
+k_multiply(GroupDev g, const double* __restrict__ u, double* __restrict__ strain, const int64_t* __restrict__ orig) {
+    using C = Cfg<SHAPE>;
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int64_t tile = (int64_t)blockIdx.x * C::WPB + wave;
+    const int el = lane / C::NIP;
+    const int64_t elem = tile * C::ET + el;
+    if (!(tile < g.ntiles && lane < C::TP && elem < g.nelem)) return;
+    double ue[C::CE];
+    const int* d = g.dof + elem * C::CE;
+#pragma unroll
+    for (int j = 0; j < C::CE; j++) ue[j] = u[d[j]];
+    double b[C::NB];
+    load_point_B<SHAPE>(g.B, tile, lane, b);
+    double e3[3];
+    strain_of<SHAPE>(b, ue, e3);
+    const int64_t o = orig[g.pt0 + tile * C::TP + lane];
+    strain[3 * o + 0] = e3[0];
+    strain[3 * o + 1] = e3[1];
+    strain[3 * o + 2] = e3[2];
+}
+
+template <int SHAPE>
+__global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave)
+k_transpose(GroupDev g, const double* __restrict__ x, double* __restrict__ r, const int64_t* __restrict__ orig) {
+    using C = Cfg<SHAPE>;
+    __shared__ double lds[C::WPB][C::TP * C::CE];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int64_t tile = (int64_t)blockIdx.x * C::WPB + wave;
+    const bool tile_ok = tile < g.ntiles;
+    const int el = lane / C::NIP;
+    const int64_t elem = tile * C::ET + el;
+    const bool slot = tile_ok && lane < C::TP;
+    const bool valid = slot && elem < g.nelem;
+    double ws[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+        const int64_t o = orig[g.pt0 + tile * C::TP + lane];
+        ws[0] = x[3 * o + 0];
+        ws[1] = x[3 * o + 1];
+        ws[2] = x[3 * o + 2];
+    }
+    double* L = lds[wave];
+    if (slot) {
+        double b[C::NB];
+        load_point_B<SHAPE>(g.B, tile, lane, b);
+        bt_partial<SHAPE>(b, ws, L + lane * C::CE);
+    }
+    __syncthreads();
+    if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, r);
+}
+
+template <bool DEP>
+__global__ void __launch_bounds__(256)
+k_sigma(int64_t npad, const double* __restrict__ strain, double* __restrict__ wsigma, double* __restrict__ wdep,
+        const int64_t* __restrict__ orig, PointState st, MatConst m, int commit) {
+    const int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= npad) return;
+    const int64_t o = orig[P];
+    if (o < 0) return;
+    const double e3[3] = {strain[3 * o], strain[3 * o + 1], strain[3 * o + 2]};
+    double ep[4] = {st.ep0[P], st.ep1[P], st.ep2[P], st.ep3[P]};
+    double alpha = st.alpha[P];
+    int mt, br;
+    double ws[3], d9[9];
+    point_sigma<DEP>(e3, ep, alpha, mt, br, m, st.w[P], ws, d9);
+    wsigma[3 * o + 0] = ws[0];
+    wsigma[3 * o + 1] = ws[1];
+    wsigma[3 * o + 2] = ws[2];
+    if (DEP) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) wdep[9 * o + i] = d9[i];
+    }
+    if (commit) {
+        st.ep0[P] = ep[0];
+        st.ep1[P] = ep[1];
+        st.ep2[P] = ep[2];
+        st.ep3[P] = ep[3];
+        st.alpha[P] = alpha;
+        st.mtype[P] = (signed char)mt;
+        st.branch[P] = (signed char)br;
+    }
+}
+
+__global__ void k_copy(int64_t n, const double* __restrict__ src, double* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_fill(int64_t n, double* __restrict__ dst, double v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+template <int SHAPE>
+dim3 tile_grid(const GroupDev& g) {
+    return dim3((unsigned)((g.ntiles + Cfg<SHAPE>::WPB - 1) / Cfg<SHAPE>::WPB));
+}
+
+template <int SHAPE>
+void residual_shape(bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
+                    const MatConst& m, hipStream_t s) {
+    if (g.ntiles == 0) return;
+    const dim3 block(Cfg<SHAPE>::WPB * kWave);
+    if (commit)
+        hipLaunchKernelGGL((k_residual<SHAPE, true>), tile_grid<SHAPE>(g), block, 0, s, g, u, r, st, m);
+    else
+        hipLaunchKernelGGL((k_residual<SHAPE, false>), tile_grid<SHAPE>(g), block, 0, s, g, u, r, st, m);
+}
+
+}  // namespace
+
+void launch_residual(int shape, bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
+                     const MatConst& m, hipStream_t s) {
+    switch (shape) {
+        case SHAPE_Q1: residual_shape<SHAPE_Q1>(commit, g, u, r, st, m, s); break;
+        case SHAPE_Q2: residual_shape<SHAPE_Q2>(commit, g, u, r, st, m, s); break;
+        case SHAPE_Q3: residual_shape<SHAPE_Q3>(commit, g, u, r, st, m, s); break;
+        default: break;
+    }
+}
+
+void launch_multiply(int shape, const GroupDev& g, const double* u, double* strain, const int64_t* orig,
+                     hipStream_t s) {
+    if (g.ntiles == 0) return;
+    switch (shape) {
+        case SHAPE_Q1:
+            hipLaunchKernelGGL(k_multiply<SHAPE_Q1>, tile_grid<SHAPE_Q1>(g), dim3(Cfg<SHAPE_Q1>::WPB * kWave), 0, s,
+                               g, u, strain, orig);
+            break;
+        case SHAPE_Q2:
+            hipLaunchKernelGGL(k_multiply<SHAPE_Q2>, tile_grid<SHAPE_Q2>(g), dim3(Cfg<SHAPE_Q2>::WPB * kWave), 0, s,
+                               g, u, strain, orig);
+            break;
+        case SHAPE_Q3:
+            hipLaunchKernelGGL(k_multiply<SHAPE_Q3>, tile_grid<SHAPE_Q3>(g), dim3(Cfg<SHAPE_Q3>::WPB * kWave), 0, s,
+                               g, u, strain, orig);
+            break;
+        default: break;
+    }
+}
+
+void launch_transpose(int shape, const GroupDev& g, const double* x, double* r, const int64_t* orig, hipStream_t s) {
+    if (g.ntiles == 0) return;
+    switch (shape) {
+        case SHAPE_Q1:
+            hipLaunchKernelGGL(k_transpose<SHAPE_Q1>, tile_grid<SHAPE_Q1>(g), dim3(Cfg<SHAPE_Q1>::WPB * kWave), 0,
+                               s, g, x, r, orig);
+            break;
+        case SHAPE_Q2:
+            hipLaunchKernelGGL(k_transpose<SHAPE_Q2>, tile_grid<SHAPE_Q2>(g), dim3(Cfg<SHAPE_Q2>::WPB * kWave), 0,
+                               s, g, x, r, orig);
+            break;
+        case SHAPE_Q3:
+            hipLaunchKernelGGL(k_transpose<SHAPE_Q3>, tile_grid<SHAPE_Q3>(g), dim3(Cfg<SHAPE_Q3>::WPB * kWave), 0,
+                               s, g, x, r, orig);
+            break;
+        default: break;
+    }
+}
+
+void launch_sigma(bool dep, bool commit, int64_t npad, const double* strain, double* wsigma, double* wdep,
+                  const int64_t* orig, const PointState& st, const MatConst& m, hipStream_t s) {
+    if (npad == 0) return;
+    const dim3 grid((unsigned)((npad + 255) / 256));
+    if (dep)
+        hipLaunchKernelGGL(k_sigma<true>, grid, dim3(256), 0, s, npad, strain, wsigma, wdep, orig, st, m,
+                           commit ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_sigma<false>, grid, dim3(256), 0, s, npad, strain, wsigma, wdep, orig, st, m,
+                           commit ? 1 : 0);
+}
+
+void launch_copy(int64_t n, const double* src, double* dst, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, src, dst);
+}
+
+void launch_fill(int64_t n, double* dst, double v, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, dst, v);
+}
+
+}  // namespace ipf

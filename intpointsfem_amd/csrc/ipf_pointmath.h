@@ -1,0 +1,519 @@
+// ipf_pointmath.h -- per-integration-point Mohr-Coulomb return map for the
+// gfx950 kernels (2D plane strain, perfect plasticity).
+//
+// Restates FunctionsStressStrain.h, FunctionsSpectralDecomp.h,
+// FunctionsSigmaProjection.h and FunctionsTangentOperator.h of IntPointsFEM
+// with the same floating-point operation order (the library is built with
+// -ffp-contract=off, so no multiply-add is fused), so strains, branch flags
+// and stresses are bit-identical to the CPU path.  Material trigonometry and
+// every material-only subexpression are evaluated once on the host
+// (MatConst::make) with the expression the reference evaluates per point.
+//
+// Layout conventions: full tensors are 6-vectors in Voigt order
+// [XX, XY, XZ, YY, YZ, ZZ] (FunctionsTangentOperator.h:40-41); the committed
+// plastic strain is stored as 4 components [XX, XY, YY, ZZ] because its XZ and
+// YZ components are +0 in plane strain (0 - (+-0) == +0 in IEEE arithmetic).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace ipf {
+
+enum { XX = 0, XY = 1, XZ = 2, YY = 3, YZ = 4, ZZ = 5 };
+enum { BR_ELASTIC = 0, BR_MAIN = 1, BR_RIGHT = 2, BR_LEFT = 3, BR_APEX = 4 };
+
+// Material constants: everything the return map needs that depends only on
+// (lambda, mu, phi, psi, c).
+struct MatConst {
+    double lambda, mu, G, K;
+    double lam2mu;          // lambda + 2*mu                      (FunctionsStressStrain.h:48-50)
+    double cs_a, cs_b;      // compliance factors                  (FunctionsStressStrain.h:74-76)
+    double inv_mu;          // 1/mu                                (FunctionsStressStrain.h:78-80)
+    double sinphi, cosphi, sinpsi;
+    double phi_c;           // 2.*c*cosphi                         (FunctionsSigmaProjection.h:12,38)
+    double edge_c;          // 2.*cosphi*c                         (FunctionsSigmaProjection.h:115,220)
+    double constA;          //                                     (FunctionsSigmaProjection.h:36)
+    double m0, m1, m2;      // principal-stress update factors     (FunctionsSigmaProjection.h:53-55)
+    double r_ab0, r_ab1, r_ji00, r_ji01, r_ji10, r_ji11;  // right edge (:117-136)
+    double l_ab0, l_ab1, l_ji00, l_ji01, l_ji10, l_ji11;  // left edge  (:217-238)
+    double omsp, opsp;      // 1 - sinpsi, 1 + sinpsi              (FunctionsSigmaProjection.h:359)
+    double c_cot;           // c * (1/tan(phi))                    (FunctionsSigmaProjection.h:293,301)
+    double apex_alpha;      // cos(phi)/sin(psi)                   (FunctionsSigmaProjection.h:300)
+    double tan_plane[9], tan_right[9], tan_left[9];
+
+    static MatConst make(double lambda, double mu, double mc_phi, double mc_psi, double c);
+};
+
+// ---------------------------------------------------------------- spectral
+
+// FunctionsSpectralDecomp.h:7-20
+__device__ __forceinline__ double normalize6(double* s) {
+    double maxel = s[0];
+#pragma unroll
+    for (int i = 1; i < 6; i++)
+        if (fabs(s[i]) > fabs(maxel)) maxel = s[i];
+    if (maxel != 0) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) s[i] /= maxel;
+    }
+    return maxel;
+}
+
+// FunctionsSpectralDecomp.h:25-56 (row 3 uses sxy + syz as the reference does)
+__device__ __forceinline__ void gershgorin(const double* s, double& upper_out, double& lower_out) {
+    const double row_1 = s[XY] + s[XZ];
+    const double lo0 = s[XX] - fabs(row_1), up0 = s[XX] + fabs(row_1);
+    const double row_2 = s[XY] + s[YZ];
+    const double lo1 = s[YY] - fabs(row_2), up1 = s[YY] + fabs(row_2);
+    const double row_3 = s[XY] + s[YZ];
+    const double lo2 = s[ZZ] - fabs(row_3), up2 = s[ZZ] + fabs(row_3);
+    double up = up0, lo = lo0;
+    if (up1 > up) up = up1;
+    if (lo1 < lo) lo = lo1;
+    if (up2 > up) up = up2;
+    if (lo2 < lo) lo = lo2;
+    upper_out = up;
+    lower_out = lo;
+}
+
+// Cubic of FunctionsSpectralDecomp.h:75-77, evaluated left to right.
+__device__ __forceinline__ double cubic(double x, const double* s, double sxy2, double sxz2, double syz2) {
+    return -x * x * x - sxz2 * s[YY] - sxy2 * s[ZZ] - syz2 * s[XX] + 2 * s[XY] * s[XZ] * s[YZ] +
+           s[XX] * s[YY] * s[ZZ] + x * x * (s[XX] + s[YY] + s[ZZ]) +
+           x * (sxy2 + sxz2 + syz2 - s[XX] * s[YY] - s[XX] * s[ZZ] - s[YY] * s[ZZ]);
+}
+
+// FunctionsSpectralDecomp.h:61-111: Newton from each Gershgorin bound
+// (20 iterations, |f| > 1e-11), third root from the trace, descending sort.
+__device__ __forceinline__ void eigenvalues_newton(double up, double lo, const double* s, double maxel, double* ev) {
+    const double sxy2 = s[XY] * s[XY];
+    const double sxz2 = s[XZ] * s[XZ];
+    const double syz2 = s[YZ] * s[YZ];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        double x = (i == 0) ? up : lo;
+        double f = cubic(x, s, sxy2, sxz2, syz2);
+        double res = fabs(f);
+        int it = 0;
+        while (it < 20 && res > 10e-12) {
+            const double df = -3 * x * x + sxy2 + sxz2 + syz2 - s[XX] * s[YY] - s[XX] * s[ZZ] - s[YY] * s[ZZ] +
+                              2 * x * (s[XX] + s[YY] + s[ZZ]);
+            x -= f / df;
+            f = cubic(x, s, sxy2, sxz2, syz2);
+            res = fabs(f);
+            it++;
+        }
+        ev[i] = x;
+    }
+    ev[2] = s[XX] + s[YY] + s[ZZ] - ev[0] - ev[1];
+    ev[0] *= maxel;
+    ev[1] *= maxel;
+    ev[2] *= maxel;
+    // i = 0: j = 1, 2; i = 1: j = 2 (FunctionsSpectralDecomp.h:102-110)
+    if (ev[0] < ev[1]) { const double a = ev[0]; ev[0] = ev[1]; ev[1] = a; }
+    if (ev[0] < ev[2]) { const double a = ev[0]; ev[0] = ev[2]; ev[2] = a; }
+    if (ev[1] < ev[2]) { const double a = ev[1]; ev[1] = ev[2]; ev[2] = a; }
+}
+
+// FunctionsSpectralDecomp.h:116-150
+__device__ __forceinline__ void eigvec_simple(const double* s, double lam, double* vec) {
+    const double d0 = (s[XX] - lam) * (s[YY] - lam) - s[XY] * s[XY];
+    const double d1 = (s[XX] - lam) * (s[ZZ] - lam) - s[XZ] * s[XZ];
+    const double d2 = (s[YY] - lam) * (s[ZZ] - lam) - s[YZ] * s[YZ];
+    double maxdet = fabs(d0);
+    if (fabs(d1) > fabs(maxdet)) maxdet = fabs(d1);
+    if (fabs(d2) > fabs(maxdet)) maxdet = fabs(d2);
+    double v0, v1, v2;
+    if (maxdet == fabs(d0)) {
+        v0 = 1 / d0 * (-(s[YY] - lam) * s[XZ] + s[XY] * s[YZ]);
+        v1 = 1 / d0 * (-(s[XX] - lam) * s[YZ] + s[XY] * s[XZ]);
+        v2 = 1;
+    } else if (maxdet == fabs(d1)) {
+        v0 = 1 / d1 * (-(s[ZZ] - lam) * s[XY] + s[XZ] * s[YZ]);
+        v1 = 1;
+        v2 = 1 / d1 * (-(s[XX] - lam) * s[YZ] + s[XY] * s[XZ]);
+    } else {
+        v0 = 1;
+        v1 = 1 / d2 * (-(s[ZZ] - lam) * s[XY] + s[XZ] * s[YZ]);
+        v2 = 1 / d2 * (-(s[YY] - lam) * s[XZ] + s[XY] * s[YZ]);
+    }
+    const double norm = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    vec[0] = v0 / norm;
+    vec[1] = v1 / norm;
+    vec[2] = v2 / norm;
+}
+
+// FunctionsSpectralDecomp.h:155-212 (second norm keeps the v2[1]*v1[1] term of :203)
+__device__ __forceinline__ void eigvec_double(const double* s, double lam, double* vec1, double* vec2) {
+    const double x0 = s[XX] - lam, x1 = s[YY] - lam, x2 = s[ZZ] - lam;
+    double maxx = fabs(x0);
+    if (fabs(x1) > fabs(maxx)) maxx = fabs(x1);
+    if (fabs(x2) > fabs(maxx)) maxx = fabs(x2);
+    double a0, a1, a2, b0, b1, b2;
+    if (maxx == fabs(x0)) {
+        a0 = -s[XY] / x0; a1 = 1; a2 = 0;
+        b0 = -s[XZ] / x0; b1 = 0; b2 = 1;
+    } else if (maxx == fabs(x1)) {
+        a0 = 1; a1 = -s[XY] / x1; a2 = 0;
+        b0 = 0; b1 = -s[YZ] / x1; b2 = 1;
+    } else {
+        a0 = 1; a1 = 0; a2 = -s[XZ] / x2;
+        b0 = 0; b1 = 1; b2 = -s[YZ] / x2;
+    }
+    const double n1 = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    const double n2 = sqrt(b0 * b0 + b1 * a1 + b2 * b2);
+    vec1[0] = a0 / n1; vec1[1] = a1 / n1; vec1[2] = a2 / n1;
+    vec2[0] = b0 / n2; vec2[1] = b1 / n2; vec2[2] = b2 / n2;
+}
+
+// FunctionsSpectralDecomp.h:217-256 followed by the normalise/scale-back of
+// :261-268.  s is normalised on entry and scaled back by maxel here.
+__device__ __forceinline__ void eigenvectors6(double* s, const double* ev, double* vec, double maxel) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) s[i] *= maxel;
+    if ((ev[0] == ev[1]) && (ev[1] == ev[2])) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) vec[i] = 0.;
+        vec[0] = 1.;
+        vec[4] = 1.;
+        vec[8] = 1.;
+        return;
+    }
+    if (ev[0] != ev[1] && ev[0] != ev[2]) {
+        eigvec_simple(s, ev[0], &vec[0]);
+    } else if (ev[0] == ev[1]) {
+        eigvec_double(s, ev[0], &vec[0], &vec[3]);
+    } else if (ev[0] == ev[2]) {
+        eigvec_double(s, ev[0], &vec[0], &vec[6]);
+    }
+    if (ev[1] != ev[0] && ev[1] != ev[2]) {
+        eigvec_simple(s, ev[1], &vec[3]);
+    } else if (ev[1] == ev[2]) {
+        eigvec_double(s, ev[1], &vec[3], &vec[6]);
+    }
+    if (ev[2] != ev[0] && ev[2] != ev[1]) {
+        eigvec_simple(s, ev[2], &vec[6]);
+    }
+}
+
+// FunctionsSpectralDecomp.h:261-268 (s is modified in place like the reference)
+__device__ __forceinline__ void spectral(double* s, double* ev, double* vec) {
+    const double maxel = normalize6(s);
+    double up, lo;
+    gershgorin(s, up, lo);
+    eigenvalues_newton(up, lo, s, maxel, ev);
+    eigenvectors6(s, ev, vec, maxel);
+}
+
+// ---------------------------------------------------------------- projection
+
+__device__ __forceinline__ bool ordered(const double* e) {
+    return (e[0] > e[1] || fabs(e[0] - e[1]) < 1.e-12) && (e[1] > e[2] || fabs(e[1] - e[2]) < 1.e-12);
+}
+
+// FunctionsSigmaProjection.h:25-63 (H = 0)
+__device__ __forceinline__ bool main_plane(const double* tr, double* sp, double& hard, const MatConst& m) {
+    double e0 = tr[0], e1 = tr[1], e2 = tr[2];
+    double phi = e0 - e2 + (e0 + e2) * m.sinphi - m.phi_c;
+    double gamma = 0;
+    const double jac = -m.constA;
+    for (int i = 0; i < 30; i++) {
+        const double delta_gamma = -phi / jac;
+        gamma += delta_gamma;
+        phi = e0 - e2 + (e0 + e2) * m.sinphi - m.phi_c - m.constA * gamma;
+        if (fabs(phi) < 1.e-12) break;
+    }
+    e0 -= m.m0 * gamma;
+    e1 += m.m1 * gamma;
+    e2 += m.m2 * gamma;
+    sp[0] = e0;
+    sp[1] = e1;
+    sp[2] = e2;
+    hard += gamma * 2. * m.cosphi;
+    return ordered(sp);
+}
+
+// 2x2 Newton of the edge returns, FunctionsSigmaProjection.h:119-146 / :223-250.
+// gamma starts at {0, 0} (the reference leaves it uninitialised, :103, :205).
+__device__ __forceinline__ void edge_newton(double sb0, double sb1, double ab0, double ab1, double ji00, double ji01,
+                                            double ji10, double ji11, double edge_c, double& g0, double& g1) {
+    double phi0 = sb0 - edge_c;
+    double phi1 = sb1 - edge_c;
+    g0 = 0.;
+    g1 = 0.;
+    for (int i = 0; i < 30; i++) {
+        g0 -= (ji00 * phi0 + ji01 * phi1);
+        g1 -= (ji10 * phi0 + ji11 * phi1);
+        phi0 = sb0 - ab0 * g0 - ab1 * g1 - edge_c;
+        phi1 = sb1 - ab1 * g0 - ab0 * g1 - edge_c;
+        const double res = (fabs(phi0) + fabs(phi1));
+        if (fabs(res) < 1.e-12) break;
+    }
+}
+
+// FunctionsSigmaProjection.h:93-158
+__device__ __forceinline__ bool right_edge(const double* tr, double* sp, double& hard, const MatConst& m) {
+    double e0 = tr[0], e1 = tr[1], e2 = tr[2];
+    const double sb0 = e0 - e2 + (e0 + e2) * m.sinphi;
+    const double sb1 = e0 - e1 + (e0 + e1) * m.sinphi;
+    double g0, g1;
+    edge_newton(sb0, sb1, m.r_ab0, m.r_ab1, m.r_ji00, m.r_ji01, m.r_ji10, m.r_ji11, m.edge_c, g0, g1);
+    e0 -= m.m0 * (g0 + g1);
+    e1 += m.m1 * g0 + m.m2 * g1;
+    e2 += m.m2 * g0 + m.m1 * g1;
+    sp[0] = e0;
+    sp[1] = e1;
+    sp[2] = e2;
+    hard += (g0 + g1) * 2. * m.cosphi;
+    return ordered(sp);
+}
+
+// FunctionsSigmaProjection.h:193-257
+__device__ __forceinline__ bool left_edge(const double* tr, double* sp, double& hard, const MatConst& m) {
+    double e0 = tr[0], e1 = tr[1], e2 = tr[2];
+    const double sb0 = e0 - e2 + (e0 + e2) * m.sinphi;
+    const double sb1 = e1 - e2 + (e1 + e2) * m.sinphi;
+    double g0, g1;
+    edge_newton(sb0, sb1, m.l_ab0, m.l_ab1, m.l_ji00, m.l_ji01, m.l_ji10, m.l_ji11, m.edge_c, g0, g1);
+    e0 += -m.m0 * g0 + m.m1 * g1;
+    e1 += m.m1 * g0 - m.m0 * g1;
+    e2 += m.m2 * (g0 + g1);
+    sp[0] = e0;
+    sp[1] = e1;
+    sp[2] = e2;
+    hard += (g0 + g1) * 2. * m.cosphi;
+    return ordered(sp);
+}
+
+// FunctionsSigmaProjection.h:292-323
+__device__ __forceinline__ void apex(const double* tr, double* sp, double& hard, const MatConst& m) {
+    double ptr = 0.;
+    ptr += tr[0];
+    ptr += tr[1];
+    ptr += tr[2];
+    ptr /= 3.;
+    double depv = 0.;
+    double res = m.c_cot - ptr;
+    double p = 0.;
+    for (int i = 0; i < 30; i++) {
+        depv -= res / m.K;
+        p = ptr - m.K * depv;
+        res = m.c_cot - p;
+        if (fabs(res) < 1.e-12) break;
+    }
+    hard += m.apex_alpha * depv;
+    sp[0] = p;
+    sp[1] = p;
+    sp[2] = p;
+}
+
+// FunctionsSigmaProjection.h:337-379.  The gradient (principal-space
+// d sigma / d sigma_trial) is produced only when WANT_GRAD.
+template <bool WANT_GRAD>
+__device__ __forceinline__ void project(const double* ev, double* sp, int& mtype, int& branch, double& hard,
+                                        const MatConst& m, double* grad) {
+    const double phi = ev[0] - ev[2] + (ev[0] + ev[2]) * m.sinphi - m.phi_c;
+    sp[0] = ev[0];
+    sp[1] = ev[1];
+    sp[2] = ev[2];
+    if ((fabs(phi) < 1.e-12) || (phi < 0.0)) {
+        mtype = 0;
+        branch = BR_ELASTIC;
+        if (WANT_GRAD) {
+#pragma unroll
+            for (int i = 0; i < 9; i++) grad[i] = 0.;
+            grad[0] = 1.;
+            grad[4] = 1.;
+            grad[8] = 1.;
+        }
+        return;
+    }
+    mtype = 1;
+    branch = BR_MAIN;
+    if (main_plane(ev, sp, hard, m)) {
+        if (WANT_GRAD) {
+#pragma unroll
+            for (int i = 0; i < 9; i++) grad[i] = m.tan_plane[i];
+        }
+        return;
+    }
+    bool ok;
+    if ((m.omsp * ev[0] - 2. * ev[1] + m.opsp * ev[2]) > 0) {
+        branch = BR_RIGHT;
+        ok = right_edge(ev, sp, hard, m);
+        if (ok && WANT_GRAD) {
+#pragma unroll
+            for (int i = 0; i < 9; i++) grad[i] = m.tan_right[i];
+        }
+    } else {
+        branch = BR_LEFT;
+        ok = left_edge(ev, sp, hard, m);
+        if (ok && WANT_GRAD) {
+#pragma unroll
+            for (int i = 0; i < 9; i++) grad[i] = m.tan_left[i];
+        }
+    }
+    if (!ok) {
+        mtype = -1;
+        branch = BR_APEX;
+        apex(ev, sp, hard, m);
+        if (WANT_GRAD) {
+#pragma unroll
+            for (int i = 0; i < 9; i++) grad[i] = 0.;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- tangent
+
+// FunctionsTangentOperator.h:37-123, reduced to the 3x3 [xx, xy, yy] block of
+// FunctionsStressStrain.h:103-119 (TranslateDep) and scaled by w.
+__device__ __forceinline__ void tangent3(const double* grad, const double* sev, const double* eev, const double* v,
+                                         double G, double lambda, double w, double* dep9) {
+    const int kival[6] = {0, 0, 0, 1, 1, 2};
+    const int kjval[6] = {0, 1, 2, 1, 2, 2};
+    double T[36];
+#pragma unroll
+    for (int l = 0; l < 36; ++l) T[l] = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int ki = kival[k], kj = kjval[k];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                double temp;
+                if (ki == kj) {
+                    temp = 2 * G * v[j * 3 + kj] * v[j * 3 + ki];
+                    temp += lambda;
+                } else {
+                    temp = 2 * G * v[j * 3 + kj] * v[j * 3 + ki];
+                    temp *= 2.;
+                }
+#pragma unroll
+                for (int l = 0; l < 6; ++l) {
+                    const int li = kival[l], lj = kjval[l];
+                    T[l * 6 + k] += temp * grad[i * 3 + j] * v[i * 3 + li] * v[i * 3 + lj];
+                }
+            }
+        }
+    }
+    double tempMat[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) tempMat[i] = 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = i + 1; j < 3; ++j) {
+            const double deps = eev[i] - eev[j];
+            const double dsig = sev[i] - sev[j];
+            double factor;
+            if (!((fabs(deps) < 1.e-12) || (deps < 0.0)))
+                factor = dsig / deps;
+            else
+                factor = G * (grad[i * 3 + i] - grad[i * 3 + j] - grad[j * 3 + i] + grad[j * 3 + j]);
+#pragma unroll
+            for (int it = 0; it < 3; ++it)
+#pragma unroll
+                for (int jt = 0; jt < 3; ++jt) tempMat[it * 3 + jt] += v[3 * i + it] * v[3 * j + jt];
+#pragma unroll
+            for (int it = 0; it < 3; ++it)
+#pragma unroll
+                for (int jt = 0; jt < 3; ++jt) tempMat[it * 3 + jt] += v[3 * j + it] * v[3 * i + jt];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int ki = kival[k], kj = kjval[k];
+                double val;
+                if (ki == kj)
+                    val = v[j * 3 + ki] * v[i * 3 + kj] * factor;
+                else
+                    val = (v[j * 3 + ki] * v[i * 3 + kj] + v[j * 3 + kj] * v[i * 3 + ki]) * factor;
+                // FromMatToVoight of val*tempMat (upper triangle, row by row)
+                int kk = 0;
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = a; b < 3; b++) {
+                        const double cv = val * tempMat[a * 3 + b];
+                        const double aux = T[kk * 6 + k];
+                        T[kk * 6 + k] = aux + cv;
+                        kk++;
+                    }
+            }
+        }
+    }
+    // TranslateDep: rows/cols XX, XY, YY; column XY halved; then * w
+    const int ind[3] = {XX, XY, YY};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        dep9[3 * i + 0] = T[XX + 6 * ind[i]];
+        dep9[3 * i + 1] = T[XY + 6 * ind[i]];
+        dep9[3 * i + 2] = T[YY + 6 * ind[i]];
+    }
+    dep9[1] /= 2;
+    dep9[4] /= 2;
+    dep9[7] /= 2;
+#pragma unroll
+    for (int i = 0; i < 9; i++) dep9[i] = dep9[i] * w;
+}
+
+// ---------------------------------------------------------------- one point
+
+// One integration point of ComputeSigma / ComputeSigmaDep
+// (TPZConstitutiveLawProcessor.cpp:150-211, :364-439).
+//   e3: [exx, gxy, eyy]; ep4: committed plastic strain [XX, XY, YY, ZZ],
+//   updated in place; alpha: committed alpha, updated (alpha_n + d_alpha).
+//   ws3: w * [sxx, sxy, syy]; sig4 (optional): [XX, XY, YY, ZZ] stress.
+template <bool WANT_DEP>
+__device__ __forceinline__ void point_sigma(const double* e3, double* ep4, double& alpha, int& mtype, int& branch,
+                                            const MatConst& m, double w, double* ws3, double* wdep9) {
+    // ComposeStrain + ElasticStrain + ComputeTrialStress (FunctionsStressStrain.h:10-55)
+    double el[6];
+    el[XX] = e3[0] - ep4[0];
+    el[XY] = e3[1] - ep4[1];
+    el[XZ] = 0.0;
+    el[YY] = e3[2] - ep4[2];
+    el[YZ] = 0.0;
+    el[ZZ] = 0 - ep4[3];
+    double s[6];
+    s[XX] = m.lam2mu * el[XX] + m.lambda * el[YY] + m.lambda * el[ZZ];
+    s[YY] = m.lambda * el[XX] + m.lam2mu * el[YY] + m.lambda * el[ZZ];
+    s[ZZ] = m.lambda * el[XX] + m.lambda * el[YY] + m.lam2mu * el[ZZ];
+    s[XY] = m.mu * el[XY];
+    s[XZ] = m.mu * el[XZ];
+    s[YZ] = m.mu * el[YZ];
+    double ev[3], vec[9];
+    spectral(s, ev, vec);
+    double sp[3];
+    if (WANT_DEP) {
+        double eev[3], evec[9], grad[9];
+        el[XY] = el[XY] / 2;
+        spectral(el, eev, evec);
+        project<true>(ev, sp, mtype, branch, alpha, m, grad);
+        tangent3(grad, ev, eev, evec, m.G, m.lambda, w, wdep9);
+    } else {
+        project<false>(ev, sp, mtype, branch, alpha, m, nullptr);
+    }
+    // ReconstructStressTensor (FunctionsStressStrain.h:60-68), in-plane and ZZ
+    const double sxx = (sp[0] * vec[0] * vec[0] + sp[1] * vec[3] * vec[3] + sp[2] * vec[6] * vec[6]);
+    const double syy = (sp[0] * vec[1] * vec[1] + sp[1] * vec[4] * vec[4] + sp[2] * vec[7] * vec[7]);
+    const double szz = (sp[0] * vec[2] * vec[2] + sp[1] * vec[5] * vec[5] + sp[2] * vec[8] * vec[8]);
+    const double sxy = (sp[0] * vec[0] * vec[1] + sp[1] * vec[3] * vec[4] + sp[2] * vec[6] * vec[7]);
+    // ComputeStrain + PlasticStrain (FunctionsStressStrain.h:73-81, :38-42)
+    const double exx = m.cs_a * sxx - m.cs_b * (syy + szz);
+    const double eyy = m.cs_a * syy - m.cs_b * (sxx + szz);
+    const double ezz = m.cs_a * szz - m.cs_b * (syy + sxx);
+    const double exy = m.inv_mu * sxy;
+    ep4[0] = e3[0] - exx;
+    ep4[1] = e3[1] - exy;
+    ep4[2] = e3[2] - eyy;
+    ep4[3] = 0 - ezz;
+    // TranslateStress * w (FunctionsStressStrain.h:86-98; TPZConstitutiveLawProcessor.cpp:194)
+    ws3[0] = sxx * w;
+    ws3[1] = sxy * w;
+    ws3[2] = syy * w;
+}
+
+}  // namespace ipf
