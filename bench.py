@@ -1,0 +1,176 @@
+"""Residual throughput of the MI355X integration-point path (one GPU).
+
+A step is one full residual evaluation r = f_lin - sum_e B_e^T (w sigma(B_e u_e))
+(TPZElastoPlasticIntPointsStructMatrix::Assemble(rhs)) on the 800x800 Q2
+quarter-annulus wellbore (640,000 elements, 5.76 M integration points,
+5,126,402 DoF), plain FP64, with u and r resident in HBM.
+
+Prints ONE JSON line on stdout (progress goes to stderr):
+  value       Mint-pts/s = points * steps / timed seconds / 1e6
+  roofline    the fused per-color residual kernel against HBM: algorithmic
+              bytes (stored B once, 4-byte DoF index, weight + 4 plastic-strain
+              components per point, u read + r written once) over the
+              kernel's average duration, from HIP events around each launch
+  cpu_baseline the CPU oracle (same algorithm, OpenMP) on the host cores
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "residual Mint-pts/s, plain FP64 vs learned+mixed, at Newton ‖r‖/‖r0‖≤1e-8"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(blocks, npts, neq):
+    import numpy as np
+    R = blocks["row_sizes"].astype(np.int64)
+    C = blocks["col_sizes"].astype(np.int64)
+    return int((8 * R * C + 4 * C).sum()) + npts * (8 + 32) + neq * (8 + 8)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mesh", default="800x800-q2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's timed work")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the path runs on a single "
+              "MI355X; multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import intpointsfem_amd as ipf
+    from intpointsfem_amd import wellbore as wb
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    mesh = wb.make_mesh(args.mesh)
+    info = mesh.info()
+    log(f"[bench] mesh {args.mesh}: {info['nelem']} elements, neq {info['neq']}")
+    sm = ipf.StructMatrix(mesh, device=0)
+    ctx = sm.context()
+    sizes = sm.sizes()
+    npts, neq = sizes["npts"], sizes["neq"]
+    log(f"[bench] setup + upload {time.time() - t0:.1f}s; {npts} points, {sizes['ncolor']} colors, "
+        f"{ctx.info()['device_bytes'] / 1e9:.2f} GB on device")
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+
+    # displacement: linear-elastic thick-cylinder field of the final load step
+    u_host = wb.lame_displacement(mesh.nodes(), wb.normal_stress(wb.NSTEPS), scale=1.0)
+    u = torch.as_tensor(u_host, dtype=torch.float64, device=dev)
+    r = torch.zeros(neq, dtype=torch.float64, device=dev)
+
+    # branch mix of this state (one committed evaluation, then reset the state)
+    ctx.set_update_mem(True)
+    ctx.residual(u, r)
+    ctx.set_update_mem(False)
+    ctx.synchronize()
+    _, _, mt, br = ctx.get_state()
+    branch_counts = {ipf.BRANCH_NAMES[i]: int((br == i).sum()) for i in range(5)}
+    ctx.set_state(np.zeros(6 * npts))
+    log(f"[bench] branches: {branch_counts}")
+
+    for _ in range(args.warmup):
+        ctx.residual(u, r)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        ctx.residual(u, r)
+    end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(end)
+    ms_per_step = total_ms / args.steps
+    value = npts * args.steps / (total_ms / 1e3) / 1e6
+    log(f"[bench] {args.steps} residuals: {ms_per_step:.4f} ms/residual, {value:.1f} Mint-pts/s")
+
+    # per-launch durations (HIP events on the launch stream) for the roofline
+    nrep = max(3, min(args.steps, 20))
+    acc = None
+    pts = None
+    for _ in range(nrep):
+        ms, p = ctx.residual_timed(u, r)
+        acc = ms if acc is None else acc + ms
+        pts = p
+    avg = acc / nrep
+    color_ms = avg[1:]
+    blocks = sm.export_blocks()
+    nbytes = algorithmic_bytes(blocks, npts, neq)
+    # each color launch is credited with the share of the residual's bytes of its points
+    per_launch_bytes = nbytes * pts[1:] / npts
+    achieved = float(per_launch_bytes.sum() / (color_ms.sum() / 1e3) / 1e9)
+    log(f"[bench] copy kernel {avg[0]:.4f} ms; color kernels {', '.join(f'{x:.4f}' for x in color_ms)} ms; "
+        f"{nbytes / npts:.1f} algorithmic B/pt; {achieved:.0f} GB/s")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_residual (one launch per element color)",
+                "bytes_per_launch": int(per_launch_bytes.mean()),
+                "avg_launch_ms": round(float(color_ms.mean()), 5)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        ix = sm.export_indexes()
+        B = O.Blocks(blocks["row_sizes"], blocks["col_sizes"], blocks["matrix_position"], blocks["row_first"],
+                     blocks["col_first"], blocks["storage"])
+        mat = mesh.material()
+        nthreads = max(1, min(16, os.cpu_count() or 1))
+        reps = 0
+        t = time.time()
+        while True:
+            O.residual(B, ix["dof_indexes"], ix["color_indexes"], sizes["ncolor"], neq, ix["weight"], mat["lambda"],
+                       mat["mu"], mat["phi"], mat["psi"], mat["cohesion"], u_host, f_lin=ix["rhs_linear"],
+                       nthreads=nthreads)
+            reps += 1
+            el = time.time() - t
+            if el >= args.cpu_seconds or reps >= 50:
+                break
+        cpu_value = npts * reps / el / 1e6
+        cpu = {"value": round(cpu_value, 3), "unit": "Mint-pts/s", "cores": nthreads, "kind": "port",
+               "sample": f"{reps} full residual(s) of the same {args.mesh} state ({npts} points) by the C oracle "
+                         f"with {nthreads} OpenMP threads, {el:.1f} s"}
+        log(f"[bench] cpu oracle: {cpu_value:.2f} Mint-pts/s on {nthreads} threads ({reps} reps)")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mint-pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain FP64 residual, {args.mesh} quarter-annulus wellbore",
+                   "elements": sizes["nblocks"], "int_points": npts, "neq": neq, "colors": sizes["ncolor"],
+                   "state": "u = linear-elastic thick-cylinder displacement of load step 10, zero plastic strain",
+                   "branches": branch_counts},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -113,6 +113,15 @@ int ipf_info(const ipf_ctx *ctx, int64_t out[8]);
 int ipf_residual(ipf_ctx *ctx, const double *u_dev, double *r_dev);
 int ipf_residual_host(ipf_ctx *ctx, const double *u_host, double *r_host);
 
+/* ipf_residual with a HIP event pair around every kernel launch on the
+ * context's stream; waits, then writes the duration of each launch in
+ * milliseconds to launch_ms (launch 0: r = f_lin copy, then one per element
+ * group in launch order) and the group's point count to launch_pts (may be
+ * NULL).  Returns the number of launches (>= 0) or a negative IPF_E* code.
+ * Measurement aid; the plain entry point records no events. */
+int ipf_residual_timed(ipf_ctx *ctx, const double *u_dev, double *r_dev, double *launch_ms, int64_t *launch_pts,
+                       int max_launches);
+
 /* ---- the reference's separate steps (same results, unfused) ---- */
 
 /* y = B x (opt == 0; x [cols], y [rows]) or y = -B^T x (opt != 0; x [rows],

@@ -39,8 +39,8 @@ def library_path():
 
 
 def build(verbose=False):
-    from . import build as _b
-    return _b.build(verbose=verbose)
+    from . import buildlib
+    return buildlib.build(verbose=verbose)
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -61,7 +61,7 @@ def lib():
     if _lib is not None:
         return _lib
     if not os.path.exists(_LIB_PATH):
-        raise IPFError(f"{_LIB_PATH} is missing: build it with `python -m intpointsfem_amd.build` "
+        raise IPFError(f"{_LIB_PATH} is missing: build it with `python -m intpointsfem_amd.buildlib` "
                        "(there is no CPU fallback)")
     L = ctypes.CDLL(_LIB_PATH)
     sig = {
@@ -80,6 +80,7 @@ def lib():
         "ipf_info": (_int, [_vp, _lp]),
         "ipf_residual": (_int, [_vp, _vp, _vp]),
         "ipf_residual_host": (_int, [_vp, _dp, _dp]),
+        "ipf_residual_timed": (_int, [_vp, _vp, _vp, _dp, _lp, _int]),
         "ipf_multiply_vector": (_int, [_vp, _vp, _vp, _int]),
         "ipf_multiply": (_int, [_vp, _vp, _vp]),
         "ipf_compute_sigma": (_int, [_vp, _vp, _vp]),
@@ -276,6 +277,15 @@ class DeviceContext:
     # ---- compute on device pointers (torch tensors or raw ints)
     def residual(self, u, r):
         self._c(lib().ipf_residual(self._h, _ptr(u), _ptr(r)), "ipf_residual")
+
+    def residual_timed(self, u, r, max_launches=64):
+        """Residual with per-launch HIP events; returns (ms[n], points[n])."""
+        ms = np.zeros(max_launches)
+        pts = np.zeros(max_launches, dtype=np.int64)
+        n = lib().ipf_residual_timed(self._h, _ptr(u), _ptr(r), _p(ms, _dp), _p(pts, _lp), max_launches)
+        if n < 0:
+            _check(n, self._h, "ipf_residual_timed")
+        return ms[:n], pts[:n]
 
     def residual_host(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)

@@ -568,6 +568,41 @@ int ipf_residual(ipf_ctx* ctx, const double* u_dev, double* r_dev) {
     return launch_status(ctx, "residual");
 }
 
+int ipf_residual_timed(ipf_ctx* ctx, const double* u_dev, double* r_dev, double* launch_ms, int64_t* launch_pts,
+                       int max_launches) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!u_dev || !r_dev || !launch_ms) return fail(ctx, IPF_EINVAL, "ipf_residual_timed: NULL pointer");
+    const int n = 1 + (int)ctx->groups.size();
+    if (max_launches < n) return fail(ctx, IPF_EINVAL, "ipf_residual_timed: need room for %d launches", n);
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<hipEvent_t> ev(n + 1, nullptr);
+    for (auto& e : ev) IPF_HIP(ctx, hipEventCreate(&e));
+    const PointState st = point_state(ctx);
+    IPF_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+    launch_copy(ctx->neq, ctx->dFlin, r_dev, ctx->stream);
+    IPF_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+    for (size_t i = 0; i < ctx->groups.size(); i++) {
+        const auto& g = ctx->groups[i];
+        launch_residual(g.shape, ctx->update_mem, group_dev(ctx, g), u_dev, r_dev, st, ctx->mat, ctx->stream);
+        IPF_HIP(ctx, hipEventRecord(ev[i + 2], ctx->stream));
+    }
+    rc = launch_status(ctx, "residual");
+    if (rc == IPF_OK) {
+        IPF_HIP(ctx, hipEventSynchronize(ev[n]));
+        for (int i = 0; i < n; i++) {
+            float ms = 0.f;
+            IPF_HIP(ctx, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            launch_ms[i] = ms;
+            if (launch_pts)
+                launch_pts[i] = i == 0 ? 0
+                                       : (int64_t)ctx->groups[i - 1].elems.size() * shape_nip(ctx->groups[i - 1].shape);
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc == IPF_OK ? n : rc;
+}
+
 int ipf_residual_host(ipf_ctx* ctx, const double* u_host, double* r_host) {
     int rc = check_ready(ctx, true);
     if (rc) return rc;
