@@ -29,16 +29,28 @@ struct Cfg {
     static constexpr int NB = 3 * CE;            // B values per point
     static constexpr bool KEEP_B = (NB <= 64);   // keep B in registers across the return map
     static constexpr int WPB = (CE <= 18) ? 4 : 2;  // wavefronts per workgroup
+    // waves per SIMD the register allocation must allow (Q2: 168 VGPRs)
+    static constexpr int MIN_WAVES = (CE <= 8) ? 4 : (CE <= 18) ? 3 : 1;
 };
+
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+// Single-use streams (B, DoF indexes, weights, committed state) are read with
+// non-temporal loads so they do not evict u and r, which every color launch
+// reads again, from L2 and the Infinity Cache.
+template <class T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
 
 // Loads the NB values of point `lane` of tile `tile` (pairs of doubles).
 template <int SHAPE>
 __device__ __forceinline__ void load_point_B(const double* __restrict__ Bg, int64_t tile, int lane, double* b) {
     using C = Cfg<SHAPE>;
-    const double2* Bt = reinterpret_cast<const double2*>(Bg + tile * (int64_t)(C::NB * C::TP));
+    const dvec2* Bt = reinterpret_cast<const dvec2*>(Bg + tile * (int64_t)(C::NB * C::TP));
 #pragma unroll
     for (int m2 = 0; m2 < C::NB / 2; m2++) {
-        const double2 v = Bt[m2 * C::TP + lane];
+        const dvec2 v = ld_stream(&Bt[m2 * C::TP + lane]);
         b[2 * m2] = v.x;
         b[2 * m2 + 1] = v.y;
     }
@@ -89,10 +101,59 @@ __device__ __forceinline__ void reduce_scatter(const double* L, const GroupDev& 
     }
 }
 
+// Reduction tasks (element e2 of the tile, column j) owned by a lane: lane and
+// lane + 64 (ET * CE <= 128 for every shape).  Within one color launch each
+// DoF belongs to exactly one task, so r can be read at the start of the tile
+// and written at the end without any other wave touching it in between.
+template <int SHAPE>
+struct RTasks {
+    int dof[2];
+    double val[2];
+};
+
+template <int SHAPE>
+__device__ __forceinline__ void prefetch_r(const GroupDev& g, int64_t tile, int lane, const double* __restrict__ r,
+                                           RTasks<SHAPE>& t) {
+    using C = Cfg<SHAPE>;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int task = lane + k * kWave;
+        t.dof[k] = -1;
+        t.val[k] = 0.0;
+        if (task < C::ET * C::CE) {
+            const int e2 = task / C::CE;
+            const int j = task - e2 * C::CE;
+            const int64_t elem = tile * C::ET + e2;
+            if (elem < g.nelem) {
+                t.dof[k] = ld_stream(&g.dof[elem * C::CE + j]);
+                t.val[k] = r[t.dof[k]];
+            }
+        }
+    }
+}
+
+template <int SHAPE>
+__device__ __forceinline__ void reduce_store(const double* L, int lane, const RTasks<SHAPE>& t,
+                                             double* __restrict__ r) {
+    using C = Cfg<SHAPE>;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        if (t.dof[k] < 0) continue;
+        const int task = lane + k * kWave;
+        const int e2 = task / C::CE;
+        const int j = task - e2 * C::CE;
+        double f = L[(e2 * C::NIP) * C::CE + j];
+#pragma unroll
+        for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e2 * C::NIP + ip) * C::CE + j];
+        r[t.dof[k]] = t.val[k] + (-f);
+    }
+}
+
 template <int SHAPE, bool COMMIT>
-__global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave)
+__global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave, Cfg<SHAPE>::MIN_WAVES)
 k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, PointState st, MatConst m) {
     using C = Cfg<SHAPE>;
+    static_assert(C::ET * C::CE <= 2 * kWave, "two reduction tasks per lane");
     __shared__ double lds[C::WPB][C::TP * C::CE];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
@@ -103,15 +164,7 @@ k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, Poi
     const bool slot = tile_ok && lane < C::TP;
     const bool valid = slot && elem < g.nelem;
 
-    double ue[C::CE];
-    if (valid) {
-        const int* d = g.dof + elem * C::CE;
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) ue[j] = u[d[j]];
-    } else {
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) ue[j] = 0.0;
-    }
+    // B stream first: the largest and independent loads
     double b[C::NB];
     if (slot) {
         load_point_B<SHAPE>(g.B, tile, lane, b);
@@ -119,16 +172,31 @@ k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, Poi
 #pragma unroll
         for (int k = 0; k < C::NB; k++) b[k] = 0.0;
     }
+    double ue[C::CE];
+    if (valid) {
+        const int* d = g.dof + elem * C::CE;
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) ue[j] = u[ld_stream(&d[j])];
+    } else {
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) ue[j] = 0.0;
+    }
+    RTasks<SHAPE> rt;
+    if (tile_ok) {
+        prefetch_r<SHAPE>(g, tile, lane, r, rt);
+    } else {
+        rt.dof[0] = rt.dof[1] = -1;
+    }
     double e3[3];
     strain_of<SHAPE>(b, ue, e3);
 
     double ws[3] = {0.0, 0.0, 0.0};
     if (valid) {
         const int64_t P = g.pt0 + tile * C::TP + lane;
-        double ep[4] = {st.ep0[P], st.ep1[P], st.ep2[P], st.ep3[P]};
-        double alpha = st.alpha[P];
+        double ep[4] = {ld_stream(&st.ep0[P]), ld_stream(&st.ep1[P]), ld_stream(&st.ep2[P]), ld_stream(&st.ep3[P])};
+        double alpha = ld_stream(&st.alpha[P]);
         int mt, br;
-        point_sigma<false>(e3, ep, alpha, mt, br, m, st.w[P], ws, nullptr);
+        point_sigma<false>(e3, ep, alpha, mt, br, m, ld_stream(&st.w[P]), ws, nullptr);
         if (COMMIT) {
             st.ep0[P] = ep[0];
             st.ep1[P] = ep[1];
@@ -143,7 +211,7 @@ k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, Poi
     double* L = lds[wave];
     if (slot) bt_partial<SHAPE>(b, ws, L + lane * C::CE);
     __syncthreads();
-    if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, r);
+    reduce_store<SHAPE>(L, lane, rt, r);
 }
 
 template <int SHAPE>
