@@ -138,6 +138,38 @@ int ipf_compute_sigma_dep(ipf_ctx *ctx, const double *strain_dev, double *wsigma
 /* res = -B^T wsigma, colored accumulation   TPZNumericalIntegrator::MultiplyTranspose (TPZNumericalIntegrator.cpp:54-105) */
 int ipf_multiply_transpose(ipf_ctx *ctx, const double *wsigma_dev, double *res_dev);
 
+/* ---- tangent, linear solve and Newton (the caller side of the path) ---- */
+
+/* Symmetry / Dirichlet DoFs (1 = fixed) used by the solver entry points,
+ * host array [neq].  The reference imposes them with NeoPZ penalty BCs; here
+ * they are eliminated (fixed DoFs keep their value, their residual is ignored). */
+int ipf_set_dirichlet_mask(ipf_ctx *ctx, const signed char *mask_host);
+
+/* Residual as ipf_residual plus the consistent tangent w*Dep of every point,
+ * kept on the device for ipf_tangent_apply / ipf_cg_solve.
+ * KAssembly's Multiply -> ComputeSigmaDep -> MultiplyTranspose
+ * (TPZNumericalIntegrator.cpp:185-194); the element matrices of
+ * MatrixAssembleKernel (KernelsMatrixAssemble.h:24-56) are applied matrix-free. */
+int ipf_residual_tangent(ipf_ctx *ctx, const double *u_dev, double *r_dev);
+
+/* y = K v with K = sum_e B_e^T (w Dep) B_e from the last ipf_residual_tangent
+ * (all DoFs, no mask), device pointers [neq]. */
+int ipf_tangent_apply(ipf_ctx *ctx, const double *v_dev, double *y_dev);
+
+/* Jacobi-preconditioned CG on the free DoFs: K x = b, x0 = 0, until
+ * ||r|| <= rtol * ||b|| or maxit iterations.  Device pointers [neq]; fixed DoFs
+ * of x are zero.  iters / relres may be NULL. */
+int ipf_cg_solve(ipf_ctx *ctx, const double *b_dev, double *x_dev, double rtol, int maxit, int *iters,
+                 double *relres);
+
+/* Newton for the current load (f_lin): u <- u + du until the free-DoF residual
+ * satisfies ||r|| <= rtol * ||r0|| (r0: residual at the entry u), then commits
+ * the state at the converged u (AcceptPseudoTimeStepSolution, main.cpp:487-522).
+ * Solution / Newton loop of main.cpp:160-262 with CG in place of LDL^T.
+ * stats[0..7] = Newton iterations, converged (0/1), ||r||/||r0||, ||r0||,
+ * ||r||, total CG iterations, worst CG relative residual, 0. */
+int ipf_newton(ipf_ctx *ctx, double *u_dev, double rtol, int maxit, double cg_rtol, int cg_maxit, double *stats);
+
 /* ---- integration-point state (TPZConstitutiveLawProcessor.h:53-63) ---- */
 
 /* Copies the committed state to host arrays (any may be NULL):

@@ -86,6 +86,11 @@ def lib():
         "ipf_compute_sigma": (_int, [_vp, _vp, _vp]),
         "ipf_compute_sigma_dep": (_int, [_vp, _vp, _vp, _vp]),
         "ipf_multiply_transpose": (_int, [_vp, _vp, _vp]),
+        "ipf_set_dirichlet_mask": (_int, [_vp, _bp]),
+        "ipf_residual_tangent": (_int, [_vp, _vp, _vp]),
+        "ipf_tangent_apply": (_int, [_vp, _vp, _vp]),
+        "ipf_cg_solve": (_int, [_vp, _vp, _vp, _d, _int, _ip, _dp]),
+        "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
         "ipf_get_state": (_int, [_vp, _dp, _dp, _ip, _ip]),
         "ipf_set_state": (_int, [_vp, _dp, _dp]),
         # front-end
@@ -308,6 +313,31 @@ class DeviceContext:
 
     def multiply_transpose(self, wsigma, res):
         self._c(lib().ipf_multiply_transpose(self._h, _ptr(wsigma), _ptr(res)), "ipf_multiply_transpose")
+
+    # ---- tangent / solver
+    def set_dirichlet_mask(self, mask):
+        m = np.ascontiguousarray(mask, dtype=np.int8)
+        self._c(lib().ipf_set_dirichlet_mask(self._h, _p(m, _bp)), "ipf_set_dirichlet_mask")
+
+    def residual_tangent(self, u, r):
+        self._c(lib().ipf_residual_tangent(self._h, _ptr(u), _ptr(r)), "ipf_residual_tangent")
+
+    def tangent_apply(self, v, y):
+        self._c(lib().ipf_tangent_apply(self._h, _ptr(v), _ptr(y)), "ipf_tangent_apply")
+
+    def cg_solve(self, b, x, rtol=1e-10, maxit=10000):
+        it = ctypes.c_int(0)
+        rel = ctypes.c_double(0.0)
+        self._c(lib().ipf_cg_solve(self._h, _ptr(b), _ptr(x), rtol, maxit, ctypes.byref(it), ctypes.byref(rel)),
+                "ipf_cg_solve")
+        return it.value, rel.value
+
+    def newton(self, u, rtol=1e-10, maxit=50, cg_rtol=1e-10, cg_maxit=20000):
+        """Newton for the current load; commits the state. Returns a stats dict."""
+        st = np.zeros(8)
+        self._c(lib().ipf_newton(self._h, _ptr(u), rtol, maxit, cg_rtol, cg_maxit, _p(st, _dp)), "ipf_newton")
+        keys = ("iterations", "converged", "rel_residual", "r0", "r", "cg_iterations", "cg_worst_relres")
+        return dict(zip(keys, (float(v) for v in st[:7])))
 
     def get_state(self):
         n = self.info()["npts"]

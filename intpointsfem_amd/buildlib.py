@@ -15,6 +15,7 @@ OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = [
     "ipf_kernels.hip",
+    "ipf_solver.hip",
     "ipf_device.hip",
     "TPZCompMesh.cpp",
     "TPZIntPoints.cpp",

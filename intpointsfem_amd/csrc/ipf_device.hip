@@ -177,6 +177,23 @@ struct ipf_ctx {
     double* dU = nullptr;       // staging for host-pointer calls
     double* dR = nullptr;
     int64_t bytes = 0;
+
+    // tangent / solver workspace (allocated on first use)
+    double* dDep = nullptr;     // 9 * npad, SoA by new point index
+    double* dStrain = nullptr;  // 3 * npts, reference order
+    double* dSigw = nullptr;    // 3 * npts
+    signed char* dMask = nullptr;  // neq
+    double* dDiag = nullptr;
+    double* dCr = nullptr;
+    double* dCz = nullptr;
+    double* dCp = nullptr;
+    double* dCq = nullptr;
+    double* dNr = nullptr;      // Newton residual
+    double* dNd = nullptr;      // Newton correction
+    double* dPartial = nullptr; // dot partials
+    double* dScal = nullptr;    // CG scalars
+    bool have_mask = false;
+    bool have_tangent = false;
 };
 
 namespace {
@@ -227,9 +244,62 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dFlin);
     dev_free(ctx->dU);
     dev_free(ctx->dR);
+    dev_free(ctx->dDep);
+    dev_free(ctx->dStrain);
+    dev_free(ctx->dSigw);
+    dev_free(ctx->dMask);
+    dev_free(ctx->dDiag);
+    dev_free(ctx->dCr);
+    dev_free(ctx->dCz);
+    dev_free(ctx->dCp);
+    dev_free(ctx->dCq);
+    dev_free(ctx->dNr);
+    dev_free(ctx->dNd);
+    dev_free(ctx->dPartial);
+    dev_free(ctx->dScal);
     ctx->groups.clear();
     ctx->have_data = false;
+    ctx->have_mask = false;
+    ctx->have_tangent = false;
     ctx->bytes = 0;
+}
+
+// Solver workspace, allocated once per transferred data set.
+int ensure_solver_buffers(ipf_ctx* ctx) {
+    if (ctx->dDep) return IPF_OK;
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->dDep, 9 * ctx->npad)) || (rc = dev_alloc(ctx, &ctx->dStrain, 3 * ctx->npts)) ||
+        (rc = dev_alloc(ctx, &ctx->dSigw, 3 * ctx->npts)) || (rc = dev_alloc(ctx, &ctx->dDiag, ctx->neq)) ||
+        (rc = dev_alloc(ctx, &ctx->dCr, ctx->neq)) || (rc = dev_alloc(ctx, &ctx->dCz, ctx->neq)) ||
+        (rc = dev_alloc(ctx, &ctx->dCp, ctx->neq)) || (rc = dev_alloc(ctx, &ctx->dCq, ctx->neq)) ||
+        (rc = dev_alloc(ctx, &ctx->dNr, ctx->neq)) || (rc = dev_alloc(ctx, &ctx->dNd, ctx->neq)) ||
+        (rc = dev_alloc(ctx, &ctx->dPartial, 1024)) || (rc = dev_alloc(ctx, &ctx->dScal, 16))) {
+        return rc;
+    }
+    IPF_HIP(ctx, hipMemset(ctx->dDep, 0, sizeof(double) * 9 * ctx->npad));
+    IPF_HIP(ctx, hipMemset(ctx->dScal, 0, sizeof(double) * 16));
+    if (!ctx->dMask) {
+        if ((rc = dev_alloc(ctx, &ctx->dMask, ctx->neq))) return rc;
+        IPF_HIP(ctx, hipMemset(ctx->dMask, 0, ctx->neq));
+    }
+    return IPF_OK;
+}
+
+int read_scalar(ipf_ctx* ctx, int slot, double* out) {
+    IPF_HIP(ctx, hipMemcpyAsync(out, ctx->dScal + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return IPF_OK;
+}
+
+// ||x|| over the free DoFs (uses the CG q vector as scratch)
+int masked_norm(ipf_ctx* ctx, const double* x, double* out) {
+    launch_mask_copy(ctx->neq, ctx->dMask, x, ctx->dCq, ctx->stream);
+    launch_dot(ctx->neq, ctx->dCq, ctx->dCq, ctx->dPartial, dot_blocks(ctx->neq), ctx->dScal + 6, ctx->stream);
+    double v;
+    int rc = read_scalar(ctx, 6, &v);
+    if (rc) return rc;
+    *out = std::sqrt(v);
+    return IPF_OK;
 }
 
 int shape_of(int nip, int ce) {
@@ -669,6 +739,145 @@ int ipf_multiply_transpose(ipf_ctx* ctx, const double* wsigma_dev, double* res_d
     for (const auto& g : ctx->groups)
         launch_transpose(g.shape, group_dev(ctx, g), wsigma_dev, res_dev, ctx->dOrig, ctx->stream);
     return launch_status(ctx, "multiply_transpose");
+}
+
+int ipf_set_dirichlet_mask(ipf_ctx* ctx, const signed char* mask_host) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!mask_host) return fail(ctx, IPF_EINVAL, "ipf_set_dirichlet_mask: NULL");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->dMask && (rc = dev_alloc(ctx, &ctx->dMask, ctx->neq))) return rc;
+    IPF_HIP(ctx, hipMemcpy(ctx->dMask, mask_host, ctx->neq, hipMemcpyHostToDevice));
+    ctx->have_mask = true;
+    return IPF_OK;
+}
+
+int ipf_residual_tangent(ipf_ctx* ctx, const double* u_dev, double* r_dev) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!u_dev || !r_dev) return fail(ctx, IPF_EINVAL, "ipf_residual_tangent: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_solver_buffers(ctx))) return rc;
+    for (const auto& g : ctx->groups)
+        launch_multiply(g.shape, group_dev(ctx, g), u_dev, ctx->dStrain, ctx->dOrig, ctx->stream);
+    launch_sigma(true, ctx->update_mem, ctx->npad, ctx->dStrain, ctx->dSigw, nullptr, ctx->dOrig, point_state(ctx),
+                 ctx->mat, ctx->stream, ctx->dDep);
+    launch_copy(ctx->neq, ctx->dFlin, r_dev, ctx->stream);
+    for (const auto& g : ctx->groups)
+        launch_transpose(g.shape, group_dev(ctx, g), ctx->dSigw, r_dev, ctx->dOrig, ctx->stream);
+    ctx->have_tangent = true;
+    return launch_status(ctx, "residual_tangent");
+}
+
+int ipf_tangent_apply(ipf_ctx* ctx, const double* v_dev, double* y_dev) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!v_dev || !y_dev) return fail(ctx, IPF_EINVAL, "ipf_tangent_apply: NULL pointer");
+    if (!ctx->have_tangent) return fail(ctx, IPF_ESTATE, "no tangent: call ipf_residual_tangent first");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    launch_fill(ctx->neq, y_dev, 0.0, ctx->stream);
+    for (const auto& g : ctx->groups)
+        launch_tangent(g.shape, group_dev(ctx, g), v_dev, y_dev, ctx->dDep, ctx->npad, ctx->stream);
+    return launch_status(ctx, "tangent_apply");
+}
+
+int ipf_cg_solve(ipf_ctx* ctx, const double* b_dev, double* x_dev, double rtol, int maxit, int* iters,
+                 double* relres) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!b_dev || !x_dev) return fail(ctx, IPF_EINVAL, "ipf_cg_solve: NULL pointer");
+    if (!ctx->have_tangent) return fail(ctx, IPF_ESTATE, "no tangent: call ipf_residual_tangent first");
+    if (!(rtol > 0) || maxit < 1) return fail(ctx, IPF_EINVAL, "ipf_cg_solve: need rtol > 0 and maxit >= 1");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_solver_buffers(ctx))) return rc;
+    const int64_t n = ctx->neq;
+    const int nb = dot_blocks(n);
+    hipStream_t s = ctx->stream;
+    // Jacobi diagonal of the current tangent
+    launch_fill(n, ctx->dDiag, 0.0, s);
+    for (const auto& g : ctx->groups) launch_tangent_diag(g.shape, group_dev(ctx, g), ctx->dDiag, ctx->dDep, ctx->npad, s);
+    launch_fill(n, x_dev, 0.0, s);
+    launch_cg_init(n, ctx->dMask, b_dev, ctx->dCr, ctx->dDiag, ctx->dCz, ctx->dCp, s);
+    launch_dot(n, ctx->dCr, ctx->dCz, ctx->dPartial, nb, ctx->dScal + 0, s);
+    launch_dot(n, ctx->dCr, ctx->dCr, ctx->dPartial, nb, ctx->dScal + 3, s);
+    double bb;
+    if ((rc = read_scalar(ctx, 3, &bb))) return rc;
+    const double bnorm = std::sqrt(bb);
+    int it = 0;
+    double rel = 0.0;
+    if (bnorm > 0.0) {
+        const int check_every = 10;
+        for (it = 1; it <= maxit; it++) {
+            // q = K p on the free DoFs
+            launch_fill(n, ctx->dCq, 0.0, s);
+            for (const auto& g : ctx->groups)
+                launch_tangent(g.shape, group_dev(ctx, g), ctx->dCp, ctx->dCq, ctx->dDep, ctx->npad, s);
+            launch_cg_mask(n, ctx->dMask, ctx->dCq, s);
+            launch_dot(n, ctx->dCp, ctx->dCq, ctx->dPartial, nb, ctx->dScal + 1, s);
+            launch_cg_scalars(ctx->dScal, 0, s);
+            launch_cg_update(n, ctx->dScal, ctx->dCp, ctx->dCq, x_dev, ctx->dCr, ctx->dDiag, ctx->dCz, s);
+            launch_dot(n, ctx->dCr, ctx->dCz, ctx->dPartial, nb, ctx->dScal + 2, s);
+            launch_cg_scalars(ctx->dScal, 1, s);
+            launch_cg_direction(n, ctx->dScal, ctx->dCz, ctx->dCp, s);
+            if (it % check_every == 0 || it == maxit) {
+                launch_dot(n, ctx->dCr, ctx->dCr, ctx->dPartial, nb, ctx->dScal + 3, s);
+                double rr;
+                if ((rc = launch_status(ctx, "cg")) || (rc = read_scalar(ctx, 3, &rr))) return rc;
+                rel = std::sqrt(rr) / bnorm;
+                if (!(rel > rtol)) break;  // also stops on NaN
+            }
+        }
+        if (it > maxit) it = maxit;
+    }
+    if (iters) *iters = it;
+    if (relres) *relres = rel;
+    return launch_status(ctx, "cg");
+}
+
+int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rtol, int cg_maxit, double* stats) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!u_dev) return fail(ctx, IPF_EINVAL, "ipf_newton: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_solver_buffers(ctx))) return rc;
+    const bool commit_flag = ctx->update_mem;
+    ctx->update_mem = false;
+    double r0 = 0.0, rn = 0.0, worst_cg = 0.0;
+    int it = 0, cg_total = 0;
+    bool converged = false;
+    for (it = 0; it <= maxit; it++) {
+        if ((rc = ipf_residual_tangent(ctx, u_dev, ctx->dNr))) return rc;
+        if ((rc = masked_norm(ctx, ctx->dNr, &rn))) return rc;
+        if (it == 0) r0 = rn;
+        if (rn <= rtol * r0 || rn == 0.0) {
+            converged = true;
+            break;
+        }
+        if (it == maxit) break;
+        int cgi = 0;
+        double cgr = 0.0;
+        if ((rc = ipf_cg_solve(ctx, ctx->dNr, ctx->dNd, cg_rtol, cg_maxit, &cgi, &cgr))) return rc;
+        cg_total += cgi;
+        worst_cg = std::max(worst_cg, cgr);
+        launch_axpy(ctx->neq, 1.0, ctx->dNd, u_dev, ctx->stream);
+    }
+    // AcceptPseudoTimeStepSolution: one residual with UpdateMem on
+    ctx->update_mem = true;
+    rc = ipf_residual(ctx, u_dev, ctx->dNr);
+    ctx->update_mem = commit_flag;
+    if (rc) return rc;
+    if ((rc = ipf_synchronize(ctx))) return rc;
+    if (stats) {
+        stats[0] = it;
+        stats[1] = converged ? 1.0 : 0.0;
+        stats[2] = r0 > 0 ? rn / r0 : 0.0;
+        stats[3] = r0;
+        stats[4] = rn;
+        stats[5] = cg_total;
+        stats[6] = worst_cg;
+        stats[7] = 0.0;
+    }
+    return IPF_OK;
 }
 
 int ipf_get_state(ipf_ctx* ctx, double* plastic_strain_host, double* alpha_host, int* mtype_host, int* branch_host) {

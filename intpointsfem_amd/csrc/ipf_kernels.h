@@ -56,8 +56,32 @@ void launch_multiply(int shape, const GroupDev& g, const double* u, double* stra
 void launch_transpose(int shape, const GroupDev& g, const double* wsigma, double* r, const int64_t* orig_of_new,
                       hipStream_t stream);
 void launch_sigma(bool dep, bool commit, int64_t npad, const double* strain, double* wsigma, double* wdep,
-                  const int64_t* orig_of_new, const PointState& st, const MatConst& m, hipStream_t stream);
+                  const int64_t* orig_of_new, const PointState& st, const MatConst& m, hipStream_t stream,
+                  double* dep_soa = nullptr);
 void launch_copy(int64_t n, const double* src, double* dst, hipStream_t stream);
 void launch_fill(int64_t n, double* dst, double value, hipStream_t stream);
+
+// Tangent operator (matrix-free): y += sum_e B_e^T (w Dep) B_e v_e for one
+// element group, w Dep stored SoA as dep_soa[k * npad + P], k = 3*row + col.
+void launch_tangent(int shape, const GroupDev& g, const double* v, double* y, const double* dep_soa, int64_t npad,
+                    hipStream_t stream);
+// diag += diagonal of the same operator.
+void launch_tangent_diag(int shape, const GroupDev& g, double* diag, const double* dep_soa, int64_t npad,
+                         hipStream_t stream);
+
+// Vector kernels of the preconditioned CG (ipf_solver.hip).  `scal` is a small
+// device array of scalars (see ipf_solver.hip for the slots).
+void launch_dot(int64_t n, const double* x, const double* y, double* partial, int nblocks, double* out,
+                hipStream_t stream);
+void launch_cg_init(int64_t n, const signed char* mask, const double* b, double* r, const double* diag, double* z,
+                    double* p, hipStream_t stream);
+void launch_cg_mask(int64_t n, const signed char* mask, double* q, hipStream_t stream);
+void launch_cg_update(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
+                      const double* diag, double* z, hipStream_t stream);
+void launch_cg_direction(int64_t n, const double* scal, const double* z, double* p, hipStream_t stream);
+void launch_cg_scalars(double* scal, int stage, hipStream_t stream);
+void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t stream);
+void launch_mask_copy(int64_t n, const signed char* mask, const double* src, double* dst, hipStream_t stream);
+int dot_blocks(int64_t n);
 
 }  // namespace ipf

@@ -271,7 +271,7 @@ k_transpose(GroupDev g, const double* __restrict__ x, double* __restrict__ r, co
 template <bool DEP>
 __global__ void __launch_bounds__(256)
 k_sigma(int64_t npad, const double* __restrict__ strain, double* __restrict__ wsigma, double* __restrict__ wdep,
-        const int64_t* __restrict__ orig, PointState st, MatConst m, int commit) {
+        const int64_t* __restrict__ orig, PointState st, MatConst m, int commit, double* __restrict__ dep_soa) {
     const int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= npad) return;
     const int64_t o = orig[P];
@@ -286,8 +286,14 @@ k_sigma(int64_t npad, const double* __restrict__ strain, double* __restrict__ ws
     wsigma[3 * o + 1] = ws[1];
     wsigma[3 * o + 2] = ws[2];
     if (DEP) {
+        if (wdep) {
 #pragma unroll
-        for (int i = 0; i < 9; i++) wdep[9 * o + i] = d9[i];
+            for (int i = 0; i < 9; i++) wdep[9 * o + i] = d9[i];
+        }
+        if (dep_soa) {
+#pragma unroll
+            for (int i = 0; i < 9; i++) dep_soa[i * npad + P] = d9[i];
+        }
     }
     if (commit) {
         st.ep0[P] = ep[0];
@@ -298,6 +304,117 @@ k_sigma(int64_t npad, const double* __restrict__ strain, double* __restrict__ ws
         st.mtype[P] = (signed char)mt;
         st.branch[P] = (signed char)br;
     }
+}
+
+// y += sum_e B_e^T (w Dep) B_e v_e over one element group (one color): the
+// matrix-free tangent of the Newton / CG solve.  Same tiling as k_residual,
+// with the return map replaced by the stored 3x3 w*Dep of each point.
+template <int SHAPE>
+__global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave, Cfg<SHAPE>::MIN_WAVES)
+k_tangent(GroupDev g, const double* __restrict__ v, double* __restrict__ y, const double* __restrict__ dep,
+          int64_t npad) {
+    using C = Cfg<SHAPE>;
+    __shared__ double lds[C::WPB][C::TP * C::CE];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int64_t tile = (int64_t)blockIdx.x * C::WPB + wave;
+    const bool tile_ok = tile < g.ntiles;
+    const int el = lane / C::NIP;
+    const int64_t elem = tile * C::ET + el;
+    const bool slot = tile_ok && lane < C::TP;
+    const bool valid = slot && elem < g.nelem;
+    double b[C::NB];
+    if (slot) {
+        load_point_B<SHAPE>(g.B, tile, lane, b);
+    } else {
+#pragma unroll
+        for (int k = 0; k < C::NB; k++) b[k] = 0.0;
+    }
+    double ve[C::CE];
+    if (valid) {
+        const int* d = g.dof + elem * C::CE;
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) ve[j] = v[ld_stream(&d[j])];
+    } else {
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) ve[j] = 0.0;
+    }
+    RTasks<SHAPE> rt;
+    if (tile_ok) {
+        prefetch_r<SHAPE>(g, tile, lane, y, rt);
+    } else {
+        rt.dof[0] = rt.dof[1] = -1;
+    }
+    double e3[3];
+    strain_of<SHAPE>(b, ve, e3);
+    double x[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+        const int64_t P = g.pt0 + tile * C::TP + lane;
+        double D[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) D[i] = ld_stream(&dep[i * npad + P]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double a = D[3 * k] * e3[0];
+            a = a + D[3 * k + 1] * e3[1];
+            a = a + D[3 * k + 2] * e3[2];
+            // reduce_store adds -(sum of partials); y accumulates +K v
+            x[k] = -a;
+        }
+    }
+    if (!C::KEEP_B && slot) load_point_B<SHAPE>(g.B, tile, lane, b);
+    double* L = lds[wave];
+    if (slot) bt_partial<SHAPE>(b, x, L + lane * C::CE);
+    __syncthreads();
+    reduce_store<SHAPE>(L, lane, rt, y);
+}
+
+// diag += diagonal of sum_e B_e^T (w Dep) B_e: per point and column j,
+// sum_{k,l} B[k][j] D[k][l] B[l][j].
+template <int SHAPE>
+__global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave)
+k_tangent_diag(GroupDev g, double* __restrict__ diag, const double* __restrict__ dep, int64_t npad) {
+    using C = Cfg<SHAPE>;
+    __shared__ double lds[C::WPB][C::TP * C::CE];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int64_t tile = (int64_t)blockIdx.x * C::WPB + wave;
+    const bool tile_ok = tile < g.ntiles;
+    const int el = lane / C::NIP;
+    const int64_t elem = tile * C::ET + el;
+    const bool slot = tile_ok && lane < C::TP;
+    const bool valid = slot && elem < g.nelem;
+    RTasks<SHAPE> rt;
+    if (tile_ok) {
+        prefetch_r<SHAPE>(g, tile, lane, diag, rt);
+    } else {
+        rt.dof[0] = rt.dof[1] = -1;
+    }
+    double* L = lds[wave];
+    if (slot) {
+        double D[9];
+        const int64_t P = g.pt0 + tile * C::TP + lane;
+#pragma unroll
+        for (int i = 0; i < 9; i++) D[i] = valid ? dep[i * npad + P] : 0.0;
+        const dvec2* Bt = reinterpret_cast<const dvec2*>(g.B + tile * (int64_t)(C::NB * C::TP));
+        for (int j = 0; j < C::CE; j++) {
+            double bj[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int mm = k * C::CE + j;
+                const dvec2 pr = Bt[(mm >> 1) * C::TP + lane];
+                bj[k] = (mm & 1) ? pr.y : pr.x;
+            }
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) s = s + bj[k] * D[3 * k + l] * bj[l];
+            L[lane * C::CE + j] = -s;
+        }
+    }
+    __syncthreads();
+    reduce_store<SHAPE>(L, lane, rt, diag);
 }
 
 __global__ void k_copy(int64_t n, const double* __restrict__ src, double* __restrict__ dst) {
@@ -378,15 +495,55 @@ void launch_transpose(int shape, const GroupDev& g, const double* x, double* r, 
 }
 
 void launch_sigma(bool dep, bool commit, int64_t npad, const double* strain, double* wsigma, double* wdep,
-                  const int64_t* orig, const PointState& st, const MatConst& m, hipStream_t s) {
+                  const int64_t* orig, const PointState& st, const MatConst& m, hipStream_t s, double* dep_soa) {
     if (npad == 0) return;
     const dim3 grid((unsigned)((npad + 255) / 256));
     if (dep)
         hipLaunchKernelGGL(k_sigma<true>, grid, dim3(256), 0, s, npad, strain, wsigma, wdep, orig, st, m,
-                           commit ? 1 : 0);
+                           commit ? 1 : 0, dep_soa);
     else
         hipLaunchKernelGGL(k_sigma<false>, grid, dim3(256), 0, s, npad, strain, wsigma, wdep, orig, st, m,
-                           commit ? 1 : 0);
+                           commit ? 1 : 0, dep_soa);
+}
+
+void launch_tangent(int shape, const GroupDev& g, const double* v, double* y, const double* dep, int64_t npad,
+                    hipStream_t s) {
+    if (g.ntiles == 0) return;
+    switch (shape) {
+        case SHAPE_Q1:
+            hipLaunchKernelGGL(k_tangent<SHAPE_Q1>, tile_grid<SHAPE_Q1>(g), dim3(Cfg<SHAPE_Q1>::WPB * kWave), 0, s,
+                               g, v, y, dep, npad);
+            break;
+        case SHAPE_Q2:
+            hipLaunchKernelGGL(k_tangent<SHAPE_Q2>, tile_grid<SHAPE_Q2>(g), dim3(Cfg<SHAPE_Q2>::WPB * kWave), 0, s,
+                               g, v, y, dep, npad);
+            break;
+        case SHAPE_Q3:
+            hipLaunchKernelGGL(k_tangent<SHAPE_Q3>, tile_grid<SHAPE_Q3>(g), dim3(Cfg<SHAPE_Q3>::WPB * kWave), 0, s,
+                               g, v, y, dep, npad);
+            break;
+        default: break;
+    }
+}
+
+void launch_tangent_diag(int shape, const GroupDev& g, double* diag, const double* dep, int64_t npad,
+                         hipStream_t s) {
+    if (g.ntiles == 0) return;
+    switch (shape) {
+        case SHAPE_Q1:
+            hipLaunchKernelGGL(k_tangent_diag<SHAPE_Q1>, tile_grid<SHAPE_Q1>(g), dim3(Cfg<SHAPE_Q1>::WPB * kWave),
+                               0, s, g, diag, dep, npad);
+            break;
+        case SHAPE_Q2:
+            hipLaunchKernelGGL(k_tangent_diag<SHAPE_Q2>, tile_grid<SHAPE_Q2>(g), dim3(Cfg<SHAPE_Q2>::WPB * kWave),
+                               0, s, g, diag, dep, npad);
+            break;
+        case SHAPE_Q3:
+            hipLaunchKernelGGL(k_tangent_diag<SHAPE_Q3>, tile_grid<SHAPE_Q3>(g), dim3(Cfg<SHAPE_Q3>::WPB * kWave),
+                               0, s, g, diag, dep, npad);
+            break;
+        default: break;
+    }
 }
 
 void launch_copy(int64_t n, const double* src, double* dst, hipStream_t s) {

@@ -1,0 +1,154 @@
+// ipf_solver.hip -- vector kernels of the preconditioned conjugate gradient
+// used for the Newton steps (the reference solves with a direct LDL^T,
+// main.cpp:283-285; its unused TPZCudaCalls::SolveCG, TPZCudaCalls.cu:213-290,
+// is a plain CG).  Dot products are FP64 with a fixed reduction order (block
+// partials in order, then one block), so repeated solves are bit-identical.
+// CG scalars live in a small device array so an iteration needs no host sync:
+//   scal[0] rz = r.z    scal[1] pq = p.q    scal[2] rz_new
+//   scal[3] rr = r.r    scal[4] alpha      scal[5] beta
+#include "ipf_kernels.h"
+
+namespace ipf {
+
+namespace {
+
+constexpr int kDotThreads = 256;
+constexpr int kDotMaxBlocks = 1024;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+__global__ void __launch_bounds__(kDotThreads)
+k_dot_partial(int64_t n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ partial) {
+    __shared__ double sh[kDotThreads];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kDotThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kDotThreads)
+        s = s + x[i] * y[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kDotThreads)
+k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out) {
+    __shared__ double sh[kDotThreads];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kDotThreads) s = s + partial[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+__device__ __forceinline__ double precond(double r, double d) { return d != 0.0 ? r / d : 0.0; }
+
+__global__ void k_cg_init(int64_t n, const signed char* __restrict__ mask, const double* __restrict__ b,
+                          double* __restrict__ r, const double* __restrict__ diag, double* __restrict__ z,
+                          double* __restrict__ p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ri = mask[i] ? 0.0 : b[i];
+    r[i] = ri;
+    const double zi = precond(ri, diag[i]);
+    z[i] = zi;
+    p[i] = zi;
+}
+
+__global__ void k_cg_mask(int64_t n, const signed char* __restrict__ mask, double* __restrict__ q) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && mask[i]) q[i] = 0.0;
+}
+
+__global__ void k_cg_update(int64_t n, const double* __restrict__ scal, const double* __restrict__ p,
+                            const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
+                            const double* __restrict__ diag, double* __restrict__ z) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double alpha = scal[4];
+    x[i] = x[i] + alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    z[i] = precond(ri, diag[i]);
+}
+
+__global__ void k_cg_direction(int64_t n, const double* __restrict__ scal, const double* __restrict__ z,
+                               double* __restrict__ p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    p[i] = z[i] + scal[5] * p[i];
+}
+
+// stage 0: alpha = rz / pq;  stage 1: beta = rz_new / rz, rz = rz_new
+__global__ void k_cg_scalars(double* scal, int stage) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (stage == 0) {
+        scal[4] = scal[1] != 0.0 ? scal[0] / scal[1] : 0.0;
+    } else {
+        scal[5] = scal[0] != 0.0 ? scal[2] / scal[0] : 0.0;
+        scal[0] = scal[2];
+    }
+}
+
+__global__ void k_axpy(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = y[i] + a * x[i];
+}
+
+__global__ void k_mask_copy(int64_t n, const signed char* __restrict__ mask, const double* __restrict__ src,
+                            double* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = mask[i] ? 0.0 : src[i];
+}
+
+inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace
+
+int dot_blocks(int64_t n) {
+    int64_t b = (n + kDotThreads - 1) / kDotThreads;
+    if (b > kDotMaxBlocks) b = kDotMaxBlocks;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+void launch_dot(int64_t n, const double* x, const double* y, double* partial, int nblocks, double* out,
+                hipStream_t s) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(nblocks), dim3(kDotThreads), 0, s, n, x, y, partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, nblocks, partial, out);
+}
+
+void launch_cg_init(int64_t n, const signed char* mask, const double* b, double* r, const double* diag, double* z,
+                    double* p, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_cg_init, grid1(n), dim3(256), 0, s, n, mask, b, r, diag, z, p);
+}
+
+void launch_cg_mask(int64_t n, const signed char* mask, double* q, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_cg_mask, grid1(n), dim3(256), 0, s, n, mask, q);
+}
+
+void launch_cg_update(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
+                      const double* diag, double* z, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_cg_update, grid1(n), dim3(256), 0, s, n, scal, p, q, x, r, diag, z);
+}
+
+void launch_cg_direction(int64_t n, const double* scal, const double* z, double* p, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_cg_direction, grid1(n), dim3(256), 0, s, n, scal, z, p);
+}
+
+void launch_cg_scalars(double* scal, int stage, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1), 0, s, scal, stage);
+}
+
+void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_axpy, grid1(n), dim3(256), 0, s, n, a, x, y);
+}
+
+void launch_mask_copy(int64_t n, const signed char* mask, const double* src, double* dst, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_mask_copy, grid1(n), dim3(256), 0, s, n, mask, src, dst);
+}
+
+}  // namespace ipf

@@ -4,10 +4,14 @@ Quarter annulus r in [0.1, 4] m, theta in [0, pi/2], plane strain, E = 5000
 MPa, nu = 0.2, Mohr-Coulomb c = 10 MPa, phi = psi = 30 deg, perfect
 plasticity.  As in the reference driver (main.cpp:395-405) the far-field
 stress sigma0 = -50 MPa is the reference state: the mesh carries zero initial
-stress and the load is the increment of the wellbore pressure, applied as the
-normal traction sigma_n = pw - sigma0 on the inner arc (pw ramps -50 ->
--19.5 MPa over 10 load steps, so sigma_n = 3.05 k at step k); the outer arc
-carries no increment.  Symmetry: u_y = 0 on theta = 0, u_x = 0 on theta = pi/2.
+stress and the load is the increment of the wellbore pressure.  The
+reference's normal-stress boundary value is -(pw - sigma0) and it is the
+radial stress sigma_rr imposed on the wall (its IntPoints CPU profile,
+mathematica_notebooks/RK_Verification/ElastoPlasticIntPoints-CPU.csv:2, has
+sigma_rr(0.1) = -9.99 for the value -10 of main.cpp:400).  pw ramps -50 ->
+-19.5 MPa over 10 load steps, so sigma_rr(r_in) = -3.05 k at step k; the outer
+arc carries no increment.  Symmetry: u_y = 0 on theta = 0, u_x = 0 on
+theta = pi/2.
 """
 import math
 
@@ -39,9 +43,10 @@ def lame_parameters(E_=E, nu=NU):
 
 
 def normal_stress(step, nsteps=NSTEPS, pw_final=PW_FINAL, sigma0=SIGMA0):
-    """sigma_n on the inner arc at load step `step` (1-based)."""
+    """Radial stress imposed on the inner arc at load step `step` (1-based):
+    the reference's boundary value -(pw - sigma0) (main.cpp:399-400)."""
     pw = sigma0 + (pw_final - sigma0) * step / nsteps
-    return pw - sigma0
+    return -(pw - sigma0)
 
 
 def make_mesh(name_or_tuple, material=True, step=NSTEPS):
@@ -56,9 +61,9 @@ def make_mesh(name_or_tuple, material=True, step=NSTEPS):
 
 
 def lame_displacement(xy, sigma_n, scale=1.0, E_=E, nu=NU, r_in=R_IN, r_out=R_OUT):
-    """Linear-elastic thick-cylinder displacement for traction sigma_n*n_out on
-    the inner radius and none on the outer radius (plane strain), at nodes
-    `xy` [n, 2]; returns the DoF vector [2n] (x, y interleaved) times `scale`.
+    """Linear-elastic thick-cylinder displacement with sigma_rr(r_in) = sigma_n
+    and sigma_rr(r_out) = 0 (plane strain), at nodes `xy` [n, 2]; returns the
+    DoF vector [2n] (x, y interleaved) times `scale`.
     sigma_rr(r) = 2(l+m)A - 2mB/r^2, u_r = A r + B / r."""
     lam, mu = lame_parameters(E_, nu)
     # sigma_rr(r_in) = sigma_n (outward normal -e_r => t = sigma_rr * n), sigma_rr(r_out) = 0
