@@ -46,6 +46,12 @@ def main():
     ap.add_argument("--mesh", default="800x800-q2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's timed work")
+    ap.add_argument("--state", choices=("lame", "newton"), default="lame",
+                    help="lame: thick-cylinder displacement of the last load step, zero plastic strain; "
+                         "newton: converged 10-step Newton solution (||r||/||r0|| <= --newton-rtol), "
+                         "committed plastic state")
+    ap.add_argument("--newton-rtol", type=float, default=1e-8)
+    ap.add_argument("--cg-rtol", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the path runs on a single "
@@ -72,20 +78,45 @@ def main():
     stream = torch.cuda.current_stream(dev)
     ctx.set_stream(stream.cuda_stream)
 
-    # displacement: linear-elastic thick-cylinder field of the final load step
-    u_host = wb.lame_displacement(mesh.nodes(), wb.normal_stress(wb.NSTEPS), scale=1.0)
-    u = torch.as_tensor(u_host, dtype=torch.float64, device=dev)
     r = torch.zeros(neq, dtype=torch.float64, device=dev)
-
-    # branch mix of this state (one committed evaluation, then reset the state)
-    ctx.set_update_mem(True)
-    ctx.residual(u, r)
-    ctx.set_update_mem(False)
-    ctx.synchronize()
-    _, _, mt, br = ctx.get_state()
+    newton_info = None
+    if args.state == "lame":
+        # displacement: linear-elastic thick-cylinder field of the final load step
+        u_host = wb.lame_displacement(mesh.nodes(), wb.normal_stress(wb.NSTEPS), scale=1.0)
+        u = torch.as_tensor(u_host, dtype=torch.float64, device=dev)
+        # branch mix of this state (one committed evaluation, then reset the state)
+        ctx.set_update_mem(True)
+        ctx.residual(u, r)
+        ctx.set_update_mem(False)
+        ctx.synchronize()
+        _, _, mt, br = ctx.get_state()
+        ctx.set_state(np.zeros(6 * npts))
+        state_desc = "u = linear-elastic thick-cylinder displacement of load step 10, zero plastic strain"
+    else:
+        # 10 load steps of Newton + Jacobi-PCG on the GPU; the state is committed after each
+        ctx.set_dirichlet_mask(mesh.dirichlet_mask())
+        f_unit = sm.export_indexes()["rhs_linear"] / wb.normal_stress(wb.NSTEPS)
+        u = torch.zeros(neq, dtype=torch.float64, device=dev)
+        t1 = time.time()
+        steps = []
+        for k in range(1, wb.NSTEPS + 1):
+            ctx.set_rhs_linear(f_unit * wb.normal_stress(k))
+            st = ctx.newton(u, rtol=args.newton_rtol, maxit=60, cg_rtol=args.cg_rtol, cg_maxit=200000)
+            steps.append(st)
+            log(f"[bench] load step {k}: newton {int(st['iterations'])}, |r|/|r0| {st['rel_residual']:.2e}, "
+                f"cg {int(st['cg_iterations'])}, {time.time() - t1:.1f}s")
+        u_host = u.cpu().numpy()
+        _, _, mt, br = ctx.get_state()
+        newton_info = {"load_steps": wb.NSTEPS, "newton_iterations": [int(s["iterations"]) for s in steps],
+                       "final_rel_residual": steps[-1]["rel_residual"],
+                       "converged": all(s["converged"] == 1.0 for s in steps),
+                       "cg_iterations": int(sum(s["cg_iterations"] for s in steps)),
+                       "solve_seconds": round(time.time() - t1, 1)}
+        state_desc = (f"converged Newton solution of load step 10 (||r||/||r0|| <= {args.newton_rtol:g}), "
+                      "committed plastic strain of the 10 load steps")
     branch_counts = {ipf.BRANCH_NAMES[i]: int((br == i).sum()) for i in range(5)}
-    ctx.set_state(np.zeros(6 * npts))
     log(f"[bench] branches: {branch_counts}")
+    ep_state, al_state, _, _ = ctx.get_state()  # the committed state the timed residuals start from
 
     for _ in range(args.warmup):
         ctx.residual(u, r)
@@ -138,7 +169,7 @@ def main():
         while True:
             O.residual(B, ix["dof_indexes"], ix["color_indexes"], sizes["ncolor"], neq, ix["weight"], mat["lambda"],
                        mat["mu"], mat["phi"], mat["psi"], mat["cohesion"], u_host, f_lin=ix["rhs_linear"],
-                       nthreads=nthreads)
+                       plastic_strain6=ep_state, alpha=al_state, nthreads=nthreads)
             reps += 1
             el = time.time() - t
             if el >= args.cpu_seconds or reps >= 50:
@@ -164,8 +195,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain FP64 residual, {args.mesh} quarter-annulus wellbore",
                    "elements": sizes["nblocks"], "int_points": npts, "neq": neq, "colors": sizes["ncolor"],
-                   "state": "u = linear-elastic thick-cylinder displacement of load step 10, zero plastic strain",
-                   "branches": branch_counts},
+                   "state": state_desc, "branches": branch_counts, "newton": newton_info},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -28,7 +28,9 @@ struct Cfg {
     static constexpr int TP = shape_tp(SHAPE);
     static constexpr int NB = 3 * CE;            // B values per point
     static constexpr bool KEEP_B = (NB <= 64);   // keep B in registers across the return map
-    static constexpr int WPB = (CE <= 18) ? 4 : 2;  // wavefronts per workgroup
+    // One wavefront per workgroup: the LDS reduction then waits for no other
+    // wave's (divergent) return map.
+    static constexpr int WPB = 1;
     // waves per SIMD the register allocation must allow (Q2: 168 VGPRs)
     static constexpr int MIN_WAVES = (CE <= 8) ? 4 : (CE <= 18) ? 3 : 1;
 };
@@ -101,54 +103,6 @@ __device__ __forceinline__ void reduce_scatter(const double* L, const GroupDev& 
     }
 }
 
-// Reduction tasks (element e2 of the tile, column j) owned by a lane: lane and
-// lane + 64 (ET * CE <= 128 for every shape).  Within one color launch each
-// DoF belongs to exactly one task, so r can be read at the start of the tile
-// and written at the end without any other wave touching it in between.
-template <int SHAPE>
-struct RTasks {
-    int dof[2];
-    double val[2];
-};
-
-template <int SHAPE>
-__device__ __forceinline__ void prefetch_r(const GroupDev& g, int64_t tile, int lane, const double* __restrict__ r,
-                                           RTasks<SHAPE>& t) {
-    using C = Cfg<SHAPE>;
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        const int task = lane + k * kWave;
-        t.dof[k] = -1;
-        t.val[k] = 0.0;
-        if (task < C::ET * C::CE) {
-            const int e2 = task / C::CE;
-            const int j = task - e2 * C::CE;
-            const int64_t elem = tile * C::ET + e2;
-            if (elem < g.nelem) {
-                t.dof[k] = ld_stream(&g.dof[elem * C::CE + j]);
-                t.val[k] = r[t.dof[k]];
-            }
-        }
-    }
-}
-
-template <int SHAPE>
-__device__ __forceinline__ void reduce_store(const double* L, int lane, const RTasks<SHAPE>& t,
-                                             double* __restrict__ r) {
-    using C = Cfg<SHAPE>;
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        if (t.dof[k] < 0) continue;
-        const int task = lane + k * kWave;
-        const int e2 = task / C::CE;
-        const int j = task - e2 * C::CE;
-        double f = L[(e2 * C::NIP) * C::CE + j];
-#pragma unroll
-        for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e2 * C::NIP + ip) * C::CE + j];
-        r[t.dof[k]] = t.val[k] + (-f);
-    }
-}
-
 template <int SHAPE, bool COMMIT>
 __global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave, Cfg<SHAPE>::MIN_WAVES)
 k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, PointState st, MatConst m) {
@@ -181,12 +135,6 @@ k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, Poi
 #pragma unroll
         for (int j = 0; j < C::CE; j++) ue[j] = 0.0;
     }
-    RTasks<SHAPE> rt;
-    if (tile_ok) {
-        prefetch_r<SHAPE>(g, tile, lane, r, rt);
-    } else {
-        rt.dof[0] = rt.dof[1] = -1;
-    }
     double e3[3];
     strain_of<SHAPE>(b, ue, e3);
 
@@ -211,7 +159,7 @@ k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, Poi
     double* L = lds[wave];
     if (slot) bt_partial<SHAPE>(b, ws, L + lane * C::CE);
     __syncthreads();
-    reduce_store<SHAPE>(L, lane, rt, r);
+    if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, r);
 }
 
 template <int SHAPE>
@@ -339,12 +287,6 @@ k_tangent(GroupDev g, const double* __restrict__ v, double* __restrict__ y, cons
 #pragma unroll
         for (int j = 0; j < C::CE; j++) ve[j] = 0.0;
     }
-    RTasks<SHAPE> rt;
-    if (tile_ok) {
-        prefetch_r<SHAPE>(g, tile, lane, y, rt);
-    } else {
-        rt.dof[0] = rt.dof[1] = -1;
-    }
     double e3[3];
     strain_of<SHAPE>(b, ve, e3);
     double x[3] = {0.0, 0.0, 0.0};
@@ -358,7 +300,7 @@ k_tangent(GroupDev g, const double* __restrict__ v, double* __restrict__ y, cons
             double a = D[3 * k] * e3[0];
             a = a + D[3 * k + 1] * e3[1];
             a = a + D[3 * k + 2] * e3[2];
-            // reduce_store adds -(sum of partials); y accumulates +K v
+            // reduce_scatter adds -(sum of partials); y accumulates +K v
             x[k] = -a;
         }
     }
@@ -366,7 +308,7 @@ k_tangent(GroupDev g, const double* __restrict__ v, double* __restrict__ y, cons
     double* L = lds[wave];
     if (slot) bt_partial<SHAPE>(b, x, L + lane * C::CE);
     __syncthreads();
-    reduce_store<SHAPE>(L, lane, rt, y);
+    if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, y);
 }
 
 // diag += diagonal of sum_e B_e^T (w Dep) B_e: per point and column j,
@@ -384,12 +326,6 @@ k_tangent_diag(GroupDev g, double* __restrict__ diag, const double* __restrict__
     const int64_t elem = tile * C::ET + el;
     const bool slot = tile_ok && lane < C::TP;
     const bool valid = slot && elem < g.nelem;
-    RTasks<SHAPE> rt;
-    if (tile_ok) {
-        prefetch_r<SHAPE>(g, tile, lane, diag, rt);
-    } else {
-        rt.dof[0] = rt.dof[1] = -1;
-    }
     double* L = lds[wave];
     if (slot) {
         double D[9];
@@ -414,7 +350,7 @@ k_tangent_diag(GroupDev g, double* __restrict__ diag, const double* __restrict__
         }
     }
     __syncthreads();
-    reduce_store<SHAPE>(L, lane, rt, diag);
+    if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, diag);
 }
 
 __global__ void k_copy(int64_t n, const double* __restrict__ src, double* __restrict__ dst) {

@@ -5,8 +5,8 @@ Bars:
     max |K v| (different summation order only);
   * CG: ||K x - b|| <= 1e-9 ||b|| measured with the oracle's K;
   * 10-step elastoplastic Newton (40 x 40 Q2 wellbore): per-step displacements
-    within 1e-9 relative (both sides converge to ||r|| <= 1e-12 ||r0||, and
-    the tangent's conditioning amplifies that floor), committed branch flags
+    within 1e-10 relative (both sides converge to ||r|| <= 1e-12 ||r0||; measured
+    ~1e-15), committed branch flags
     bit-identical;
   * elastic limit (c -> inf): the FE displacement matches the thick-cylinder
     closed form to 5e-4 relative (the straight-sided cells approximate the
@@ -118,7 +118,7 @@ def test_newton_load_steps_match_oracle(gpu):
         print(f"step {k}: newton {int(st['iterations'])} (oracle {so['iterations']}), cg {int(st['cg_iterations'])}, "
               f"|du|/|u| {rel:.2e}, plastic {int((mt != 0).sum())}, flag mismatches {int((br != so['branch']).sum())}",
               flush=True)
-        assert rel <= 1e-9, (k, rel)
+        assert rel <= 1e-10, (k, rel)
         assert np.array_equal(br, so["branch"]), k
     assert (mt != 0).sum() > 1000
 
