@@ -150,8 +150,18 @@ def main():
     achieved = float(per_launch_bytes.sum() / (color_ms.sum() / 1e3) / 1e9)
     log(f"[bench] copy kernel {avg[0]:.4f} ms; color kernels {', '.join(f'{x:.4f}' for x in color_ms)} ms; "
         f"{nbytes / npts:.1f} algorithmic B/pt; {achieved:.0f} GB/s")
+    # HBM bytes per launch measured by rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE calibrated on
+    # k_copy) for this workload, committed with the profile it comes from
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"k_residual_{args.mesh}_traffic.json")
+    if args.state == "lame" and os.path.exists(tfile):
+        with open(tfile) as f:
+            tk = json.load(f)["kernels"]
+        key = next((k for k in tk if k.startswith("void k_residual") and "false" in k), None)
+        if key:
+            traffic = int(tk[key]["total_bytes"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_residual (one launch per element color)",
                 "bytes_per_launch": int(per_launch_bytes.mean()),
                 "avg_launch_ms": round(float(color_ms.mean()), 5)}
