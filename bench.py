@@ -12,6 +12,9 @@ Prints ONE JSON line on stdout (progress goes to stderr):
               components per point, u read + r written once) over the
               kernel's average duration, from HIP events around each launch
   cpu_baseline the CPU oracle (same algorithm, OpenMP) on the host cores
+  b_format_variants  the same residual with compact FP64, dense FP32 and
+              compact FP32 B (ipf_set_b_format): time, roofline on each
+              format's own algorithmic bytes, deviation from the FP64 residual
 """
 import argparse
 import json
@@ -31,11 +34,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(blocks, npts, neq):
+def algorithmic_bytes(blocks, npts, neq, fmt="dense-fp64"):
+    """Bytes one residual must move: the stored B values once (dense: every
+    block entry; compact: dN/dx, dN/dy per node and point), a 4-byte DoF index
+    per block column, weight + 4 plastic-strain components per point, u read
+    and r read-modify-written once."""
     import numpy as np
     R = blocks["row_sizes"].astype(np.int64)
     C = blocks["col_sizes"].astype(np.int64)
-    return int((8 * R * C + 4 * C).sum()) + npts * (8 + 32) + neq * (8 + 8)
+    word = 4 if fmt.endswith("fp32") else 8
+    nval = (R // 3) * C if fmt.startswith("compact") else R * C
+    return int((word * nval + 4 * C).sum()) + npts * (8 + 32) + neq * (8 + 8)
+
+
+def time_residuals(ctx, u, r, stream, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        ctx.residual(u, r)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        ctx.residual(u, r)
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end)
+
+
+def launch_times(ctx, u, r, nrep):
+    """Average per-launch durations (HIP events on the launch stream): index 0
+    is the f_lin copy, then one entry per color kernel; and the points each
+    launch covers."""
+    acc = None
+    pts = None
+    for _ in range(nrep):
+        ms, p = ctx.residual_timed(u, r)
+        acc = ms if acc is None else acc + ms
+        pts = p
+    return acc / nrep, pts
 
 
 def main():
@@ -45,6 +82,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mesh", default="800x800-q2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-formats", action="store_true", help="skip the compact / FP32 B format variants")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's timed work")
     ap.add_argument("--state", choices=("lame", "newton"), default="lame",
                     help="lame: thick-cylinder displacement of the last load step, zero plastic strain; "
@@ -118,30 +156,14 @@ def main():
     log(f"[bench] branches: {branch_counts}")
     ep_state, al_state, _, _ = ctx.get_state()  # the committed state the timed residuals start from
 
-    for _ in range(args.warmup):
-        ctx.residual(u, r)
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(args.steps):
-        ctx.residual(u, r)
-    end.record(stream)
-    torch.cuda.synchronize()
-    total_ms = start.elapsed_time(end)
+    total_ms = time_residuals(ctx, u, r, stream, args.steps, args.warmup)
     ms_per_step = total_ms / args.steps
     value = npts * args.steps / (total_ms / 1e3) / 1e6
     log(f"[bench] {args.steps} residuals: {ms_per_step:.4f} ms/residual, {value:.1f} Mint-pts/s")
 
     # per-launch durations (HIP events on the launch stream) for the roofline
     nrep = max(3, min(args.steps, 20))
-    acc = None
-    pts = None
-    for _ in range(nrep):
-        ms, p = ctx.residual_timed(u, r)
-        acc = ms if acc is None else acc + ms
-        pts = p
-    avg = acc / nrep
+    avg, pts = launch_times(ctx, u, r, nrep)
     color_ms = avg[1:]
     blocks = sm.export_blocks()
     nbytes = algorithmic_bytes(blocks, npts, neq)
@@ -165,6 +187,33 @@ def main():
                 "kernel": "k_residual (one launch per element color)",
                 "bytes_per_launch": int(per_launch_bytes.mean()),
                 "avg_launch_ms": round(float(color_ms.mean()), 5)}
+
+    # the other B formats (same state, same u): timing, roofline on their own
+    # algorithmic bytes, and the deviation from the FP64 residual
+    variants = {}
+    if not args.no_formats:
+        ctx.residual(u, r)
+        ctx.synchronize()
+        r64 = r.cpu().numpy()
+        fint = float(np.abs(r64 - sm.export_indexes()["rhs_linear"]).max())
+        for fmt in ipf.B_FORMATS[1:]:
+            ctx.set_b_format(fmt)
+            tms = time_residuals(ctx, u, r, stream, args.steps, args.warmup)
+            ctx.residual(u, r)
+            ctx.synchronize()
+            err = float(np.abs(r.cpu().numpy() - r64).max()) / fint
+            favg, fpts = launch_times(ctx, u, r, nrep)
+            fb = algorithmic_bytes(blocks, npts, neq, fmt) * fpts[1:] / npts
+            fach = float(fb.sum() / (favg[1:].sum() / 1e3) / 1e9)
+            variants[fmt] = {"ms_per_step": round(tms / args.steps, 5),
+                             "value": round(npts * args.steps / (tms / 1e3) / 1e6, 2),
+                             "avg_launch_ms": round(float(favg[1:].mean()), 5),
+                             "algorithmic_bytes_per_pt": round(algorithmic_bytes(blocks, npts, neq, fmt) / npts, 1),
+                             "achieved_GBs": round(fach, 1), "frac": round(fach / HBM_PEAK_GBS, 4),
+                             "max_dev_vs_fp64": err}
+            log(f"[bench] {fmt}: {tms / args.steps:.4f} ms/residual, {variants[fmt]['value']:.1f} Mint-pts/s, "
+                f"{fach:.0f} GB/s algorithmic, |r - r_fp64| / max|f_int| = {err:.2e}")
+        ctx.set_b_format("dense-fp64")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -208,6 +257,7 @@ def main():
                    "state": state_desc, "branches": branch_counts, "newton": newton_info},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "b_format_variants": variants,
     }
     print(json.dumps(out), flush=True)
 

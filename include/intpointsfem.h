@@ -138,6 +138,31 @@ int ipf_compute_sigma_dep(ipf_ctx *ctx, const double *strain_dev, double *wsigma
 /* res = -B^T wsigma, colored accumulation   TPZNumericalIntegrator::MultiplyTranspose (TPZNumericalIntegrator.cpp:54-105) */
 int ipf_multiply_transpose(ipf_ctx *ctx, const double *wsigma_dev, double *res_dev);
 
+/* ---- storage format of B ---- */
+
+enum {
+    IPF_B_DENSE_FP64 = 0,   /* every stored value of the reference's blocks (default) */
+    IPF_B_COMPACT_FP64 = 1, /* (dN/dx, dN/dy) per node and point; same results */
+    IPF_B_DENSE_FP32 = 2,   /* FP32 products with FP32 accumulation */
+    IPF_B_COMPACT_FP32 = 3
+};
+
+/* Selects the B representation of the fused residual (ipf_residual,
+ * ipf_residual_timed, and the commit inside ipf_newton) and, separately, of
+ * the tangent product (ipf_tangent_apply and the CG matvecs).  The
+ * separate-step entry points and ipf_residual_tangent keep the dense FP64
+ * blocks.  The compact forms hold only dN_a/dx and dN_a/dy per point -- the
+ * reference's blocks carry nothing else (TPZNumericalIntegrator.cpp:371-381);
+ * ipf_transfer_data checks that pattern exactly and this call fails with
+ * IPF_EINVAL when it does not hold.  Compact FP64 gives the dense FP64
+ * results (the skipped terms are exact zeros); the FP32 forms compute B.u and
+ * B^T(w sigma) in FP32 with FP32 accumulation, everything else in FP64.  An
+ * FP64 residual with an FP32 tangent makes ipf_newton an iterative refinement:
+ * the FP32 products only steer the correction, the FP64 residual decides
+ * convergence.  No reference counterpart: the reference stores dense FP64
+ * blocks only (TPZIrregularBlocksMatrix.h:20). */
+int ipf_set_b_format(ipf_ctx *ctx, int residual_format, int tangent_format);
+
 /* ---- tangent, linear solve and Newton (the caller side of the path) ---- */
 
 /* Symmetry / Dirichlet DoFs (1 = fixed) used by the solver entry points,

@@ -28,6 +28,7 @@ _LIB_PATH = os.path.join(_HERE, "libintpointsfem.so")
 
 IPF_OK = 0
 BRANCH_NAMES = ("elastic", "main", "right_edge", "left_edge", "apex")
+B_FORMATS = ("dense-fp64", "compact-fp64", "dense-fp32", "compact-fp32")
 
 
 class IPFError(RuntimeError):
@@ -86,6 +87,7 @@ def lib():
         "ipf_compute_sigma": (_int, [_vp, _vp, _vp]),
         "ipf_compute_sigma_dep": (_int, [_vp, _vp, _vp, _vp]),
         "ipf_multiply_transpose": (_int, [_vp, _vp, _vp]),
+        "ipf_set_b_format": (_int, [_vp, _int, _int]),
         "ipf_set_dirichlet_mask": (_int, [_vp, _bp]),
         "ipf_residual_tangent": (_int, [_vp, _vp, _vp]),
         "ipf_tangent_apply": (_int, [_vp, _vp, _vp]),
@@ -313,6 +315,16 @@ class DeviceContext:
 
     def multiply_transpose(self, wsigma, res):
         self._c(lib().ipf_multiply_transpose(self._h, _ptr(wsigma), _ptr(res)), "ipf_multiply_transpose")
+
+    def set_b_format(self, fmt, tangent=None):
+        """B format of the fused residual and (``tangent``, default: the same)
+        of the tangent product: 0 dense FP64 (default), 1 compact FP64,
+        2 dense FP32, 3 compact FP32, or the names in B_FORMATS."""
+        def code(f):
+            return B_FORMATS.index(f) if isinstance(f, str) else int(f)
+        fr = code(fmt)
+        ft = fr if tangent is None else code(tangent)
+        self._c(lib().ipf_set_b_format(self._h, fr, ft), "ipf_set_b_format")
 
     # ---- tangent / solver
     def set_dirichlet_mask(self, mask):

@@ -16,6 +16,7 @@ OBJDIR = os.path.join(HERE, "_build")
 SOURCES = [
     "ipf_kernels.hip",
     "ipf_solver.hip",
+    "ipf_kernels_fmt.hip",
     "ipf_device.hip",
     "TPZCompMesh.cpp",
     "TPZIntPoints.cpp",

@@ -139,6 +139,7 @@ struct Group {
     int64_t pt0 = 0;
     int64_t b_off = 0;
     int64_t dof_off = 0;
+    int64_t f_off[FMT_COUNT] = {0, 0, 0, 0};  // first value of the group in each B format
 };
 
 template <typename T>
@@ -193,6 +194,12 @@ struct ipf_ctx {
     double* dPartial = nullptr; // dot partials
     double* dScal = nullptr;    // CG scalars
     bool have_mask = false;
+
+    // alternative B formats (index FMT_*; FMT_D64 is dB)
+    void* dBf[FMT_COUNT] = {nullptr, nullptr, nullptr, nullptr};
+    int rfmt = FMT_D64;  // B format of the fused residual
+    int tfmt = FMT_D64;  // B format of the tangent product
+    bool compact_ok = false;  // every block has the 2D elasticity pattern
     bool have_tangent = false;
 };
 
@@ -257,6 +264,9 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dNd);
     dev_free(ctx->dPartial);
     dev_free(ctx->dScal);
+    for (int f = 1; f < FMT_COUNT; f++) dev_free(ctx->dBf[f]);
+    ctx->rfmt = ctx->tfmt = FMT_D64;
+    ctx->compact_ok = false;
     ctx->groups.clear();
     ctx->have_data = false;
     ctx->have_mask = false;
@@ -331,6 +341,28 @@ PointState point_state(ipf_ctx* ctx) {
     return s;
 }
 
+const void* group_fmt_B(const ipf_ctx* ctx, const Group& g, int fmt) {
+    return static_cast<const char*>(ctx->dBf[fmt]) + format_bytes(fmt) * (size_t)g.f_off[fmt];
+}
+
+// One residual launch for a group in the selected B format.
+void launch_group_residual(ipf_ctx* ctx, const Group& g, const double* u, double* r, const PointState& st) {
+    if (ctx->rfmt == FMT_D64)
+        launch_residual(g.shape, ctx->update_mem, group_dev(ctx, g), u, r, st, ctx->mat, ctx->stream);
+    else
+        launch_residual_fmt(ctx->rfmt, g.shape, ctx->update_mem, group_dev(ctx, g), group_fmt_B(ctx, g, ctx->rfmt),
+                            u, r, st, ctx->mat, ctx->stream);
+}
+
+// One tangent (K v) launch for a group in the selected B format.
+void launch_group_tangent(ipf_ctx* ctx, const Group& g, const double* v, double* y) {
+    if (ctx->tfmt == FMT_D64)
+        launch_tangent(g.shape, group_dev(ctx, g), v, y, ctx->dDep, ctx->npad, ctx->stream);
+    else
+        launch_tangent_fmt(ctx->tfmt, g.shape, group_dev(ctx, g), group_fmt_B(ctx, g, ctx->tfmt), v, y, ctx->dDep,
+                           ctx->npad, ctx->stream);
+}
+
 int check_ready(ipf_ctx* ctx, bool need_material) {
     if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
     if (!ctx->have_data) return fail(ctx, IPF_ESTATE, "integrator data not transferred (call ipf_transfer_data)");
@@ -341,6 +373,27 @@ int check_ready(ipf_ctx* ctx, bool need_material) {
 int launch_status(ipf_ctx* ctx, const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ctx, IPF_EHIP, "%s launch failed: %s", what, hipGetErrorString(e));
+    return IPF_OK;
+}
+
+// Makes format `fmt` resident (converted on the device from the dense FP64 tiles).
+int ensure_format(ipf_ctx* ctx, int fmt) {
+    if (fmt < 0 || fmt >= FMT_COUNT) return fail(ctx, IPF_EINVAL, "ipf_set_b_format: unknown format %d", fmt);
+    if (format_compact(fmt) && !ctx->compact_ok)
+        return fail(ctx, IPF_EINVAL, "compact B needs every block in the 2D elasticity pattern; this data is not");
+    if (fmt == FMT_D64 || ctx->dBf[fmt]) return IPF_OK;
+    int64_t n = 0;
+    for (const auto& g : ctx->groups)
+        n = std::max(n, g.f_off[fmt] + g.ntiles * (int64_t)format_values(g.shape, fmt) * shape_tp(g.shape));
+    char* p = nullptr;
+    int rc = dev_alloc(ctx, &p, n * format_bytes(fmt));
+    if (rc) return rc;
+    ctx->dBf[fmt] = p;
+    for (const auto& g : ctx->groups)
+        launch_convert_B(fmt, g.shape, ctx->dB + g.b_off, p + (size_t)format_bytes(fmt) * g.f_off[fmt], g.ntiles,
+                         ctx->stream);
+    if ((rc = launch_status(ctx, "convert_B"))) return rc;
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return IPF_OK;
 }
 
@@ -502,15 +555,41 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
             if (!g.elems.empty()) groups.push_back(std::move(g));
         }
     int64_t npad = 0, nb = 0, nd = 0;
+    int64_t nf[FMT_COUNT] = {0, 0, 0, 0};
     for (auto& g : groups) {
         const int ET = shape_et(g.shape), TP = shape_tp(g.shape), CE = shape_ce(g.shape);
         g.ntiles = ((int64_t)g.elems.size() + ET - 1) / ET;
         g.pt0 = npad;
         g.b_off = nb;
         g.dof_off = nd;
+        for (int f = 0; f < FMT_COUNT; f++) {
+            g.f_off[f] = nf[f];
+            nf[f] += g.ntiles * (int64_t)format_values(g.shape, f) * TP;
+        }
         npad += g.ntiles * TP;
         nb += g.ntiles * (int64_t)(3 * CE) * TP;
         nd += (int64_t)g.elems.size() * CE;
+    }
+    // --- compact B possible? Every point of every block must have the 2D
+    // elasticity pattern B(3ip + c + j, 2a + c) = dN_a/dx_j
+    // (TPZNumericalIntegrator.cpp:371-381): row xx has zeros in the y columns,
+    // row yy zeros in the x columns, and row xy repeats the other two.
+    bool compact_ok = true;
+    for (int64_t e = 0; e < nblocks && compact_ok; e++) {
+        const int C = col_sizes[e], nip = row_sizes[e] / 3;
+        const double* blk = storage + matrix_position[e];
+        for (int ip = 0; ip < nip && compact_ok; ip++) {
+            const double* rxx = blk + (int64_t)(3 * ip) * C;
+            const double* rxy = rxx + C;
+            const double* ryy = rxy + C;
+            for (int a = 0; a < C / 2; a++) {
+                if (rxx[2 * a + 1] != 0.0 || ryy[2 * a] != 0.0 || rxy[2 * a] != ryy[2 * a + 1] ||
+                    rxy[2 * a + 1] != rxx[2 * a]) {
+                    compact_ok = false;
+                    break;
+                }
+            }
+        }
     }
 
     // --- host staging in the tiled layout
@@ -594,6 +673,8 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
     ctx->nstorage = nblocks ? matrix_position[nblocks] : 0;
     ctx->npad = npad;
     ctx->groups = std::move(groups);
+    ctx->compact_ok = compact_ok;
+    ctx->rfmt = ctx->tfmt = FMT_D64;
     ctx->have_data = true;
     return IPF_OK;
 }
@@ -634,7 +715,7 @@ int ipf_residual(ipf_ctx* ctx, const double* u_dev, double* r_dev) {
     launch_copy(ctx->neq, ctx->dFlin, r_dev, ctx->stream);
     const PointState st = point_state(ctx);
     for (const auto& g : ctx->groups)
-        launch_residual(g.shape, ctx->update_mem, group_dev(ctx, g), u_dev, r_dev, st, ctx->mat, ctx->stream);
+        launch_group_residual(ctx, g, u_dev, r_dev, st);
     return launch_status(ctx, "residual");
 }
 
@@ -654,7 +735,7 @@ int ipf_residual_timed(ipf_ctx* ctx, const double* u_dev, double* r_dev, double*
     IPF_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
     for (size_t i = 0; i < ctx->groups.size(); i++) {
         const auto& g = ctx->groups[i];
-        launch_residual(g.shape, ctx->update_mem, group_dev(ctx, g), u_dev, r_dev, st, ctx->mat, ctx->stream);
+        launch_group_residual(ctx, g, u_dev, r_dev, st);
         IPF_HIP(ctx, hipEventRecord(ev[i + 2], ctx->stream));
     }
     rc = launch_status(ctx, "residual");
@@ -741,6 +822,16 @@ int ipf_multiply_transpose(ipf_ctx* ctx, const double* wsigma_dev, double* res_d
     return launch_status(ctx, "multiply_transpose");
 }
 
+int ipf_set_b_format(ipf_ctx* ctx, int residual_format, int tangent_format) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_format(ctx, residual_format)) || (rc = ensure_format(ctx, tangent_format))) return rc;
+    ctx->rfmt = residual_format;
+    ctx->tfmt = tangent_format;
+    return IPF_OK;
+}
+
 int ipf_set_dirichlet_mask(ipf_ctx* ctx, const signed char* mask_host) {
     int rc = check_ready(ctx, false);
     if (rc) return rc;
@@ -777,7 +868,7 @@ int ipf_tangent_apply(ipf_ctx* ctx, const double* v_dev, double* y_dev) {
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     launch_fill(ctx->neq, y_dev, 0.0, ctx->stream);
     for (const auto& g : ctx->groups)
-        launch_tangent(g.shape, group_dev(ctx, g), v_dev, y_dev, ctx->dDep, ctx->npad, ctx->stream);
+        launch_group_tangent(ctx, g, v_dev, y_dev);
     return launch_status(ctx, "tangent_apply");
 }
 
@@ -811,7 +902,7 @@ int ipf_cg_solve(ipf_ctx* ctx, const double* b_dev, double* x_dev, double rtol, 
             // q = K p on the free DoFs
             launch_fill(n, ctx->dCq, 0.0, s);
             for (const auto& g : ctx->groups)
-                launch_tangent(g.shape, group_dev(ctx, g), ctx->dCp, ctx->dCq, ctx->dDep, ctx->npad, s);
+                launch_group_tangent(ctx, g, ctx->dCp, ctx->dCq);
             launch_cg_mask(n, ctx->dMask, ctx->dCq, s);
             launch_dot(n, ctx->dCp, ctx->dCq, ctx->dPartial, nb, ctx->dScal + 1, s);
             launch_cg_scalars(ctx->dScal, 0, s);

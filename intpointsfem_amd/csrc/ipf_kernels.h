@@ -48,6 +48,19 @@ constexpr int shape_ce(int s) { return s == SHAPE_Q1 ? 8 : s == SHAPE_Q2 ? 18 : 
 constexpr int shape_et(int s) { return 64 / shape_nip(s); }
 constexpr int shape_tp(int s) { return shape_et(s) * shape_nip(s); }
 
+// Storage formats of B (same tile order, point-interleaved 16-byte vectors):
+//   FMT_D64  dense FP64, every stored value of the reference's block
+//   FMT_C64  compact FP64, (dN_a/dx, dN_a/dy) per node and point
+//   FMT_D32  dense FP32, FMT_C32 compact FP32 (values padded to 4 per point)
+enum BFormat { FMT_D64 = 0, FMT_C64 = 1, FMT_D32 = 2, FMT_C32 = 3, FMT_COUNT = 4 };
+constexpr int format_bytes(int f) { return (f == FMT_D32 || f == FMT_C32) ? 4 : 8; }
+constexpr bool format_compact(int f) { return f == FMT_C64 || f == FMT_C32; }
+// values stored per integration point (padded to a whole 16-byte vector)
+constexpr int format_values(int s, int f) {
+    return ((format_compact(f) ? shape_ce(s) : 3 * shape_ce(s)) + (16 / format_bytes(f)) - 1) /
+           (16 / format_bytes(f)) * (16 / format_bytes(f));
+}
+
 // Kernel launchers (ipf_kernels.hip).  All enqueue on `stream`.
 void launch_residual(int shape, bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
                      const MatConst& m, hipStream_t stream);
@@ -68,6 +81,15 @@ void launch_tangent(int shape, const GroupDev& g, const double* v, double* y, co
 // diag += diagonal of the same operator.
 void launch_tangent_diag(int shape, const GroupDev& g, double* diag, const double* dep_soa, int64_t npad,
                          hipStream_t stream);
+
+// Format variants (ipf_kernels_fmt.hip): Bv points at the group's first tile
+// in format fmt (FMT_C64, FMT_D32 or FMT_C32).
+void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
+                         double* r, const PointState& st, const MatConst& m, hipStream_t stream);
+void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, const double* v, double* y,
+                        const double* dep_soa, int64_t npad, hipStream_t stream);
+// dense FP64 tiles of one group -> format fmt
+void launch_convert_B(int fmt, int shape, const double* dense, void* out, int64_t ntiles, hipStream_t stream);
 
 // Vector kernels of the preconditioned CG (ipf_solver.hip).  `scal` is a small
 // device array of scalars (see ipf_solver.hip for the slots).

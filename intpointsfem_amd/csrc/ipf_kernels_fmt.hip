@@ -1,0 +1,365 @@
+// ipf_kernels_fmt.hip -- the residual and tangent kernels for the alternative
+// storage formats of B (see ipf_kernels.h, BFormat):
+//
+//   FMT_C64  compact FP64: per point only (dN_a/dx, dN_a/dy), the values the
+//            reference writes into its B block (TPZNumericalIntegrator.cpp:
+//            371-381; the other entries of the 3 x 2n block are zeros or
+//            copies).  The products keep the dense order term by term and
+//            only skip exact zeros, so strains, stresses and forces equal the
+//            dense FP64 results (up to the sign of a zero).
+//   FMT_D32  dense FP32: the stored block in FP32, products accumulated in FP32.
+//   FMT_C32  compact FP32.
+//
+// FP32 formats compute B.u and B^T(w sigma) in FP32 with FP32 accumulation;
+// the return map, the assembly into r and everything else stay FP64.
+#include "ipf_kernels.h"
+
+namespace ipf {
+
+namespace {
+
+constexpr int kWave = 64;
+
+template <int FMT>
+struct FmtTraits;
+template <>
+struct FmtTraits<FMT_D64> { using T = double; static constexpr bool COMPACT = false; static constexpr int VEC = 2; };
+template <>
+struct FmtTraits<FMT_C64> { using T = double; static constexpr bool COMPACT = true; static constexpr int VEC = 2; };
+template <>
+struct FmtTraits<FMT_D32> { using T = float; static constexpr bool COMPACT = false; static constexpr int VEC = 4; };
+template <>
+struct FmtTraits<FMT_C32> { using T = float; static constexpr bool COMPACT = true; static constexpr int VEC = 4; };
+
+template <int SHAPE, int FMT>
+struct FC {
+    using F = FmtTraits<FMT>;
+    using T = typename F::T;
+    static constexpr int NIP = shape_nip(SHAPE);
+    static constexpr int CE = shape_ce(SHAPE);
+    static constexpr int ET = shape_et(SHAPE);
+    static constexpr int TP = shape_tp(SHAPE);
+    static constexpr int NV = format_values(SHAPE, FMT);  // stored values per point (padded)
+    static constexpr int VEC = F::VEC;
+    static constexpr int MIN_WAVES = F::COMPACT ? 4 : ((CE <= 8) ? 4 : (CE <= 18) ? 3 : 1);
+};
+
+template <class T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+typedef double dv2 __attribute__((ext_vector_type(2)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
+
+template <int SHAPE, int FMT>
+__device__ __forceinline__ void load_values(const void* __restrict__ Bg, int64_t tile, int lane,
+                                            typename FC<SHAPE, FMT>::T* b) {
+    using C = FC<SHAPE, FMT>;
+    if constexpr (C::VEC == 2) {
+        const dv2* Bt = reinterpret_cast<const dv2*>(Bg) + tile * (int64_t)(C::NV / 2 * C::TP);
+#pragma unroll
+        for (int m = 0; m < C::NV / 2; m++) {
+            const dv2 v = ld_stream(&Bt[m * C::TP + lane]);
+            b[2 * m] = v.x;
+            b[2 * m + 1] = v.y;
+        }
+    } else {
+        const fv4* Bt = reinterpret_cast<const fv4*>(Bg) + tile * (int64_t)(C::NV / 4 * C::TP);
+#pragma unroll
+        for (int m = 0; m < C::NV / 4; m++) {
+            const fv4 v = ld_stream(&Bt[m * C::TP + lane]);
+            b[4 * m] = v.x;
+            b[4 * m + 1] = v.y;
+            b[4 * m + 2] = v.z;
+            b[4 * m + 3] = v.w;
+        }
+    }
+}
+
+// strain in the accumulation type T.  Dense: rows summed over columns j
+// ascending.  Compact: the same sums with the zero terms left out.
+template <int SHAPE, int FMT>
+__device__ __forceinline__ void strain_fmt(const typename FC<SHAPE, FMT>::T* b, const double* ue, double* e3) {
+    using C = FC<SHAPE, FMT>;
+    using T = typename C::T;
+    if constexpr (FmtTraits<FMT>::COMPACT) {
+        T a0 = T(0), a1 = T(0), a2 = T(0);
+#pragma unroll
+        for (int a = 0; a < C::CE / 2; a++) {
+            const T dx = b[2 * a], dy = b[2 * a + 1];
+            const T ux = T(ue[2 * a]), uy = T(ue[2 * a + 1]);
+            a0 = a0 + dx * ux;
+            a1 = a1 + dy * ux;
+            a1 = a1 + dx * uy;
+            a2 = a2 + dy * uy;
+        }
+        e3[0] = double(a0);
+        e3[1] = double(a1);
+        e3[2] = double(a2);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 0; j < C::CE; j++) acc = acc + b[k * C::CE + j] * T(ue[j]);
+            e3[k] = double(acc);
+        }
+    }
+}
+
+template <int SHAPE, int FMT>
+__device__ __forceinline__ void bt_fmt(const typename FC<SHAPE, FMT>::T* b, const double* x,
+                                       typename FC<SHAPE, FMT>::T* row) {
+    using C = FC<SHAPE, FMT>;
+    using T = typename C::T;
+    const T x0 = T(x[0]), x1 = T(x[1]), x2 = T(x[2]);
+    if constexpr (FmtTraits<FMT>::COMPACT) {
+#pragma unroll
+        for (int a = 0; a < C::CE / 2; a++) {
+            const T dx = b[2 * a], dy = b[2 * a + 1];
+            T g = dx * x0;
+            g = g + dy * x1;
+            row[2 * a] = g;
+            T h = dx * x1;
+            h = h + dy * x2;
+            row[2 * a + 1] = h;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) {
+            T g = b[j] * x0;
+            g = g + b[C::CE + j] * x1;
+            g = g + b[2 * C::CE + j] * x2;
+            row[j] = g;
+        }
+    }
+}
+
+template <int SHAPE, int FMT>
+__device__ __forceinline__ void reduce_fmt(const typename FC<SHAPE, FMT>::T* L, const GroupDev& g, int64_t tile,
+                                           int lane, double* __restrict__ r) {
+    using C = FC<SHAPE, FMT>;
+    using T = typename C::T;
+    for (int task = lane; task < C::ET * C::CE; task += kWave) {
+        const int e2 = task / C::CE;
+        const int j = task - e2 * C::CE;
+        const int64_t elem = tile * C::ET + e2;
+        if (elem < g.nelem) {
+            T f = L[(e2 * C::NIP) * C::CE + j];
+#pragma unroll
+            for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e2 * C::NIP + ip) * C::CE + j];
+            const int d = g.dof[elem * C::CE + j];
+            r[d] = r[d] + (-double(f));
+        }
+    }
+}
+
+template <int SHAPE, int FMT, bool COMMIT>
+__global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
+k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ u, double* __restrict__ r,
+               PointState st, MatConst m) {
+    using C = FC<SHAPE, FMT>;
+    using T = typename C::T;
+    __shared__ T lds[C::TP * C::CE];
+    const int lane = threadIdx.x;
+    const int64_t tile = blockIdx.x;
+    const int el = lane / C::NIP;
+    const int64_t elem = tile * C::ET + el;
+    const bool slot = lane < C::TP;
+    const bool valid = slot && elem < g.nelem;
+    T b[C::NV];
+    if (slot) {
+        load_values<SHAPE, FMT>(Bv, tile, lane, b);
+    } else {
+#pragma unroll
+        for (int k = 0; k < C::NV; k++) b[k] = T(0);
+    }
+    double ue[C::CE];
+    if (valid) {
+        const int* d = g.dof + elem * C::CE;
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) ue[j] = u[ld_stream(&d[j])];
+    } else {
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) ue[j] = 0.0;
+    }
+    double e3[3];
+    strain_fmt<SHAPE, FMT>(b, ue, e3);
+    double ws[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+        const int64_t P = g.pt0 + tile * C::TP + lane;
+        double ep[4] = {ld_stream(&st.ep0[P]), ld_stream(&st.ep1[P]), ld_stream(&st.ep2[P]), ld_stream(&st.ep3[P])};
+        double alpha = ld_stream(&st.alpha[P]);
+        int mt, br;
+        point_sigma<false>(e3, ep, alpha, mt, br, m, ld_stream(&st.w[P]), ws, nullptr);
+        if (COMMIT) {
+            st.ep0[P] = ep[0];
+            st.ep1[P] = ep[1];
+            st.ep2[P] = ep[2];
+            st.ep3[P] = ep[3];
+            st.alpha[P] = alpha;
+            st.mtype[P] = (signed char)mt;
+            st.branch[P] = (signed char)br;
+        }
+    }
+    if (slot) bt_fmt<SHAPE, FMT>(b, ws, lds + lane * C::CE);
+    __syncthreads();
+    reduce_fmt<SHAPE, FMT>(lds, g, tile, lane, r);
+}
+
+// y += sum_e B_e^T (w Dep) B_e v_e with B in format FMT (FP32 formats: FP32
+// products and accumulation, the matvec of the inner CG solves).
+template <int SHAPE, int FMT>
+__global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
+k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ v, double* __restrict__ y,
+              const double* __restrict__ dep, int64_t npad) {
+    using C = FC<SHAPE, FMT>;
+    using T = typename C::T;
+    __shared__ T lds[C::TP * C::CE];
+    const int lane = threadIdx.x;
+    const int64_t tile = blockIdx.x;
+    const int el = lane / C::NIP;
+    const int64_t elem = tile * C::ET + el;
+    const bool slot = lane < C::TP;
+    const bool valid = slot && elem < g.nelem;
+    T b[C::NV];
+    if (slot) {
+        load_values<SHAPE, FMT>(Bv, tile, lane, b);
+    } else {
+#pragma unroll
+        for (int k = 0; k < C::NV; k++) b[k] = T(0);
+    }
+    double ve[C::CE];
+    if (valid) {
+        const int* d = g.dof + elem * C::CE;
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) ve[j] = v[ld_stream(&d[j])];
+    } else {
+#pragma unroll
+        for (int j = 0; j < C::CE; j++) ve[j] = 0.0;
+    }
+    double e3[3];
+    strain_fmt<SHAPE, FMT>(b, ve, e3);
+    double x[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+        const int64_t P = g.pt0 + tile * C::TP + lane;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double a = ld_stream(&dep[(3 * k) * npad + P]) * e3[0];
+            a = a + ld_stream(&dep[(3 * k + 1) * npad + P]) * e3[1];
+            a = a + ld_stream(&dep[(3 * k + 2) * npad + P]) * e3[2];
+            x[k] = -a;  // reduce_fmt adds -(sum); y accumulates +K v
+        }
+    }
+    if (slot) bt_fmt<SHAPE, FMT>(b, x, lds + lane * C::CE);
+    __syncthreads();
+    reduce_fmt<SHAPE, FMT>(lds, g, tile, lane, y);
+}
+
+// Builds format FMT from the dense FP64 tile layout of one group (device side).
+template <int SHAPE, int FMT>
+__global__ void k_convert_B(const double* __restrict__ dense, void* __restrict__ out, int64_t ntiles) {
+    using C = FC<SHAPE, FMT>;
+    using T = typename C::T;
+    const int64_t tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (tile >= ntiles || lane >= C::TP) return;
+    constexpr int NB = 3 * C::CE;
+    const double* src = dense + tile * (int64_t)(NB * C::TP);
+    T* dst = reinterpret_cast<T*>(out) + tile * (int64_t)(C::NV * C::TP);
+    auto dense_at = [&](int mm) -> double { return src[((int64_t)(mm >> 1) * C::TP + lane) * 2 + (mm & 1)]; };
+    for (int k = 0; k < C::NV; k++) {
+        double val = 0.0;
+        if (FmtTraits<FMT>::COMPACT) {
+            if (k < C::CE) {
+                const int a = k >> 1;
+                // dx_a = B[xx][2a], dy_a = B[yy][2a+1]
+                val = (k & 1) ? dense_at(2 * C::CE + 2 * a + 1) : dense_at(2 * a);
+            }
+        } else if (k < NB) {
+            val = dense_at(k);
+        }
+        dst[((int64_t)(k / C::VEC) * C::TP + lane) * C::VEC + (k % C::VEC)] = T(val);
+    }
+}
+
+template <int SHAPE, int FMT>
+void residual_fmt_shape(bool commit, const GroupDev& g, const void* Bv, const double* u, double* r,
+                        const PointState& st, const MatConst& m, hipStream_t s) {
+    const dim3 grid((unsigned)g.ntiles), block(kWave);
+    if (commit)
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, true>), grid, block, 0, s, g, Bv, u, r, st, m);
+    else
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false>), grid, block, 0, s, g, Bv, u, r, st, m);
+}
+
+template <int FMT>
+void residual_fmt(int shape, bool commit, const GroupDev& g, const void* Bv, const double* u, double* r,
+                  const PointState& st, const MatConst& m, hipStream_t s) {
+    switch (shape) {
+        case SHAPE_Q1: residual_fmt_shape<SHAPE_Q1, FMT>(commit, g, Bv, u, r, st, m, s); break;
+        case SHAPE_Q2: residual_fmt_shape<SHAPE_Q2, FMT>(commit, g, Bv, u, r, st, m, s); break;
+        case SHAPE_Q3: residual_fmt_shape<SHAPE_Q3, FMT>(commit, g, Bv, u, r, st, m, s); break;
+        default: break;
+    }
+}
+
+template <int FMT>
+void tangent_fmt(int shape, const GroupDev& g, const void* Bv, const double* v, double* y, const double* dep,
+                 int64_t npad, hipStream_t s) {
+    const dim3 grid((unsigned)g.ntiles), block(kWave);
+    switch (shape) {
+        case SHAPE_Q1: hipLaunchKernelGGL((k_tangent_fmt<SHAPE_Q1, FMT>), grid, block, 0, s, g, Bv, v, y, dep, npad); break;
+        case SHAPE_Q2: hipLaunchKernelGGL((k_tangent_fmt<SHAPE_Q2, FMT>), grid, block, 0, s, g, Bv, v, y, dep, npad); break;
+        case SHAPE_Q3: hipLaunchKernelGGL((k_tangent_fmt<SHAPE_Q3, FMT>), grid, block, 0, s, g, Bv, v, y, dep, npad); break;
+        default: break;
+    }
+}
+
+template <int FMT>
+void convert_fmt(int shape, const double* dense, void* out, int64_t ntiles, hipStream_t s) {
+    const dim3 grid((unsigned)ntiles), block(kWave);
+    switch (shape) {
+        case SHAPE_Q1: hipLaunchKernelGGL((k_convert_B<SHAPE_Q1, FMT>), grid, block, 0, s, dense, out, ntiles); break;
+        case SHAPE_Q2: hipLaunchKernelGGL((k_convert_B<SHAPE_Q2, FMT>), grid, block, 0, s, dense, out, ntiles); break;
+        case SHAPE_Q3: hipLaunchKernelGGL((k_convert_B<SHAPE_Q3, FMT>), grid, block, 0, s, dense, out, ntiles); break;
+        default: break;
+    }
+}
+
+}  // namespace
+
+void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
+                         double* r, const PointState& st, const MatConst& m, hipStream_t s) {
+    if (g.ntiles == 0) return;
+    switch (fmt) {
+        case FMT_C64: residual_fmt<FMT_C64>(shape, commit, g, Bv, u, r, st, m, s); break;
+        case FMT_D32: residual_fmt<FMT_D32>(shape, commit, g, Bv, u, r, st, m, s); break;
+        case FMT_C32: residual_fmt<FMT_C32>(shape, commit, g, Bv, u, r, st, m, s); break;
+        default: break;
+    }
+}
+
+void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, const double* v, double* y,
+                        const double* dep, int64_t npad, hipStream_t s) {
+    if (g.ntiles == 0) return;
+    switch (fmt) {
+        case FMT_C64: tangent_fmt<FMT_C64>(shape, g, Bv, v, y, dep, npad, s); break;
+        case FMT_D32: tangent_fmt<FMT_D32>(shape, g, Bv, v, y, dep, npad, s); break;
+        case FMT_C32: tangent_fmt<FMT_C32>(shape, g, Bv, v, y, dep, npad, s); break;
+        default: break;
+    }
+}
+
+void launch_convert_B(int fmt, int shape, const double* dense, void* out, int64_t ntiles, hipStream_t s) {
+    if (ntiles == 0) return;
+    switch (fmt) {
+        case FMT_C64: convert_fmt<FMT_C64>(shape, dense, out, ntiles, s); break;
+        case FMT_D32: convert_fmt<FMT_D32>(shape, dense, out, ntiles, s); break;
+        case FMT_C32: convert_fmt<FMT_C32>(shape, dense, out, ntiles, s); break;
+        default: break;
+    }
+}
+
+}  // namespace ipf
