@@ -162,6 +162,9 @@ enum {
  * convergence.  No reference counterpart: the reference stores dense FP64
  * blocks only (TPZIrregularBlocksMatrix.h:20). */
 int ipf_set_b_format(ipf_ctx *ctx, int residual_format, int tangent_format);
+/* The formats in use.  ipf_transfer_data selects IPF_B_COMPACT_FP64 for both
+ * when every block has the 2D elasticity pattern, IPF_B_DENSE_FP64 otherwise. */
+int ipf_get_b_format(const ipf_ctx *ctx, int *residual_format, int *tangent_format);
 
 /* ---- tangent, linear solve and Newton (the caller side of the path) ---- */
 

@@ -88,6 +88,7 @@ def lib():
         "ipf_compute_sigma_dep": (_int, [_vp, _vp, _vp, _vp]),
         "ipf_multiply_transpose": (_int, [_vp, _vp, _vp]),
         "ipf_set_b_format": (_int, [_vp, _int, _int]),
+        "ipf_get_b_format": (_int, [_vp, _ip, _ip]),
         "ipf_set_dirichlet_mask": (_int, [_vp, _bp]),
         "ipf_residual_tangent": (_int, [_vp, _vp, _vp]),
         "ipf_tangent_apply": (_int, [_vp, _vp, _vp]),
@@ -325,6 +326,13 @@ class DeviceContext:
         fr = code(fmt)
         ft = fr if tangent is None else code(tangent)
         self._c(lib().ipf_set_b_format(self._h, fr, ft), "ipf_set_b_format")
+
+    def b_format(self):
+        """(residual format, tangent format) names in use."""
+        fr = ctypes.c_int()
+        ft = ctypes.c_int()
+        self._c(lib().ipf_get_b_format(self._h, ctypes.byref(fr), ctypes.byref(ft)), "ipf_get_b_format")
+        return B_FORMATS[fr.value], B_FORMATS[ft.value]
 
     # ---- tangent / solver
     def set_dirichlet_mask(self, mask):

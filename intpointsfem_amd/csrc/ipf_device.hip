@@ -676,6 +676,12 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
     ctx->compact_ok = compact_ok;
     ctx->rfmt = ctx->tfmt = FMT_D64;
     ctx->have_data = true;
+    // Blocks in the 2D elasticity pattern default to compact FP64: the same
+    // results from a third of the B bytes.
+    if (compact_ok) {
+        if ((rc = ensure_format(ctx, FMT_C64))) return rc;
+        ctx->rfmt = ctx->tfmt = FMT_C64;
+    }
     return IPF_OK;
 }
 
@@ -820,6 +826,13 @@ int ipf_multiply_transpose(ipf_ctx* ctx, const double* wsigma_dev, double* res_d
     for (const auto& g : ctx->groups)
         launch_transpose(g.shape, group_dev(ctx, g), wsigma_dev, res_dev, ctx->dOrig, ctx->stream);
     return launch_status(ctx, "multiply_transpose");
+}
+
+int ipf_get_b_format(const ipf_ctx* ctx, int* residual_format, int* tangent_format) {
+    if (!ctx || !residual_format || !tangent_format) return fail(nullptr, IPF_EINVAL, "ipf_get_b_format: null argument");
+    *residual_format = ctx->rfmt;
+    *tangent_format = ctx->tfmt;
+    return IPF_OK;
 }
 
 int ipf_set_b_format(ipf_ctx* ctx, int residual_format, int tangent_format) {

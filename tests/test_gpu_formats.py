@@ -57,6 +57,7 @@ def test_compact_fp64_equals_dense(gpu, mesh):
     neq = m.info()["neq"]
     npts = sm.sizes()["npts"]
     u = _t(wb.lame_displacement(m.nodes(), wb.normal_stress(10), scale=9.0), gpu)
+    ctx.set_b_format("dense-fp64")  # compact FP64 is the default for these blocks
     r_d = _residual(ctx, u, neq, gpu, commit=True)
     ep_d, al_d, mt_d, br_d = ctx.get_state()
     ctx.set_state(np.zeros(6 * npts))
@@ -79,6 +80,7 @@ def test_compact_fp64_tangent_equals_dense(gpu):
     v = _t(np.random.default_rng(2).standard_normal(neq), gpu)
     y_d = torch.zeros(neq, dtype=torch.float64, device=gpu)
     y_c = torch.zeros(neq, dtype=torch.float64, device=gpu)
+    ctx.set_b_format("dense-fp64")
     ctx.tangent_apply(v, y_d)
     ctx.set_b_format("compact-fp64")
     ctx.tangent_apply(v, y_c)
@@ -160,6 +162,7 @@ def test_compact_rejected_for_general_blocks(gpu):
     ctx = ipf.DeviceContext(device=0)
     ctx.transfer_data(b["row_sizes"], b["col_sizes"], b["matrix_position"], b["row_first"], b["col_first"], storage,
                       ix["dof_indexes"], ix["color_indexes"], s["ncolor"], s["neq"], ix["weight"])
+    assert ctx.b_format() == ("dense-fp64", "dense-fp64")
     with pytest.raises(ipf.IPFError, match="compact B"):
         ctx.set_b_format("compact-fp64")
     with pytest.raises(ipf.IPFError, match="compact B"):
@@ -169,3 +172,10 @@ def test_compact_rejected_for_general_blocks(gpu):
     eps = torch.zeros(s["rows"], dtype=torch.float64, device=gpu)
     ctx.multiply(u, eps)
     ctx.synchronize()
+
+
+def test_compact_fp64_is_the_default(gpu):
+    m, sm, ctx, B, ix = _setup((4, 3, 2, 1.2))
+    assert ctx.b_format() == ("compact-fp64", "compact-fp64")
+    ctx.set_b_format("dense-fp64", tangent="compact-fp32")
+    assert ctx.b_format() == ("dense-fp64", "compact-fp32")
