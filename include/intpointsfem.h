@@ -180,6 +180,23 @@ int ipf_set_dirichlet_mask(ipf_ctx *ctx, const signed char *mask_host);
  * MatrixAssembleKernel (KernelsMatrixAssemble.h:24-56) are applied matrix-free. */
 int ipf_residual_tangent(ipf_ctx *ctx, const double *u_dev, double *r_dev);
 
+/* CSR positions of the packed element matrices for ipf_assemble_tangent, as
+ * TPZNumericalIntegrator::FillLIndexes builds them (TPZNumericalIntegrator.cpp:
+ * 538-590): el_color_index [nblocks] = fElColorIndex (elements color by
+ * color), color_l_sequence [sum_e C_e (C_e + 1) / 2] = fColorLSequence (for
+ * each element in that order, the position in the values array of the
+ * symmetric upper-triangle CSR (TPZSYsmpMatrix) of K_e(a, c), a <= c, row by
+ * row), nnz = number of CSR values.  Host arrays, copied; checked for range
+ * and color order. */
+int ipf_set_tangent_pattern(ipf_ctx *ctx, const int *el_color_index, const int64_t *color_l_sequence, int64_t nnz);
+
+/* TPZNumericalIntegrator::KAssembly (TPZNumericalIntegrator.cpp:131-214, GPU
+ * variant :184-214 with MatrixAssembleKernel, KernelsMatrixAssemble.h:24-56):
+ * ipf_residual_tangent, then kg_dev[nnz] = 0 and, color by color, every
+ * element's K_e = sum_ip B_ip^T (w Dep_ip) B_ip added at its CSR positions.
+ * Device pointers; kg_dev holds nnz doubles. */
+int ipf_assemble_tangent(ipf_ctx *ctx, const double *u_dev, double *r_dev, double *kg_dev);
+
 /* y = K v with K = sum_e B_e^T (w Dep) B_e from the last ipf_residual_tangent
  * (all DoFs, no mask), device pointers [neq]. */
 int ipf_tangent_apply(ipf_ctx *ctx, const double *v_dev, double *y_dev);

@@ -63,6 +63,19 @@ int ipf_sm_export_indexes(const ipf_sm *sm, int *dof_indexes, int *color_indexes
 /* Assemble(rhs) for u (TPZElastoPlasticIntPointsStructMatrix.cpp:137-154),
  * host arrays [neq]; commits the state when the mesh's UpdateMem is set. */
 int ipf_sm_assemble_rhs(ipf_sm *sm, const double *u_host, double *rhs_host);
+/* Create() (TPZElastoPlasticIntPointsStructMatrix.cpp:34-60): the symmetric
+ * upper-triangle CSR pattern of K and its per-color positions
+ * (TPZNumericalIntegrator::FillLIndexes, .cpp:538-590); *nnz = CSR values.
+ * Host work only: needs no GPU. */
+int ipf_sm_create_tangent(ipf_sm *sm, int64_t *nnz);
+/* IA [neq+1], JA [nnz], fColorLSequence [sum_e C_e (C_e+1)/2],
+ * fFirstColorLIndex [ncolor+1] of that pattern; any may be NULL. */
+int ipf_sm_export_tangent_pattern(const ipf_sm *sm, int64_t *ia, int64_t *ja, int64_t *color_l_sequence,
+                                  int64_t *first_color_l_index);
+/* Assemble(K, rhs) for u (TPZElastoPlasticIntPointsStructMatrix.cpp:98-135):
+ * the tangent's CSR values kg [nnz] and rhs [neq], host arrays; commits the
+ * state when the mesh's UpdateMem is set. */
+int ipf_sm_assemble_tangent(ipf_sm *sm, const double *u_host, double *rhs_host, double *kg_host);
 
 #ifdef __cplusplus
 }

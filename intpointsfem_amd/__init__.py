@@ -92,6 +92,8 @@ def lib():
         "ipf_set_dirichlet_mask": (_int, [_vp, _bp]),
         "ipf_residual_tangent": (_int, [_vp, _vp, _vp]),
         "ipf_tangent_apply": (_int, [_vp, _vp, _vp]),
+        "ipf_set_tangent_pattern": (_int, [_vp, _ip, _lp, _i64]),
+        "ipf_assemble_tangent": (_int, [_vp, _vp, _vp, _vp]),
         "ipf_cg_solve": (_int, [_vp, _vp, _vp, _d, _int, _ip, _dp]),
         "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
         "ipf_get_state": (_int, [_vp, _dp, _dp, _ip, _ip]),
@@ -114,6 +116,9 @@ def lib():
         "ipf_sm_export_blocks": (_int, [_vp, _ip, _ip, _lp, _ip, _ip, _dp]),
         "ipf_sm_export_indexes": (_int, [_vp, _ip, _ip, _ip, _lp, _dp, _dp]),
         "ipf_sm_assemble_rhs": (_int, [_vp, _dp, _dp]),
+        "ipf_sm_create_tangent": (_int, [_vp, _lp]),
+        "ipf_sm_export_tangent_pattern": (_int, [_vp, _lp, _lp, _lp, _lp]),
+        "ipf_sm_assemble_tangent": (_int, [_vp, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -342,6 +347,16 @@ class DeviceContext:
     def residual_tangent(self, u, r):
         self._c(lib().ipf_residual_tangent(self._h, _ptr(u), _ptr(r)), "ipf_residual_tangent")
 
+    def set_tangent_pattern(self, el_color_index, color_l_sequence, nnz):
+        """fElColorIndex, fColorLSequence and the CSR value count (FillLIndexes)."""
+        e = np.ascontiguousarray(el_color_index, dtype=np.int32)
+        l = np.ascontiguousarray(color_l_sequence, dtype=np.int64)
+        self._c(lib().ipf_set_tangent_pattern(self._h, _p(e, _ip), _p(l, _lp), int(nnz)), "ipf_set_tangent_pattern")
+
+    def assemble_tangent(self, u, r, kg):
+        """KAssembly: residual into r and the CSR values of K into kg (device tensors)."""
+        self._c(lib().ipf_assemble_tangent(self._h, _ptr(u), _ptr(r), _ptr(kg)), "ipf_assemble_tangent")
+
     def tangent_apply(self, v, y):
         self._c(lib().ipf_tangent_apply(self._h, _ptr(v), _ptr(y)), "ipf_tangent_apply")
 
@@ -436,3 +451,33 @@ class StructMatrix:
         r = np.zeros_like(u)
         _check(lib().ipf_sm_assemble_rhs(self._h, _p(u, _dp), _p(r, _dp)), what="ipf_sm_assemble_rhs")
         return r
+
+    def create_tangent(self):
+        """Create(): builds the symmetric CSR pattern and FillLIndexes; returns nnz."""
+        n = np.zeros(1, dtype=np.int64)
+        _check(lib().ipf_sm_create_tangent(self._h, _p(n, _lp)), what="ipf_sm_create_tangent")
+        return int(n[0])
+
+    def tangent_pattern(self):
+        """dict(ia, ja, color_l_sequence, first_color_l_index) of the pattern Create() built."""
+        nnz = self.create_tangent()
+        s = self.sizes()
+        C = self.export_blocks()["col_sizes"].astype(np.int64)
+        ia = np.zeros(s["neq"] + 1, dtype=np.int64)
+        ja = np.zeros(nnz, dtype=np.int64)
+        cl = np.zeros(int(((C * C + C) // 2).sum()), dtype=np.int64)
+        fl = np.zeros(s["ncolor"] + 1, dtype=np.int64)
+        _check(lib().ipf_sm_export_tangent_pattern(self._h, _p(ia, _lp), _p(ja, _lp), _p(cl, _lp), _p(fl, _lp)),
+               what="ipf_sm_export_tangent_pattern")
+        return {"ia": ia, "ja": ja, "color_l_sequence": cl, "first_color_l_index": fl}
+
+    def assemble_tangent(self, u):
+        """Assemble(K, rhs) at u (host arrays): returns (rhs, kg) with kg the
+        upper-triangle CSR values in the pattern of tangent_pattern()."""
+        nnz = self.create_tangent()
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.zeros_like(u)
+        kg = np.zeros(nnz)
+        _check(lib().ipf_sm_assemble_tangent(self._h, _p(u, _dp), _p(r, _dp), _p(kg, _dp)),
+               what="ipf_sm_assemble_tangent")
+        return r, kg

@@ -17,6 +17,7 @@ SOURCES = [
     "ipf_kernels.hip",
     "ipf_solver.hip",
     "ipf_kernels_fmt.hip",
+    "ipf_assemble.hip",
     "ipf_device.hip",
     "TPZCompMesh.cpp",
     "TPZIntPoints.cpp",

@@ -19,6 +19,10 @@ public:
     TPZSYsmpMatrix<STATE>* Create();
     // rhs = f_lin - sum B^T (w sigma(B u)) for u = cmesh->Solution() (.cpp:137-154).
     void Assemble(TPZFMatrix<STATE>& rhs);
+    // K's values = the tangent at u = cmesh->Solution(), rhs as above
+    // (.cpp:98-135; K from Create(); the reference's penalty-BC part
+    // fSparseMatrixLinear has no counterpart: Dirichlet DoFs are eliminated).
+    void Assemble(TPZSYsmpMatrix<STATE>& K, TPZFMatrix<STATE>& rhs);
     // Builds the integrator data and uploads it without the CSR pattern
     // (enough for residual evaluations).
     void SetUpDataStructure();

@@ -26,6 +26,11 @@ public:
 
     // K_e = sum_ip B_ip^T Dep_ip B_ip (host, TPZNumericalIntegrator.cpp:236-270)
     void ComputeTangentMatrix(int64_t iel, TPZFMatrix<REAL>& Dep, TPZFMatrix<REAL>& K);
+    // TPZNumericalIntegrator.cpp:131-214 on the GPU: Kg = the CSR values of
+    // sum_e B_e^T (w Dep) B_e (zeroed first) and rhs = f_lin - B^T (w sigma)
+    // (the reference adds f_lin in the caller).  Needs FillLIndexes.
+    void KAssembly(TPZFMatrix<REAL>& solution, TPZVecGPU<STATE>& Kg, TPZVecGPU<STATE>& rhs);
+    void KAssembly(TPZFMatrix<REAL>& solution, TPZVec<STATE>& Kg, TPZFMatrix<STATE>& rhs);
 
     // Setup (TPZNumericalIntegrator.cpp:272-536)
     void SetElementIndexes(TPZVec<int> element_indexes) { fElementIndex = element_indexes; }
@@ -51,6 +56,7 @@ public:
     TPZVec<int64_t>& FirstColorIndex() { return fFirstColorIndex; }
     TPZVec<int64_t>& ColorLSequence() { return fColorLSequence; }
     TPZVec<int64_t>& FirstColorLIndex() { return fFirstColorLIndex; }
+    int64_t NnzK() const { return fNnzK; }
 
 private:
     TPZVec<int> fElementIndex;
@@ -64,6 +70,8 @@ private:
     TPZVec<int64_t> fFirstColorIndex;
     TPZVec<int64_t> fColorLSequence;
     TPZVec<int64_t> fFirstColorLIndex;
+    int64_t fNnzK = 0;               // CSR values of the pattern FillLIndexes saw
+    bool fPatternOnDevice = false;
     TPZCompMesh* fMesh = nullptr;
     std::shared_ptr<ipf_ctx> fCtx;
 };

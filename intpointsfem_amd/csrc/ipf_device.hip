@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -140,6 +141,7 @@ struct Group {
     int64_t b_off = 0;
     int64_t dof_off = 0;
     int64_t f_off[FMT_COUNT] = {0, 0, 0, 0};  // first value of the group in each B format
+    int64_t l_off = 0;                         // first CSR position index of the group (tangent pattern)
 };
 
 template <typename T>
@@ -201,6 +203,12 @@ struct ipf_ctx {
     int tfmt = FMT_D64;  // B format of the tangent product
     bool compact_ok = false;  // every block has the 2D elasticity pattern
     bool have_tangent = false;
+
+    // tangent CSR pattern (ipf_set_tangent_pattern): per group element, the
+    // CSR positions of its packed upper triangle
+    int* dLidx = nullptr;
+    int64_t nnz_k = 0;
+    bool have_pattern = false;
 };
 
 namespace {
@@ -265,6 +273,9 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dPartial);
     dev_free(ctx->dScal);
     for (int f = 1; f < FMT_COUNT; f++) dev_free(ctx->dBf[f]);
+    dev_free(ctx->dLidx);
+    ctx->nnz_k = 0;
+    ctx->have_pattern = false;
     ctx->rfmt = ctx->tfmt = FMT_D64;
     ctx->compact_ok = false;
     ctx->groups.clear();
@@ -871,6 +882,81 @@ int ipf_residual_tangent(ipf_ctx* ctx, const double* u_dev, double* r_dev) {
         launch_transpose(g.shape, group_dev(ctx, g), ctx->dSigw, r_dev, ctx->dOrig, ctx->stream);
     ctx->have_tangent = true;
     return launch_status(ctx, "residual_tangent");
+}
+
+int ipf_set_tangent_pattern(ipf_ctx* ctx, const int* el_color_index, const int64_t* color_l_sequence, int64_t nnz) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!el_color_index || !color_l_sequence) return fail(ctx, IPF_EINVAL, "ipf_set_tangent_pattern: NULL pointer");
+    if (nnz < 1 || nnz > INT32_MAX)
+        return fail(ctx, IPF_EINVAL, "ipf_set_tangent_pattern: nnz %lld outside [1, 2^31 - 1]", (long long)nnz);
+    const int64_t n = ctx->nblocks;
+    std::vector<int> grp(n, -1);
+    std::vector<int64_t> pos(n, 0);
+    int64_t total = 0;
+    for (size_t gi = 0; gi < ctx->groups.size(); gi++) {
+        Group& g = ctx->groups[gi];
+        const int64_t ce = shape_ce(g.shape);
+        g.l_off = total;
+        total += (int64_t)g.elems.size() * (ce * (ce + 1) / 2);
+        for (size_t k = 0; k < g.elems.size(); k++) {
+            grp[g.elems[k]] = (int)gi;
+            pos[g.elems[k]] = (int64_t)k;
+        }
+    }
+    std::vector<int> L;
+    std::vector<char> seen(n, 0);
+    try {
+        L.assign(total, 0);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, IPF_EINVAL, "ipf_set_tangent_pattern: host allocation failed");
+    }
+    int prev_color = -1;
+    int64_t off = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t e = el_color_index[i];
+        if (e < 0 || e >= n || seen[e])
+            return fail(ctx, IPF_EINVAL, "el_color_index[%lld] = %lld is not a permutation entry", (long long)i,
+                        (long long)e);
+        seen[e] = 1;
+        const Group& g = ctx->groups[grp[e]];
+        if (g.color < prev_color)
+            return fail(ctx, IPF_EINVAL, "el_color_index is not in color order at position %lld", (long long)i);
+        prev_color = g.color;
+        const int64_t ce = shape_ce(g.shape);
+        const int64_t nt = ce * (ce + 1) / 2;
+        int* dst = L.data() + g.l_off + pos[e] * nt;
+        for (int64_t t = 0; t < nt; t++) {
+            const int64_t l = color_l_sequence[off + t];
+            if (l < 0 || l >= nnz)
+                return fail(ctx, IPF_EINVAL, "color_l_sequence[%lld] = %lld outside [0, %lld)", (long long)(off + t),
+                            (long long)l, (long long)nnz);
+            dst[t] = (int)l;
+        }
+        off += nt;
+    }
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dev_free(ctx->dLidx);
+    ctx->have_pattern = false;
+    if ((rc = dev_alloc(ctx, &ctx->dLidx, total))) return rc;
+    if (total) IPF_HIP(ctx, hipMemcpy(ctx->dLidx, L.data(), sizeof(int) * (size_t)total, hipMemcpyHostToDevice));
+    ctx->nnz_k = nnz;
+    ctx->have_pattern = true;
+    return IPF_OK;
+}
+
+int ipf_assemble_tangent(ipf_ctx* ctx, const double* u_dev, double* r_dev, double* kg_dev) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!kg_dev) return fail(ctx, IPF_EINVAL, "ipf_assemble_tangent: NULL pointer");
+    if (!ctx->have_pattern) return fail(ctx, IPF_ESTATE, "no tangent pattern: call ipf_set_tangent_pattern first");
+    if ((rc = ipf_residual_tangent(ctx, u_dev, r_dev))) return rc;
+    launch_fill(ctx->nnz_k, kg_dev, 0.0, ctx->stream);
+    for (const auto& g : ctx->groups)
+        launch_assemble_K(g.shape, group_dev(ctx, g), ctx->dDep, ctx->npad, ctx->dLidx + g.l_off, kg_dev,
+                          ctx->stream);
+    return launch_status(ctx, "assemble_tangent");
 }
 
 int ipf_tangent_apply(ipf_ctx* ctx, const double* v_dev, double* y_dev) {

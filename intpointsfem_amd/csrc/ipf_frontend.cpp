@@ -22,6 +22,7 @@ struct ipf_cmesh {
 struct ipf_sm {
     ipf_cmesh* mesh;
     std::unique_ptr<TPZElastoPlasticIntPointsStructMatrix> sm;
+    std::unique_ptr<TPZSYsmpMatrix<STATE>> K;  // from Create(), on demand
 };
 
 namespace {
@@ -145,7 +146,7 @@ int ipf_sm_create(ipf_cmesh* m, int device, ipf_sm** out) {
     return guarded([&] {
         auto s = std::make_unique<ipf_sm>();
         s->mesh = m;
-        s->sm = std::make_unique<TPZElastoPlasticIntPointsStructMatrix>(m->mesh.get(), device < 0 ? 0 : device);
+        s->sm = std::make_unique<TPZElastoPlasticIntPointsStructMatrix>(m->mesh.get(), device);
         s->sm->SetUpDataStructure();
         if (device >= 0) s->sm->TransferDataToGPU();
         *out = s.release();
@@ -215,6 +216,50 @@ int ipf_sm_assemble_rhs(ipf_sm* sm, const double* u_host, double* rhs_host) {
         TPZFMatrix<STATE> rhs;
         sm->sm->Assemble(rhs);
         std::memcpy(rhs_host, rhs.data(), sizeof(double) * m->NEquations());
+        return IPF_OK;
+    });
+}
+
+int ipf_sm_create_tangent(ipf_sm* sm, int64_t* nnz) {
+    if (!sm || !nnz) return null_arg("ipf_sm_create_tangent");
+    return guarded([&] {
+        if (!sm->K) sm->K.reset(sm->sm->Create());
+        *nnz = sm->K->JA().size();
+        return IPF_OK;
+    });
+}
+
+int ipf_sm_export_tangent_pattern(const ipf_sm* sm, int64_t* ia, int64_t* ja, int64_t* color_l_sequence,
+                                  int64_t* first_color_l_index) {
+    if (!sm) return null_arg("ipf_sm_export_tangent_pattern");
+    if (!sm->K) {
+        ipf_internal_set_error("ipf_sm_export_tangent_pattern: call ipf_sm_create_tangent first");
+        return IPF_ESTATE;
+    }
+    auto& in = sm->sm->Integrator();
+    if (ia) std::memcpy(ia, sm->K->IA().data(), sizeof(int64_t) * sm->K->IA().size());
+    if (ja) std::memcpy(ja, sm->K->JA().data(), sizeof(int64_t) * sm->K->JA().size());
+    if (color_l_sequence)
+        std::memcpy(color_l_sequence, in.ColorLSequence().data(), sizeof(int64_t) * in.ColorLSequence().size());
+    if (first_color_l_index)
+        std::memcpy(first_color_l_index, in.FirstColorLIndex().data(),
+                    sizeof(int64_t) * in.FirstColorLIndex().size());
+    return IPF_OK;
+}
+
+int ipf_sm_assemble_tangent(ipf_sm* sm, const double* u_host, double* rhs_host, double* kg_host) {
+    if (!sm || !u_host || !rhs_host || !kg_host) return null_arg("ipf_sm_assemble_tangent");
+    if (!sm->K) {
+        ipf_internal_set_error("ipf_sm_assemble_tangent: call ipf_sm_create_tangent first");
+        return IPF_ESTATE;
+    }
+    return guarded([&] {
+        TPZCompMesh* m = sm->mesh->mesh.get();
+        std::memcpy(m->Solution().data(), u_host, sizeof(double) * m->NEquations());
+        TPZFMatrix<STATE> rhs;
+        sm->sm->Assemble(*sm->K, rhs);
+        std::memcpy(rhs_host, rhs.data(), sizeof(double) * m->NEquations());
+        std::memcpy(kg_host, sm->K->A().data(), sizeof(double) * sm->K->A().size());
         return IPF_OK;
     });
 }

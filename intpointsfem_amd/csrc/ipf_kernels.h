@@ -82,6 +82,12 @@ void launch_tangent(int shape, const GroupDev& g, const double* v, double* y, co
 void launch_tangent_diag(int shape, const GroupDev& g, double* diag, const double* dep_soa, int64_t npad,
                          hipStream_t stream);
 
+// Tangent stiffness into CSR values (ipf_assemble.hip): kg[lidx[e * NT + t]] +=
+// entry t of element e's packed upper triangle (NT = CE (CE + 1) / 2), for
+// the elements of one group (B: the dense FP64 tiles).
+void launch_assemble_K(int shape, const GroupDev& g, const double* dep_soa, int64_t npad, const int* lidx,
+                       double* kg, hipStream_t stream);
+
 // Format variants (ipf_kernels_fmt.hip): Bv points at the group's first tile
 // in format fmt (FMT_C64, FMT_D32 or FMT_C32).
 void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,

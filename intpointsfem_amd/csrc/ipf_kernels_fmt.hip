@@ -108,48 +108,57 @@ __device__ __forceinline__ void strain_fmt(const typename FC<SHAPE, FMT>::T* b, 
     }
 }
 
-template <int SHAPE, int FMT>
-__device__ __forceinline__ void bt_fmt(const typename FC<SHAPE, FMT>::T* b, const double* x,
-                                       typename FC<SHAPE, FMT>::T* row) {
+// Columns [J0, J1) of one point's B^T (w sigma) partials into row[j - J0].
+// Dense: rows summed in order per column; compact: the same sums without the
+// zero terms.  The residual and tangent kernels stage half the columns at a
+// time, which halves the LDS a wave holds (one wave per workgroup) so that
+// registers, not LDS, bound the waves per CU.
+template <int SHAPE, int FMT, int J0, int J1>
+__device__ __forceinline__ void bt_cols(const typename FC<SHAPE, FMT>::T* b, const double* x,
+                                        typename FC<SHAPE, FMT>::T* row) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
     const T x0 = T(x[0]), x1 = T(x[1]), x2 = T(x[2]);
-    if constexpr (FmtTraits<FMT>::COMPACT) {
 #pragma unroll
-        for (int a = 0; a < C::CE / 2; a++) {
+    for (int j = J0; j < J1; j++) {
+        T g;
+        if constexpr (FmtTraits<FMT>::COMPACT) {
+            const int a = j / 2;
             const T dx = b[2 * a], dy = b[2 * a + 1];
-            T g = dx * x0;
-            g = g + dy * x1;
-            row[2 * a] = g;
-            T h = dx * x1;
-            h = h + dy * x2;
-            row[2 * a + 1] = h;
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) {
-            T g = b[j] * x0;
+            if ((j & 1) == 0) {
+                g = dx * x0;
+                g = g + dy * x1;
+            } else {
+                g = dx * x1;
+                g = g + dy * x2;
+            }
+        } else {
+            g = b[j] * x0;
             g = g + b[C::CE + j] * x1;
             g = g + b[2 * C::CE + j] * x2;
-            row[j] = g;
         }
+        row[j - J0] = g;
     }
 }
 
-template <int SHAPE, int FMT>
-__device__ __forceinline__ void reduce_fmt(const typename FC<SHAPE, FMT>::T* L, const GroupDev& g, int64_t tile,
-                                           int lane, double* __restrict__ r) {
+// Per element and column: the sum of its points' partials in point order,
+// added negated to r (TPZNumericalIntegrator::MultiplyTranspose's
+// y = -B^T x; the color's elements share no DoF, so no atomics).
+template <int SHAPE, int FMT, int J0, int J1>
+__device__ __forceinline__ void reduce_cols(const typename FC<SHAPE, FMT>::T* L, const GroupDev& g, int64_t tile,
+                                            int lane, double* __restrict__ r) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
-    for (int task = lane; task < C::ET * C::CE; task += kWave) {
-        const int e2 = task / C::CE;
-        const int j = task - e2 * C::CE;
+    constexpr int W = J1 - J0;
+    for (int task = lane; task < C::ET * W; task += kWave) {
+        const int e2 = task / W;
+        const int jj = task - e2 * W;
         const int64_t elem = tile * C::ET + e2;
         if (elem < g.nelem) {
-            T f = L[(e2 * C::NIP) * C::CE + j];
+            T f = L[(e2 * C::NIP) * W + jj];
 #pragma unroll
-            for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e2 * C::NIP + ip) * C::CE + j];
-            const int d = g.dof[elem * C::CE + j];
+            for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e2 * C::NIP + ip) * W + jj];
+            const int d = g.dof[elem * C::CE + J0 + jj];
             r[d] = r[d] + (-double(f));
         }
     }
@@ -161,7 +170,7 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
                PointState st, MatConst m) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
-    __shared__ T lds[C::TP * C::CE];
+    __shared__ T lds[C::TP * (C::CE - C::CE / 2)];
     const int lane = threadIdx.x;
     const int64_t tile = blockIdx.x;
     const int el = lane / C::NIP;
@@ -203,9 +212,14 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
             st.branch[P] = (signed char)br;
         }
     }
-    if (slot) bt_fmt<SHAPE, FMT>(b, ws, lds + lane * C::CE);
+    constexpr int H = C::CE / 2;
+    if (slot) bt_cols<SHAPE, FMT, 0, H>(b, ws, lds + lane * H);
     __syncthreads();
-    reduce_fmt<SHAPE, FMT>(lds, g, tile, lane, r);
+    reduce_cols<SHAPE, FMT, 0, H>(lds, g, tile, lane, r);
+    __syncthreads();
+    if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, ws, lds + lane * (C::CE - H));
+    __syncthreads();
+    reduce_cols<SHAPE, FMT, H, C::CE>(lds, g, tile, lane, r);
 }
 
 // y += sum_e B_e^T (w Dep) B_e v_e with B in format FMT (FP32 formats: FP32
@@ -216,7 +230,7 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
               const double* __restrict__ dep, int64_t npad) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
-    __shared__ T lds[C::TP * C::CE];
+    __shared__ T lds[C::TP * (C::CE - C::CE / 2)];
     const int lane = threadIdx.x;
     const int64_t tile = blockIdx.x;
     const int el = lane / C::NIP;
@@ -249,12 +263,17 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
             double a = ld_stream(&dep[(3 * k) * npad + P]) * e3[0];
             a = a + ld_stream(&dep[(3 * k + 1) * npad + P]) * e3[1];
             a = a + ld_stream(&dep[(3 * k + 2) * npad + P]) * e3[2];
-            x[k] = -a;  // reduce_fmt adds -(sum); y accumulates +K v
+            x[k] = -a;  // reduce_cols adds -(sum); y accumulates +K v
         }
     }
-    if (slot) bt_fmt<SHAPE, FMT>(b, x, lds + lane * C::CE);
+    constexpr int H = C::CE / 2;
+    if (slot) bt_cols<SHAPE, FMT, 0, H>(b, x, lds + lane * H);
     __syncthreads();
-    reduce_fmt<SHAPE, FMT>(lds, g, tile, lane, y);
+    reduce_cols<SHAPE, FMT, 0, H>(lds, g, tile, lane, y);
+    __syncthreads();
+    if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, x, lds + lane * (C::CE - H));
+    __syncthreads();
+    reduce_cols<SHAPE, FMT, H, C::CE>(lds, g, tile, lane, y);
 }
 
 // Builds format FMT from the dense FP64 tile layout of one group (device side).

@@ -118,6 +118,53 @@ def test_sizes_dofs_and_colors(p):
     np.testing.assert_array_equal(ix["first_color_index"], np.concatenate([[0], np.cumsum(np.bincount(col))]))
 
 
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_tangent_pattern_and_l_indexes(p):
+    """Create() / FillLIndexes: the upper-triangle CSR pattern is exactly the
+    set of DoF pairs sharing an element (rows sorted, diagonal present), and
+    fColorLSequence maps each element's packed K_e(a, c), a <= c, in
+    fElColorIndex order, to the CSR position of (min, max) of its DoFs --
+    checked against a dictionary built independently."""
+    m = ipf.CompMesh(5, 4, p, 0.1, 4.0, 1.2)
+    m.set_material(5000, 0.2, 10, 0.5, 0.5)
+    sm, b, ix = host_sm(m)
+    s = sm.sizes()
+    pat = sm.tangent_pattern()
+    ia, ja = pat["ia"], pat["ja"]
+    neq = s["neq"]
+    C = b["col_sizes"]
+    dof = ix["dof_indexes"]
+    pairs = set()
+    for e in range(s["nblocks"]):
+        d = dof[b["col_first"][e]:b["col_first"][e + 1]]
+        for x in d:
+            for y in d:
+                if x <= y:
+                    pairs.add((int(x), int(y)))
+    assert len(ja) == len(pairs) and ia[0] == 0 and ia[-1] == len(ja)
+    pos = {}
+    for row in range(neq):
+        cols = ja[ia[row]:ia[row + 1]]
+        assert (np.diff(cols) > 0).all() and cols[0] == row
+        for k, c in enumerate(cols):
+            pos[(row, int(c))] = ia[row] + k
+    assert set(pos) == pairs
+    expect = []
+    for e in ix["el_color_index"]:
+        d = dof[b["col_first"][e]:b["col_first"][e + 1]]
+        for a in range(C[e]):
+            for c in range(a, C[e]):
+                expect.append(pos[(min(d[a], d[c]), max(d[a], d[c]))])
+    np.testing.assert_array_equal(pat["color_l_sequence"], np.array(expect))
+    tri = (C * C + C) // 2
+    col = np.zeros(s["nblocks"], dtype=np.int64)
+    fci = ix["first_color_index"]
+    for c in range(s["ncolor"]):
+        col[ix["el_color_index"][fci[c]:fci[c + 1]]] = c
+    np.testing.assert_array_equal(pat["first_color_l_index"],
+                                  np.concatenate([[0], np.cumsum(np.bincount(col, weights=tri).astype(np.int64))]))
+
+
 def test_config1_sizes():
     """40 x 40 Q2 with geometric ratio 1.1: 1,600 elements, 14,400 points,
     13,122 equations; first radial cell h0 = 3.9*0.1/(1.1^40 - 1)."""
