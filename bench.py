@@ -101,12 +101,101 @@ def launch_times(ctx, u, r, nrep):
     return acc / nrep, pts
 
 
+def run_batch(args):
+    """The 64 uneven wellbore instances (tests/golden/batch64_instances.json,
+    drawn with seed 0) in one context: one launch per (color, shape) for the
+    whole batch.  Same JSON line; cpu_baseline times the oracle on whole
+    instances, in table order, until --cpu-seconds."""
+    import numpy as np
+    import torch
+
+    import intpointsfem_amd as ipf
+    from intpointsfem_amd import batch
+    from intpointsfem_amd import wellbore as wb
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    table = batch.draw_instances(64, seed=0)
+    meshes = [batch.instance_mesh(i) for i in table]
+    B = batch.Batch(meshes, device=0)
+    ctx = B.ctx
+    npts, neq = B.npts, B.neq
+    log(f"[bench] batch of {B.n} instances: {len(B.row_sizes)} elements, {npts} points, neq {neq}; setup + upload "
+        f"{time.time() - t0:.1f}s, {ctx.info()['device_bytes'] / 1e9:.2f} GB on device")
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+    us = [wb.lame_displacement(m.nodes(), wb.normal_stress(wb.NSTEPS, wb.NSTEPS, inst["pw_final"], wb.SIGMA0),
+                               E_=inst["E"], nu=inst["nu"]) for inst, m in zip(table, meshes)]
+    u_host = np.concatenate(us)
+    u = torch.as_tensor(u_host, dtype=torch.float64, device=dev)
+    r = torch.zeros(neq, dtype=torch.float64, device=dev)
+    ctx.set_update_mem(True)
+    ctx.residual(u, r)
+    ctx.set_update_mem(False)
+    ctx.synchronize()
+    _, _, mt, br = ctx.get_state()
+    ctx.set_state(np.zeros(6 * npts))
+    branch_counts = {ipf.BRANCH_NAMES[i]: int((br == i).sum()) for i in range(5)}
+    log(f"[bench] branches: {branch_counts}")
+    fmt0 = ctx.b_format()[0]
+    total_ms = time_residuals(ctx, u, r, stream, args.steps, args.warmup)
+    ms_per_step = total_ms / args.steps
+    value = npts * args.steps / (total_ms / 1e3) / 1e6
+    log(f"[bench] {args.steps} batched residuals ({fmt0} B): {ms_per_step:.4f} ms, {value:.1f} Mint-pts/s")
+    blocks = {"row_sizes": B.row_sizes, "col_sizes": B.col_sizes}
+    nrep = max(3, min(args.steps, 20))
+    roof = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt0)
+    log(f"[bench] launches {', '.join(f'{x:.4f}' for x in roof['color_ms'])} ms; {roof['achieved']:.0f} GB/s on "
+        f"plain FP64 bytes, {roof['achieved_stored']:.0f} GB/s on stored bytes")
+    roofline = {"bound": "hbm", "achieved": round(roof["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(roof["achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_residual_fmt, one launch per (color, element order) over all instances",
+                "avg_launch_ms": round(roof["avg_launch_ms"], 5),
+                "achieved_on_stored_bytes": round(roof["achieved_stored"], 1),
+                "frac_on_stored_bytes": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        nthreads = max(1, min(16, os.cpu_count() or 1))
+        done_pts, spent, ninst = 0, 0.0, 0
+        for inst, m, ui in zip(table, meshes, us):
+            sm = ipf.StructMatrix(m, device=-1)
+            b = sm.export_blocks()
+            ix = sm.export_indexes()
+            s = sm.sizes()
+            Bo = O.Blocks(b["row_sizes"], b["col_sizes"], b["matrix_position"], b["row_first"], b["col_first"],
+                          b["storage"])
+            d = m.material()
+            t = time.time()
+            O.residual(Bo, ix["dof_indexes"], ix["color_indexes"], s["ncolor"], s["neq"], ix["weight"], d["lambda"],
+                       d["mu"], d["phi"], d["psi"], d["cohesion"], ui, f_lin=ix["rhs_linear"], nthreads=nthreads)
+            spent += time.time() - t
+            done_pts += s["npts"]
+            ninst += 1
+            if spent >= args.cpu_seconds:
+                break
+        cpu = {"value": round(done_pts / spent / 1e6, 3), "unit": "Mint-pts/s", "cores": nthreads, "kind": "port",
+               "sample": f"one residual of each of the first {ninst} instances ({done_pts} points) by the C oracle "
+                         f"with {nthreads} OpenMP threads, {spent:.1f} s"}
+        log(f"[bench] cpu oracle: {cpu['value']:.2f} Mint-pts/s ({ninst} instances)")
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "Mint-pts/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "plain FP64 batched residual, 64 uneven wellbore instances (seed-0 table)",
+                      "b_format": fmt0, "instances": B.n, "elements": int(len(B.row_sizes)), "int_points": npts,
+                      "neq": neq, "colors": B.ncolor, "branches": branch_counts,
+                      "state": "u = each instance's linear-elastic thick-cylinder displacement, zero plastic strain"},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--mesh", default="800x800-q2")
+    ap.add_argument("--mesh", default="800x800-q2",
+                    help="800x800-q2 (default), 800x800-q3, 40x40-q2, or batch64 (64 uneven instances)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-formats", action="store_true", help="skip the compact / FP32 B format variants")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's timed work")
@@ -116,12 +205,16 @@ def main():
                          "committed plastic state")
     ap.add_argument("--newton-rtol", type=float, default=1e-8)
     ap.add_argument("--cg-rtol", type=float, default=1e-3)
+    ap.add_argument("--solve-tangent-format", default="compact-fp32",
+                    help="B format of the CG matvecs of the --state newton solve (the residual stays FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the path runs on a single "
               "MI355X; multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
 
+    if args.mesh == "batch64":
+        return run_batch(args)
     import numpy as np
     import torch
 
@@ -159,6 +252,9 @@ def main():
     else:
         # 10 load steps of Newton + Jacobi-PCG on the GPU; the state is committed after each
         ctx.set_dirichlet_mask(mesh.dirichlet_mask())
+        fmt_default = ctx.b_format()
+        # mixed Newton: FP64 residual, tangent products in the chosen format
+        ctx.set_b_format(fmt_default[0], tangent=args.solve_tangent_format)
         f_unit = sm.export_indexes()["rhs_linear"] / wb.normal_stress(wb.NSTEPS)
         u = torch.zeros(neq, dtype=torch.float64, device=dev)
         t1 = time.time()
@@ -169,9 +265,10 @@ def main():
             steps.append(st)
             log(f"[bench] load step {k}: newton {int(st['iterations'])}, |r|/|r0| {st['rel_residual']:.2e}, "
                 f"cg {int(st['cg_iterations'])}, {time.time() - t1:.1f}s")
+        ctx.set_b_format(*fmt_default)
         u_host = u.cpu().numpy()
         _, _, mt, br = ctx.get_state()
-        newton_info = {"load_steps": wb.NSTEPS, "newton_iterations": [int(s["iterations"]) for s in steps],
+        newton_info = {"solve_tangent_format": args.solve_tangent_format, "load_steps": wb.NSTEPS, "newton_iterations": [int(s["iterations"]) for s in steps],
                        "final_rel_residual": steps[-1]["rel_residual"],
                        "converged": all(s["converged"] == 1.0 for s in steps),
                        "cg_iterations": int(sum(s["cg_iterations"] for s in steps)),

@@ -90,6 +90,16 @@ int ipf_transfer_data(ipf_ctx *ctx, int64_t nblocks, const int *row_sizes, const
  * Angles in radians. */
 int ipf_set_material(ipf_ctx *ctx, double lambda, double mu, double mc_phi, double mc_psi, double mc_cohesion);
 
+/* Several materials (the reference has one, FindMaterial(1),
+ * TPZNumericalIntegrator.cpp:446): params [5*nmat] = (lambda, mu, phi, psi,
+ * c) per material; ipf_set_material is the one-entry table.  The element ->
+ * material map is given before the data: ipf_set_element_materials(ctx,
+ * nblocks, ids [nblocks]) applies to the next ipf_transfer_data, which keeps
+ * every wavefront tile within one material (each color/shape launch then
+ * covers all materials).  Without it every element has material 0. */
+int ipf_set_material_table(ipf_ctx *ctx, int nmat, const double *params);
+int ipf_set_element_materials(ipf_ctx *ctx, int64_t nblocks, const int *element_material);
+
 /* fRhsLinear / dRhsLinear (TPZElastoPlasticIntPointsStructMatrix.cpp:63-68,156-167), [neq] host array. */
 int ipf_set_rhs_linear(ipf_ctx *ctx, const double *f_lin_host);
 

@@ -76,6 +76,8 @@ def lib():
         "ipf_synchronize": (_int, [_vp]),
         "ipf_transfer_data": (_int, [_vp, _i64, _ip, _ip, _lp, _ip, _ip, _dp, _ip, _ip, _i64, _i64, _dp]),
         "ipf_set_material": (_int, [_vp, _d, _d, _d, _d, _d]),
+        "ipf_set_material_table": (_int, [_vp, _int, _dp]),
+        "ipf_set_element_materials": (_int, [_vp, _i64, _ip]),
         "ipf_set_rhs_linear": (_int, [_vp, _dp]),
         "ipf_set_update_mem": (_int, [_vp, _int]),
         "ipf_info": (_int, [_vp, _lp]),
@@ -264,6 +266,16 @@ class DeviceContext:
 
     def set_material(self, lam, mu, phi, psi, cohesion):
         self._c(lib().ipf_set_material(self._h, lam, mu, phi, psi, cohesion), "ipf_set_material")
+
+    def set_material_table(self, params):
+        """params [nmat, 5]: (lambda, mu, phi, psi, cohesion) per material (radians)."""
+        q = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, 5)
+        self._c(lib().ipf_set_material_table(self._h, q.shape[0], _p(q, _dp)), "ipf_set_material_table")
+
+    def set_element_materials(self, ids):
+        """Material id per block; applies to the next transfer_data."""
+        a = np.ascontiguousarray(ids, dtype=np.int32)
+        self._c(lib().ipf_set_element_materials(self._h, a.size, _p(a, _ip)), "ipf_set_element_materials")
 
     def set_rhs_linear(self, f):
         f = np.ascontiguousarray(f, dtype=np.float64)

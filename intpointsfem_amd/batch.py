@@ -1,0 +1,135 @@
+"""Many independent wellbore instances of uneven size, order and material in
+one device context.
+
+The instances' integrator data are concatenated: blocks, points and
+equations are stacked with offsets, every element keeps its instance's color
+(instances share no DoF, so the union of the colorings is a coloring), and
+each element carries its instance's material id.  The device context then
+runs one launch per (color, element shape) for the whole batch -- at most
+4 colors x 3 shapes = 12 residual launches however many instances there are
+-- with every wavefront tile inside one instance's material
+(ipf_set_element_materials).  Residuals, branch flags and plastic strains of
+an instance are those of the instance alone: each residual entry still
+receives its contributions in color order, one per launch.
+
+The reference solves one mesh with one material (FindMaterial(1),
+TPZNumericalIntegrator.cpp:446); the batch is an extension built on the same
+entry points.
+"""
+import math
+
+import numpy as np
+
+from . import DeviceContext, StructMatrix
+from . import wellbore as wb
+
+SIGMA0 = wb.SIGMA0
+
+
+def draw_instances(n=64, seed=0):
+    """The batched-instances table: N_r = N_theta ~ U{50..400}, p ~ U{1,2,3},
+    phi ~ U(20, 40) deg with psi = phi, c ~ LogU(1, 20) MPa,
+    E ~ U(2000, 20000) MPa, nu ~ U(0.15, 0.3), final wellbore pressure
+    ~ U(-30, -15) MPa; numpy PCG64(seed), drawn column by column."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    N = rng.integers(50, 400, size=n, endpoint=True)
+    p = rng.integers(1, 3, size=n, endpoint=True)
+    phi = rng.uniform(20.0, 40.0, size=n)
+    c = np.exp(rng.uniform(math.log(1.0), math.log(20.0), size=n))
+    E = rng.uniform(2000.0, 20000.0, size=n)
+    nu = rng.uniform(0.15, 0.3, size=n)
+    pw = rng.uniform(-30.0, -15.0, size=n)
+    return [dict(N=int(N[i]), p=int(p[i]), phi_deg=float(phi[i]), cohesion=float(c[i]), E=float(E[i]),
+                 nu=float(nu[i]), pw_final=float(pw[i])) for i in range(n)]
+
+
+def instance_mesh(inst, step=wb.NSTEPS, nsteps=wb.NSTEPS):
+    """CompMesh of one instance: the wellbore geometry with N x N cells of
+    order p, the radial grading profile of the 40-cell mesh (ratio 1.1^(40/N))
+    and the instance's material; inner-wall load of load step `step`."""
+    from . import CompMesh
+    N, p = inst["N"], inst["p"]
+    m = CompMesh(N, N, p, wb.R_IN, wb.R_OUT, 1.1 ** (40.0 / N))
+    phi = math.radians(inst["phi_deg"])
+    m.set_material(inst["E"], inst["nu"], inst["cohesion"], phi, phi)
+    m.set_normal_stress(wb.normal_stress(step, nsteps, inst["pw_final"], SIGMA0), 0.0)
+    return m
+
+
+def concat_instances(meshes):
+    """Host side of a batch: the instances' IrregularBlocks, DoF / color
+    indexes, weights and f_lin stacked with offsets (the arrays
+    ipf_transfer_data takes), the per-block material id and the material
+    table.  Needs no GPU."""
+    parts = []
+    for m in meshes:
+        sm = StructMatrix(m, device=-1)
+        parts.append((m, sm.sizes(), sm.export_blocks(), sm.export_indexes()))
+    n = len(parts)
+    nb = np.array([p[1]["nblocks"] for p in parts], dtype=np.int64)
+    neq = np.array([p[1]["neq"] for p in parts], dtype=np.int64)
+    npt = np.array([p[1]["npts"] for p in parts], dtype=np.int64)
+    rows = np.array([p[1]["rows"] for p in parts], dtype=np.int64)
+    cols = np.array([p[1]["cols"] for p in parts], dtype=np.int64)
+    nsto = np.array([p[1]["nstorage"] for p in parts], dtype=np.int64)
+
+    def off(a):
+        return np.concatenate([[0], np.cumsum(a)]).astype(np.int64)
+
+    eq0, r0, c0, s0, pt0 = off(neq), off(rows), off(cols), off(nsto), off(npt)
+    ncolor = max(p[1]["ncolor"] for p in parts)
+    NEQ = int(eq0[-1])
+    if r0[-1] >= 2 ** 31 or c0[-1] >= 2 ** 31 or NEQ * ncolor >= 2 ** 31:
+        raise ValueError("batch too large for the int32 row / column / color indexes of the block format")
+    dof = np.concatenate([p[3]["dof_indexes"].astype(np.int64) + eq0[i] for i, p in enumerate(parts)])
+    # color of every column entry from the instance's fColorIndexes = dof + color * neq_i
+    colr = np.concatenate([(p[3]["color_indexes"].astype(np.int64) - p[3]["dof_indexes"]) // neq[i]
+                           for i, p in enumerate(parts)])
+    params = []
+    for m, *_ in parts:
+        d = m.material()
+        params.append([d["lambda"], d["mu"], d["phi"], d["psi"], d["cohesion"]])
+    return dict(
+        row_sizes=np.concatenate([p[2]["row_sizes"] for p in parts]),
+        col_sizes=np.concatenate([p[2]["col_sizes"] for p in parts]),
+        matrix_position=np.concatenate([p[2]["matrix_position"][:-1] + s0[i] for i, p in enumerate(parts)]
+                                       + [[s0[-1]]]),
+        row_first=np.concatenate([p[2]["row_first"][:-1] + r0[i] for i, p in enumerate(parts)] + [[r0[-1]]]),
+        col_first=np.concatenate([p[2]["col_first"][:-1] + c0[i] for i, p in enumerate(parts)] + [[c0[-1]]]),
+        storage=np.concatenate([p[2]["storage"] for p in parts]),
+        dof_indexes=dof.astype(np.int32),
+        color_indexes=(dof + colr * NEQ).astype(np.int32),
+        ncolor=ncolor, neq=NEQ,
+        weight=np.concatenate([p[3]["weight"] for p in parts]),
+        rhs_linear=np.concatenate([p[3]["rhs_linear"] for p in parts]),
+        element_material=np.repeat(np.arange(n, dtype=np.int32), nb),
+        material_table=np.array(params),
+        eq_offsets=eq0, pt_offsets=pt0, block_offsets=off(nb))
+
+
+class Batch:
+    """Independent instances in one DeviceContext (see the module docstring).
+
+    meshes: CompMesh objects (material and boundary load set).  Attributes:
+    ctx, n, neq (total), npts, eq_offsets [n+1], pt_offsets [n+1] (each
+    instance's points in its reference order, stacked), rhs_linear."""
+
+    def __init__(self, meshes, device=0):
+        h = concat_instances(meshes)
+        self.ctx = DeviceContext(device=device)
+        self.ctx.set_element_materials(h["element_material"])
+        self.ctx.transfer_data(h["row_sizes"], h["col_sizes"], h["matrix_position"], h["row_first"], h["col_first"],
+                               h["storage"], h["dof_indexes"], h["color_indexes"], h["ncolor"], h["neq"],
+                               h["weight"])
+        self.ctx.set_material_table(h["material_table"])
+        self.ctx.set_rhs_linear(h["rhs_linear"])
+        self.n = len(meshes)
+        self.neq = h["neq"]
+        self.npts = int(h["pt_offsets"][-1])
+        self.ncolor = h["ncolor"]
+        self.eq_offsets = h["eq_offsets"]
+        self.pt_offsets = h["pt_offsets"]
+        self.rhs_linear = h["rhs_linear"]
+        self.row_sizes = h["row_sizes"]
+        self.col_sizes = h["col_sizes"]
+        self.meshes = list(meshes)

@@ -39,9 +39,10 @@ k_assemble_K(GroupDev g, const double* __restrict__ dep, int64_t npad, const int
     __shared__ double Ds[NIP * 9];
     __shared__ double DBs[NIP * NB];
     const int lane = threadIdx.x;
-    const int64_t elem = blockIdx.x;  // grid = g.nelem
+    const int64_t elem = blockIdx.x;  // grid = g.nelem element slots
     const int64_t tile = elem / ET;
     const int el = (int)(elem - tile * ET);
+    if (el >= tile_nvalid(g, tile)) return;  // padding slot
     // dense tile layout: value m of tile slot q at (m / 2) * 2 TP + 2 q + (m & 1)
     const double* Bt = g.B + tile * (int64_t)(NB * TP);
     for (int t = lane; t < NIP * NB; t += kWave) {

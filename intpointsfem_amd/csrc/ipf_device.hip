@@ -135,7 +135,12 @@ thread_local std::string g_last_error;
 struct Group {
     int color;
     int shape;
-    std::vector<int64_t> elems;  // reference block indexes, ascending
+    // element slots: reference block indexes sorted by (material, index), each
+    // material run padded with -1 to whole tiles
+    std::vector<int64_t> elems;
+    std::vector<int> tmeta;      // per tile: (material << 8) | valid elements
+    int64_t nreal = 0;           // real elements
+    int64_t tile0 = 0;           // first tile in the context's tile metadata
     int64_t ntiles = 0;
     int64_t pt0 = 0;
     int64_t b_off = 0;
@@ -160,9 +165,15 @@ struct ipf_ctx {
 
     // sizes
     int64_t nblocks = 0, npts = 0, rows = 0, cols = 0, neq = 0, ncolor = 0, nstorage = 0, npad = 0;
-    bool have_data = false, have_material = false;
+    bool have_data = false;
     bool update_mem = false;
-    MatConst mat{};
+    // material table (index = element material id) and the per-element ids
+    // that the next ipf_transfer_data groups by
+    std::vector<MatConst> mats;
+    MatConst* dMats = nullptr;
+    std::vector<int> elem_mat_next;
+    int max_mat = 0;              // largest material id of the transferred data
+    int* dTmeta = nullptr;
 
     std::vector<Group> groups;
 
@@ -274,6 +285,8 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dScal);
     for (int f = 1; f < FMT_COUNT; f++) dev_free(ctx->dBf[f]);
     dev_free(ctx->dLidx);
+    dev_free(ctx->dTmeta);
+    ctx->max_mat = 0;
     ctx->nnz_k = 0;
     ctx->have_pattern = false;
     ctx->rfmt = ctx->tfmt = FMT_D64;
@@ -334,6 +347,8 @@ GroupDev group_dev(const ipf_ctx* ctx, const Group& g, bool columns = false) {
     d.B = ctx->dB + g.b_off;
     d.dof = (columns ? ctx->dCol : ctx->dDof) + g.dof_off;
     d.nelem = (int64_t)g.elems.size();
+    d.tmeta = ctx->dTmeta + g.tile0;
+    d.mats = ctx->dMats;
     d.ntiles = g.ntiles;
     d.pt0 = g.pt0;
     return d;
@@ -359,10 +374,10 @@ const void* group_fmt_B(const ipf_ctx* ctx, const Group& g, int fmt) {
 // One residual launch for a group in the selected B format.
 void launch_group_residual(ipf_ctx* ctx, const Group& g, const double* u, double* r, const PointState& st) {
     if (ctx->rfmt == FMT_D64)
-        launch_residual(g.shape, ctx->update_mem, group_dev(ctx, g), u, r, st, ctx->mat, ctx->stream);
+        launch_residual(g.shape, ctx->update_mem, group_dev(ctx, g), u, r, st, ctx->stream);
     else
         launch_residual_fmt(ctx->rfmt, g.shape, ctx->update_mem, group_dev(ctx, g), group_fmt_B(ctx, g, ctx->rfmt),
-                            u, r, st, ctx->mat, ctx->stream);
+                            u, r, st, ctx->stream);
 }
 
 // One tangent (K v) launch for a group in the selected B format.
@@ -377,7 +392,9 @@ void launch_group_tangent(ipf_ctx* ctx, const Group& g, const double* v, double*
 int check_ready(ipf_ctx* ctx, bool need_material) {
     if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
     if (!ctx->have_data) return fail(ctx, IPF_ESTATE, "integrator data not transferred (call ipf_transfer_data)");
-    if (need_material && !ctx->have_material) return fail(ctx, IPF_ESTATE, "material not set (call ipf_set_material)");
+    if (need_material && (int)ctx->mats.size() <= ctx->max_mat)
+        return fail(ctx, IPF_ESTATE, "material %d not set (call ipf_set_material / ipf_set_material_table)",
+                    ctx->mats.empty() ? 0 : ctx->max_mat);
     return IPF_OK;
 }
 
@@ -446,6 +463,7 @@ int ipf_ctx_destroy(ipf_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     release_data(ctx);
+    dev_free(ctx->dMats);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return IPF_OK;
@@ -554,22 +572,54 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
         }
     }
 
-    // --- groups: color-major, then shape, elements ascending
+    // --- element materials (ipf_set_element_materials before this call; all 0 otherwise)
+    std::vector<int> emat(nblocks, 0);
+    if (!ctx->elem_mat_next.empty()) {
+        if ((int64_t)ctx->elem_mat_next.size() != nblocks)
+            return fail(ctx, IPF_EINVAL, "element materials were set for %lld blocks, the data has %lld",
+                        (long long)ctx->elem_mat_next.size(), (long long)nblocks);
+        emat = ctx->elem_mat_next;
+    }
+    int max_mat = 0;
+    for (int64_t e = 0; e < nblocks; e++) max_mat = std::max(max_mat, emat[e]);
+
+    // --- groups: color-major, then shape; inside a group elements by
+    // (material, index), every material run padded to whole tiles
     std::vector<Group> groups;
+    int64_t ntile_all = 0;
     for (int c = 0; c < ncolor; c++)
         for (int s = 0; s < SHAPE_COUNT; s++) {
+            std::vector<int64_t> es;
+            for (int64_t e = 0; e < nblocks; e++)
+                if (elem_color[e] == c && elem_shape[e] == s) es.push_back(e);
+            if (es.empty()) continue;
+            std::stable_sort(es.begin(), es.end(), [&](int64_t a, int64_t b) { return emat[a] < emat[b]; });
             Group g;
             g.color = c;
             g.shape = s;
-            for (int64_t e = 0; e < nblocks; e++)
-                if (elem_color[e] == c && elem_shape[e] == s) g.elems.push_back(e);
-            if (!g.elems.empty()) groups.push_back(std::move(g));
+            g.nreal = (int64_t)es.size();
+            const int ET = shape_et(s);
+            size_t i = 0;
+            while (i < es.size()) {
+                const int mid = emat[es[i]];
+                size_t j = i;
+                while (j < es.size() && emat[es[j]] == mid) j++;
+                for (size_t k = i; k < j; k += ET) {
+                    const int nv = (int)std::min<size_t>(ET, j - k);
+                    for (int q = 0; q < ET; q++) g.elems.push_back(q < nv ? es[k + q] : -1);
+                    g.tmeta.push_back((mid << 8) | nv);
+                }
+                i = j;
+            }
+            g.tile0 = ntile_all;
+            ntile_all += (int64_t)g.tmeta.size();
+            groups.push_back(std::move(g));
         }
     int64_t npad = 0, nb = 0, nd = 0;
     int64_t nf[FMT_COUNT] = {0, 0, 0, 0};
     for (auto& g : groups) {
-        const int ET = shape_et(g.shape), TP = shape_tp(g.shape), CE = shape_ce(g.shape);
-        g.ntiles = ((int64_t)g.elems.size() + ET - 1) / ET;
+        const int TP = shape_tp(g.shape), CE = shape_ce(g.shape);
+        g.ntiles = (int64_t)g.tmeta.size();
         g.pt0 = npad;
         g.b_off = nb;
         g.dof_off = nd;
@@ -621,6 +671,10 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
         const int64_t ne = (int64_t)g.elems.size();
         for (int64_t k = 0; k < ne; k++) {
             const int64_t e = g.elems[k];
+            if (e < 0) {  // padding slot: zero B, DoF 0 (never read), no point
+                for (int j = 0; j < CE; j++) hDof[g.dof_off + k * CE + j] = hCol[g.dof_off + k * CE + j] = 0;
+                continue;
+            }
             std::memcpy(&hDof[g.dof_off + k * CE], &dof_indexes[col_first[e]], sizeof(int) * CE);
             for (int j = 0; j < CE; j++) hCol[g.dof_off + k * CE + j] = col_first[e] + j;
             const int64_t t = k / ET;
@@ -650,7 +704,8 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
         (rc = dev_alloc(ctx, &ctx->dEp[3], npad)) || (rc = dev_alloc(ctx, &ctx->dAlpha, npad)) ||
         (rc = dev_alloc(ctx, &ctx->dMtype, npad)) || (rc = dev_alloc(ctx, &ctx->dBranch, npad)) ||
         (rc = dev_alloc(ctx, &ctx->dOrig, npad)) || (rc = dev_alloc(ctx, &ctx->dFlin, neq)) ||
-        (rc = dev_alloc(ctx, &ctx->dU, neq)) || (rc = dev_alloc(ctx, &ctx->dR, neq))) {
+        (rc = dev_alloc(ctx, &ctx->dU, neq)) || (rc = dev_alloc(ctx, &ctx->dR, neq)) ||
+        (rc = dev_alloc(ctx, &ctx->dTmeta, ntile_all))) {
         release_data(ctx);
         return rc;
     }
@@ -675,6 +730,11 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
         release_data(ctx);
         return rc;
     }
+    for (const auto& g : groups)
+        if ((rc = up(ctx->dTmeta + g.tile0, g.tmeta.data(), sizeof(int) * g.tmeta.size()))) {
+        release_data(ctx);
+        return rc;
+    }
     ctx->nblocks = nblocks;
     ctx->npts = npts;
     ctx->rows = rows;
@@ -683,6 +743,7 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
     ctx->ncolor = ncolor;
     ctx->nstorage = nblocks ? matrix_position[nblocks] : 0;
     ctx->npad = npad;
+    ctx->max_mat = max_mat;
     ctx->groups = std::move(groups);
     ctx->compact_ok = compact_ok;
     ctx->rfmt = ctx->tfmt = FMT_D64;
@@ -696,14 +757,47 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
     return IPF_OK;
 }
 
-int ipf_set_material(ipf_ctx* ctx, double lambda, double mu, double mc_phi, double mc_psi, double mc_cohesion) {
+int ipf_set_material_table(ipf_ctx* ctx, int nmat, const double* params) {
     if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
-    if (!(mu > 0) || !std::isfinite(lambda) || !std::isfinite(mc_phi) || !std::isfinite(mc_psi) ||
-        !std::isfinite(mc_cohesion))
-        return fail(ctx, IPF_EINVAL, "bad material: lambda=%g mu=%g phi=%g psi=%g c=%g", lambda, mu, mc_phi, mc_psi,
-                    mc_cohesion);
-    ctx->mat = MatConst::make(lambda, mu, mc_phi, mc_psi, mc_cohesion);
-    ctx->have_material = true;
+    if (nmat < 1 || nmat > (1 << 23) || !params) return fail(ctx, IPF_EINVAL, "ipf_set_material_table: nmat %d", nmat);
+    std::vector<MatConst> mats(nmat);
+    for (int i = 0; i < nmat; i++) {
+        const double* q = params + 5 * i;
+        const double lambda = q[0], mu = q[1], phi = q[2], psi = q[3], c = q[4];
+        if (!(mu > 0) || !std::isfinite(lambda) || !std::isfinite(phi) || !std::isfinite(psi) || !std::isfinite(c))
+            return fail(ctx, IPF_EINVAL, "bad material %d: lambda=%g mu=%g phi=%g psi=%g c=%g", i, lambda, mu, phi,
+                        psi, c);
+        mats[i] = MatConst::make(lambda, mu, phi, psi, c);
+    }
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->mats.size() != mats.size()) {
+        dev_free(ctx->dMats);
+        int rc = dev_alloc(ctx, &ctx->dMats, (int64_t)nmat);
+        if (rc) {
+            ctx->mats.clear();
+            return rc;
+        }
+    }
+    IPF_HIP(ctx, hipMemcpy(ctx->dMats, mats.data(), sizeof(MatConst) * nmat, hipMemcpyHostToDevice));
+    ctx->mats = std::move(mats);
+    return IPF_OK;
+}
+
+int ipf_set_material(ipf_ctx* ctx, double lambda, double mu, double mc_phi, double mc_psi, double mc_cohesion) {
+    const double q[5] = {lambda, mu, mc_phi, mc_psi, mc_cohesion};
+    return ipf_set_material_table(ctx, 1, q);
+}
+
+int ipf_set_element_materials(ipf_ctx* ctx, int64_t nblocks, const int* element_material) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    if (nblocks < 0 || (nblocks > 0 && !element_material))
+        return fail(ctx, IPF_EINVAL, "ipf_set_element_materials: bad arguments");
+    std::vector<int> v(element_material, element_material + nblocks);
+    for (int64_t e = 0; e < nblocks; e++)
+        if (v[e] < 0 || v[e] >= (1 << 23))
+            return fail(ctx, IPF_EINVAL, "element %lld: material id %d outside [0, 2^23)", (long long)e, v[e]);
+    ctx->elem_mat_next = std::move(v);
     return IPF_OK;
 }
 
@@ -764,7 +858,7 @@ int ipf_residual_timed(ipf_ctx* ctx, const double* u_dev, double* r_dev, double*
             launch_ms[i] = ms;
             if (launch_pts)
                 launch_pts[i] = i == 0 ? 0
-                                       : (int64_t)ctx->groups[i - 1].elems.size() * shape_nip(ctx->groups[i - 1].shape);
+                                       : ctx->groups[i - 1].nreal * shape_nip(ctx->groups[i - 1].shape);
         }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
@@ -813,8 +907,9 @@ int ipf_compute_sigma(ipf_ctx* ctx, const double* strain_dev, double* wsigma_dev
     if (rc) return rc;
     if (!strain_dev || !wsigma_dev) return fail(ctx, IPF_EINVAL, "ipf_compute_sigma: NULL pointer");
     IPF_HIP(ctx, hipSetDevice(ctx->device));
-    launch_sigma(false, ctx->update_mem, ctx->npad, strain_dev, wsigma_dev, nullptr, ctx->dOrig, point_state(ctx),
-                 ctx->mat, ctx->stream);
+    for (const auto& g : ctx->groups)
+        launch_sigma(false, ctx->update_mem, g.shape, group_dev(ctx, g), ctx->npad, strain_dev, wsigma_dev, nullptr,
+                     ctx->dOrig, point_state(ctx), ctx->stream);
     return launch_status(ctx, "compute_sigma");
 }
 
@@ -823,8 +918,9 @@ int ipf_compute_sigma_dep(ipf_ctx* ctx, const double* strain_dev, double* wsigma
     if (rc) return rc;
     if (!strain_dev || !wsigma_dev || !wdep_dev) return fail(ctx, IPF_EINVAL, "ipf_compute_sigma_dep: NULL pointer");
     IPF_HIP(ctx, hipSetDevice(ctx->device));
-    launch_sigma(true, ctx->update_mem, ctx->npad, strain_dev, wsigma_dev, wdep_dev, ctx->dOrig, point_state(ctx),
-                 ctx->mat, ctx->stream);
+    for (const auto& g : ctx->groups)
+        launch_sigma(true, ctx->update_mem, g.shape, group_dev(ctx, g), ctx->npad, strain_dev, wsigma_dev, wdep_dev,
+                     ctx->dOrig, point_state(ctx), ctx->stream);
     return launch_status(ctx, "compute_sigma_dep");
 }
 
@@ -875,8 +971,9 @@ int ipf_residual_tangent(ipf_ctx* ctx, const double* u_dev, double* r_dev) {
     if ((rc = ensure_solver_buffers(ctx))) return rc;
     for (const auto& g : ctx->groups)
         launch_multiply(g.shape, group_dev(ctx, g), u_dev, ctx->dStrain, ctx->dOrig, ctx->stream);
-    launch_sigma(true, ctx->update_mem, ctx->npad, ctx->dStrain, ctx->dSigw, nullptr, ctx->dOrig, point_state(ctx),
-                 ctx->mat, ctx->stream, ctx->dDep);
+    for (const auto& g : ctx->groups)
+        launch_sigma(true, ctx->update_mem, g.shape, group_dev(ctx, g), ctx->npad, ctx->dStrain, ctx->dSigw, nullptr,
+                     ctx->dOrig, point_state(ctx), ctx->stream, ctx->dDep);
     launch_copy(ctx->neq, ctx->dFlin, r_dev, ctx->stream);
     for (const auto& g : ctx->groups)
         launch_transpose(g.shape, group_dev(ctx, g), ctx->dSigw, r_dev, ctx->dOrig, ctx->stream);
@@ -900,6 +997,7 @@ int ipf_set_tangent_pattern(ipf_ctx* ctx, const int* el_color_index, const int64
         g.l_off = total;
         total += (int64_t)g.elems.size() * (ce * (ce + 1) / 2);
         for (size_t k = 0; k < g.elems.size(); k++) {
+            if (g.elems[k] < 0) continue;
             grp[g.elems[k]] = (int)gi;
             pos[g.elems[k]] = (int64_t)k;
         }

@@ -20,13 +20,25 @@
 
 namespace ipf {
 
+// One launch unit: the tiles of one color and one block shape.  Tiles never
+// mix materials (elements are sorted by material inside the group and the
+// last tile of every material run is padded), so a wavefront reads one
+// material; tmeta[t] = (material << 8) | (valid elements of tile t), the
+// valid elements being the first ones of the tile.
 struct GroupDev {
-    const double* B;      // tile-major, point-interleaved B values of the group
-    const int* dof;       // [nelem * CE] DoF indexes, element order of the group
-    int64_t nelem;        // elements in the group
-    int64_t ntiles;       // ceil(nelem / ET)
-    int64_t pt0;          // new-point index of the group's first slot
+    const double* B;       // tile-major, point-interleaved B values of the group
+    const int* dof;        // [nelem * CE] DoF indexes by element slot (padding slots unused)
+    const int* tmeta;      // [ntiles] (material << 8) | valid elements
+    const MatConst* mats;  // material table
+    int64_t nelem;         // element slots of the group (ntiles * ET)
+    int64_t ntiles;
+    int64_t pt0;           // new-point index of the group's first slot
 };
+
+__device__ __forceinline__ int tile_nvalid(const GroupDev& g, int64_t tile) { return g.tmeta[tile] & 255; }
+__device__ __forceinline__ const MatConst& tile_material(const GroupDev& g, int64_t tile) {
+    return g.mats[g.tmeta[tile] >> 8];
+}
 
 struct PointState {
     const double* w;
@@ -63,13 +75,15 @@ constexpr int format_values(int s, int f) {
 
 // Kernel launchers (ipf_kernels.hip).  All enqueue on `stream`.
 void launch_residual(int shape, bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
-                     const MatConst& m, hipStream_t stream);
+                     hipStream_t stream);
 void launch_multiply(int shape, const GroupDev& g, const double* u, double* strain, const int64_t* orig_of_new,
                      hipStream_t stream);
 void launch_transpose(int shape, const GroupDev& g, const double* wsigma, double* r, const int64_t* orig_of_new,
                       hipStream_t stream);
-void launch_sigma(bool dep, bool commit, int64_t npad, const double* strain, double* wsigma, double* wdep,
-                  const int64_t* orig_of_new, const PointState& st, const MatConst& m, hipStream_t stream,
+// Return map of one group's points (per-point arrays in the reference's AoS
+// order through orig_of_new; padding slots have orig_of_new < 0).
+void launch_sigma(bool dep, bool commit, int shape, const GroupDev& g, int64_t npad, const double* strain,
+                  double* wsigma, double* wdep, const int64_t* orig_of_new, const PointState& st, hipStream_t stream,
                   double* dep_soa = nullptr);
 void launch_copy(int64_t n, const double* src, double* dst, hipStream_t stream);
 void launch_fill(int64_t n, double* dst, double value, hipStream_t stream);
@@ -91,7 +105,7 @@ void launch_assemble_K(int shape, const GroupDev& g, const double* dep_soa, int6
 // Format variants (ipf_kernels_fmt.hip): Bv points at the group's first tile
 // in format fmt (FMT_C64, FMT_D32 or FMT_C32).
 void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
-                         double* r, const PointState& st, const MatConst& m, hipStream_t stream);
+                         double* r, const PointState& st, hipStream_t stream);
 void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, const double* v, double* y,
                         const double* dep_soa, int64_t npad, hipStream_t stream);
 // dense FP64 tiles of one group -> format fmt

@@ -89,11 +89,12 @@ template <int SHAPE>
 __device__ __forceinline__ void reduce_scatter(const double* L, const GroupDev& g, int64_t tile, int lane,
                                                double* __restrict__ r) {
     using C = Cfg<SHAPE>;
+    const int nv = tile_nvalid(g, tile);
     for (int task = lane; task < C::ET * C::CE; task += kWave) {
         const int e2 = task / C::CE;
         const int j = task - e2 * C::CE;
         const int64_t elem = tile * C::ET + e2;
-        if (elem < g.nelem) {
+        if (e2 < nv) {
             double f = L[(e2 * C::NIP) * C::CE + j];
 #pragma unroll
             for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e2 * C::NIP + ip) * C::CE + j];
@@ -105,7 +106,7 @@ __device__ __forceinline__ void reduce_scatter(const double* L, const GroupDev& 
 
 template <int SHAPE, bool COMMIT>
 __global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave, Cfg<SHAPE>::MIN_WAVES)
-k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, PointState st, MatConst m) {
+k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, PointState st) {
     using C = Cfg<SHAPE>;
     static_assert(C::ET * C::CE <= 2 * kWave, "two reduction tasks per lane");
     __shared__ double lds[C::WPB][C::TP * C::CE];
@@ -116,7 +117,7 @@ k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, Poi
     const int el = lane / C::NIP;
     const int64_t elem = tile * C::ET + el;
     const bool slot = tile_ok && lane < C::TP;
-    const bool valid = slot && elem < g.nelem;
+    const bool valid = slot && el < tile_nvalid(g, tile);
 
     // B stream first: the largest and independent loads
     double b[C::NB];
@@ -144,7 +145,7 @@ k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, Poi
         double ep[4] = {ld_stream(&st.ep0[P]), ld_stream(&st.ep1[P]), ld_stream(&st.ep2[P]), ld_stream(&st.ep3[P])};
         double alpha = ld_stream(&st.alpha[P]);
         int mt, br;
-        point_sigma<false>(e3, ep, alpha, mt, br, m, ld_stream(&st.w[P]), ws, nullptr);
+        point_sigma<false>(e3, ep, alpha, mt, br, tile_material(g, tile), ld_stream(&st.w[P]), ws, nullptr);
         if (COMMIT) {
             st.ep0[P] = ep[0];
             st.ep1[P] = ep[1];
@@ -171,7 +172,7 @@ k_multiply(GroupDev g, const double* __restrict__ u, double* __restrict__ strain
     const int64_t tile = (int64_t)blockIdx.x * C::WPB + wave;
     const int el = lane / C::NIP;
     const int64_t elem = tile * C::ET + el;
-    if (!(tile < g.ntiles && lane < C::TP && elem < g.nelem)) return;
+    if (!(tile < g.ntiles && lane < C::TP && el < tile_nvalid(g, tile))) return;
     double ue[C::CE];
     const int* d = g.dof + elem * C::CE;
 #pragma unroll
@@ -198,7 +199,7 @@ k_transpose(GroupDev g, const double* __restrict__ x, double* __restrict__ r, co
     const int el = lane / C::NIP;
     const int64_t elem = tile * C::ET + el;
     const bool slot = tile_ok && lane < C::TP;
-    const bool valid = slot && elem < g.nelem;
+    const bool valid = slot && el < tile_nvalid(g, tile);
     double ws[3] = {0.0, 0.0, 0.0};
     if (valid) {
         const int64_t o = orig[g.pt0 + tile * C::TP + lane];
@@ -216,14 +217,19 @@ k_transpose(GroupDev g, const double* __restrict__ x, double* __restrict__ r, co
     if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, r);
 }
 
+// One wavefront per tile of a group; lane q < tp owns point pt0 + tile*tp + q.
 template <bool DEP>
-__global__ void __launch_bounds__(256)
-k_sigma(int64_t npad, const double* __restrict__ strain, double* __restrict__ wsigma, double* __restrict__ wdep,
-        const int64_t* __restrict__ orig, PointState st, MatConst m, int commit, double* __restrict__ dep_soa) {
-    const int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= npad) return;
+__global__ void __launch_bounds__(kWave)
+k_sigma(GroupDev g, int tp, int64_t npad, const double* __restrict__ strain, double* __restrict__ wsigma,
+        double* __restrict__ wdep, const int64_t* __restrict__ orig, PointState st, int commit,
+        double* __restrict__ dep_soa) {
+    const int64_t tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (lane >= tp) return;
+    const int64_t P = g.pt0 + tile * tp + lane;
     const int64_t o = orig[P];
     if (o < 0) return;
+    const MatConst& m = tile_material(g, tile);
     const double e3[3] = {strain[3 * o], strain[3 * o + 1], strain[3 * o + 2]};
     double ep[4] = {st.ep0[P], st.ep1[P], st.ep2[P], st.ep3[P]};
     double alpha = st.alpha[P];
@@ -270,7 +276,7 @@ k_tangent(GroupDev g, const double* __restrict__ v, double* __restrict__ y, cons
     const int el = lane / C::NIP;
     const int64_t elem = tile * C::ET + el;
     const bool slot = tile_ok && lane < C::TP;
-    const bool valid = slot && elem < g.nelem;
+    const bool valid = slot && el < tile_nvalid(g, tile);
     double b[C::NB];
     if (slot) {
         load_point_B<SHAPE>(g.B, tile, lane, b);
@@ -325,7 +331,7 @@ k_tangent_diag(GroupDev g, double* __restrict__ diag, const double* __restrict__
     const int el = lane / C::NIP;
     const int64_t elem = tile * C::ET + el;
     const bool slot = tile_ok && lane < C::TP;
-    const bool valid = slot && elem < g.nelem;
+    const bool valid = slot && el < tile_nvalid(g, tile);
     double* L = lds[wave];
     if (slot) {
         double D[9];
@@ -370,23 +376,23 @@ dim3 tile_grid(const GroupDev& g) {
 
 template <int SHAPE>
 void residual_shape(bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
-                    const MatConst& m, hipStream_t s) {
+                    hipStream_t s) {
     if (g.ntiles == 0) return;
     const dim3 block(Cfg<SHAPE>::WPB * kWave);
     if (commit)
-        hipLaunchKernelGGL((k_residual<SHAPE, true>), tile_grid<SHAPE>(g), block, 0, s, g, u, r, st, m);
+        hipLaunchKernelGGL((k_residual<SHAPE, true>), tile_grid<SHAPE>(g), block, 0, s, g, u, r, st);
     else
-        hipLaunchKernelGGL((k_residual<SHAPE, false>), tile_grid<SHAPE>(g), block, 0, s, g, u, r, st, m);
+        hipLaunchKernelGGL((k_residual<SHAPE, false>), tile_grid<SHAPE>(g), block, 0, s, g, u, r, st);
 }
 
 }  // namespace
 
 void launch_residual(int shape, bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
-                     const MatConst& m, hipStream_t s) {
+                     hipStream_t s) {
     switch (shape) {
-        case SHAPE_Q1: residual_shape<SHAPE_Q1>(commit, g, u, r, st, m, s); break;
-        case SHAPE_Q2: residual_shape<SHAPE_Q2>(commit, g, u, r, st, m, s); break;
-        case SHAPE_Q3: residual_shape<SHAPE_Q3>(commit, g, u, r, st, m, s); break;
+        case SHAPE_Q1: residual_shape<SHAPE_Q1>(commit, g, u, r, st, s); break;
+        case SHAPE_Q2: residual_shape<SHAPE_Q2>(commit, g, u, r, st, s); break;
+        case SHAPE_Q3: residual_shape<SHAPE_Q3>(commit, g, u, r, st, s); break;
         default: break;
     }
 }
@@ -430,15 +436,17 @@ void launch_transpose(int shape, const GroupDev& g, const double* x, double* r, 
     }
 }
 
-void launch_sigma(bool dep, bool commit, int64_t npad, const double* strain, double* wsigma, double* wdep,
-                  const int64_t* orig, const PointState& st, const MatConst& m, hipStream_t s, double* dep_soa) {
-    if (npad == 0) return;
-    const dim3 grid((unsigned)((npad + 255) / 256));
+void launch_sigma(bool dep, bool commit, int shape, const GroupDev& g, int64_t npad, const double* strain,
+                  double* wsigma, double* wdep, const int64_t* orig, const PointState& st, hipStream_t s,
+                  double* dep_soa) {
+    if (g.ntiles == 0) return;
+    const int tp = shape_tp(shape);
+    const dim3 grid((unsigned)g.ntiles);
     if (dep)
-        hipLaunchKernelGGL(k_sigma<true>, grid, dim3(256), 0, s, npad, strain, wsigma, wdep, orig, st, m,
+        hipLaunchKernelGGL(k_sigma<true>, grid, dim3(kWave), 0, s, g, tp, npad, strain, wsigma, wdep, orig, st,
                            commit ? 1 : 0, dep_soa);
     else
-        hipLaunchKernelGGL(k_sigma<false>, grid, dim3(256), 0, s, npad, strain, wsigma, wdep, orig, st, m,
+        hipLaunchKernelGGL(k_sigma<false>, grid, dim3(kWave), 0, s, g, tp, npad, strain, wsigma, wdep, orig, st,
                            commit ? 1 : 0, dep_soa);
 }
 

@@ -150,11 +150,12 @@ __device__ __forceinline__ void reduce_cols(const typename FC<SHAPE, FMT>::T* L,
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
     constexpr int W = J1 - J0;
+    const int nv = tile_nvalid(g, tile);
     for (int task = lane; task < C::ET * W; task += kWave) {
         const int e2 = task / W;
         const int jj = task - e2 * W;
         const int64_t elem = tile * C::ET + e2;
-        if (elem < g.nelem) {
+        if (e2 < nv) {
             T f = L[(e2 * C::NIP) * W + jj];
 #pragma unroll
             for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e2 * C::NIP + ip) * W + jj];
@@ -167,7 +168,7 @@ __device__ __forceinline__ void reduce_cols(const typename FC<SHAPE, FMT>::T* L,
 template <int SHAPE, int FMT, bool COMMIT>
 __global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
 k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ u, double* __restrict__ r,
-               PointState st, MatConst m) {
+               PointState st) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
     __shared__ T lds[C::TP * (C::CE - C::CE / 2)];
@@ -176,7 +177,7 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     const int el = lane / C::NIP;
     const int64_t elem = tile * C::ET + el;
     const bool slot = lane < C::TP;
-    const bool valid = slot && elem < g.nelem;
+    const bool valid = slot && el < tile_nvalid(g, tile);
     T b[C::NV];
     if (slot) {
         load_values<SHAPE, FMT>(Bv, tile, lane, b);
@@ -201,7 +202,7 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
         double ep[4] = {ld_stream(&st.ep0[P]), ld_stream(&st.ep1[P]), ld_stream(&st.ep2[P]), ld_stream(&st.ep3[P])};
         double alpha = ld_stream(&st.alpha[P]);
         int mt, br;
-        point_sigma<false>(e3, ep, alpha, mt, br, m, ld_stream(&st.w[P]), ws, nullptr);
+        point_sigma<false>(e3, ep, alpha, mt, br, tile_material(g, tile), ld_stream(&st.w[P]), ws, nullptr);
         if (COMMIT) {
             st.ep0[P] = ep[0];
             st.ep1[P] = ep[1];
@@ -236,7 +237,7 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     const int el = lane / C::NIP;
     const int64_t elem = tile * C::ET + el;
     const bool slot = lane < C::TP;
-    const bool valid = slot && elem < g.nelem;
+    const bool valid = slot && el < tile_nvalid(g, tile);
     T b[C::NV];
     if (slot) {
         load_values<SHAPE, FMT>(Bv, tile, lane, b);
@@ -305,21 +306,21 @@ __global__ void k_convert_B(const double* __restrict__ dense, void* __restrict__
 
 template <int SHAPE, int FMT>
 void residual_fmt_shape(bool commit, const GroupDev& g, const void* Bv, const double* u, double* r,
-                        const PointState& st, const MatConst& m, hipStream_t s) {
+                        const PointState& st, hipStream_t s) {
     const dim3 grid((unsigned)g.ntiles), block(kWave);
     if (commit)
-        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, true>), grid, block, 0, s, g, Bv, u, r, st, m);
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, true>), grid, block, 0, s, g, Bv, u, r, st);
     else
-        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false>), grid, block, 0, s, g, Bv, u, r, st, m);
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false>), grid, block, 0, s, g, Bv, u, r, st);
 }
 
 template <int FMT>
 void residual_fmt(int shape, bool commit, const GroupDev& g, const void* Bv, const double* u, double* r,
-                  const PointState& st, const MatConst& m, hipStream_t s) {
+                  const PointState& st, hipStream_t s) {
     switch (shape) {
-        case SHAPE_Q1: residual_fmt_shape<SHAPE_Q1, FMT>(commit, g, Bv, u, r, st, m, s); break;
-        case SHAPE_Q2: residual_fmt_shape<SHAPE_Q2, FMT>(commit, g, Bv, u, r, st, m, s); break;
-        case SHAPE_Q3: residual_fmt_shape<SHAPE_Q3, FMT>(commit, g, Bv, u, r, st, m, s); break;
+        case SHAPE_Q1: residual_fmt_shape<SHAPE_Q1, FMT>(commit, g, Bv, u, r, st, s); break;
+        case SHAPE_Q2: residual_fmt_shape<SHAPE_Q2, FMT>(commit, g, Bv, u, r, st, s); break;
+        case SHAPE_Q3: residual_fmt_shape<SHAPE_Q3, FMT>(commit, g, Bv, u, r, st, s); break;
         default: break;
     }
 }
@@ -350,12 +351,12 @@ void convert_fmt(int shape, const double* dense, void* out, int64_t ntiles, hipS
 }  // namespace
 
 void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
-                         double* r, const PointState& st, const MatConst& m, hipStream_t s) {
+                         double* r, const PointState& st, hipStream_t s) {
     if (g.ntiles == 0) return;
     switch (fmt) {
-        case FMT_C64: residual_fmt<FMT_C64>(shape, commit, g, Bv, u, r, st, m, s); break;
-        case FMT_D32: residual_fmt<FMT_D32>(shape, commit, g, Bv, u, r, st, m, s); break;
-        case FMT_C32: residual_fmt<FMT_C32>(shape, commit, g, Bv, u, r, st, m, s); break;
+        case FMT_C64: residual_fmt<FMT_C64>(shape, commit, g, Bv, u, r, st, s); break;
+        case FMT_D32: residual_fmt<FMT_D32>(shape, commit, g, Bv, u, r, st, s); break;
+        case FMT_C32: residual_fmt<FMT_C32>(shape, commit, g, Bv, u, r, st, s); break;
         default: break;
     }
 }
