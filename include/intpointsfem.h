@@ -176,6 +176,47 @@ int ipf_set_b_format(ipf_ctx *ctx, int residual_format, int tangent_format);
  * when every block has the 2D elasticity pattern, IPF_B_DENSE_FP64 otherwise. */
 int ipf_get_b_format(const ipf_ctx *ctx, int *residual_format, int *tangent_format);
 
+/* ---- learned return-branch predictor (no reference counterpart) ---- */
+
+/* Training pairs of the exact return map at the trial state of u (the
+ * committed state is read, not changed), reference point order:
+ * feat_dev [8*npts] floats = (s1/c, s2/c, s3/c, ln c, phi, psi, 1e3*alpha, 0)
+ * with s1 >= s2 >= s3 the trial principal stresses
+ * (FunctionsSpectralDecomp.h:261-268), branch_dev [npts] the branch id of
+ * ProjectSigma (FunctionsSigmaProjection.h:337-379; IPF_BR_*), target_dev
+ * [npts] 1e3 * the alpha increment of that return. */
+int ipf_point_features(ipf_ctx *ctx, const double *u_dev, float *feat_dev, int *branch_dev, float *target_dev);
+
+/* A 7 -> 64 -> 64 -> 6 MLP (GELU; BF16 weights and activations on the
+ * matrix cores, FP32 accumulation): 5 branch logits + the alpha increment.
+ * Parameters are NPAR FP32 values (ipf_mlp_num_params; padded entries stay
+ * zero), Glorot-uniform from `seed`.  Samples are device arrays in the
+ * layout of ipf_point_features.  Loss: mean cross-entropy of the branch +
+ * 0.1 * mean squared error of the increment.  ipf_mlp_gradient returns the
+ * gradient of that loss over n samples (summed in a fixed order);
+ * ipf_mlp_train_step applies one Adam step (beta 0.9 / 0.999, eps 1e-8);
+ * ipf_mlp_predict writes argmax branches [n] and / or the 6 outputs [6n].
+ * Calls wait for their own stream; loss may be NULL (then train_step does
+ * not wait). */
+/* Branch sorting: order_dev [npts] = the reference point indexes grouped by
+ * branch_dev (keys 0..4, stable inside a key); ipf_compute_sigma_ordered is
+ * ipf_compute_sigma with the points visited in that order (each wavefront
+ * then sees points of one predicted branch) -- the same results. */
+int ipf_branch_order(ipf_ctx *ctx, const int *branch_dev, int *order_dev);
+int ipf_compute_sigma_ordered(ipf_ctx *ctx, const double *strain_dev, double *wsigma_dev, const int *order_dev);
+
+typedef struct ipf_mlp ipf_mlp;
+int ipf_mlp_create(int device, uint64_t seed, ipf_mlp **out);
+int ipf_mlp_destroy(ipf_mlp *mlp);
+int ipf_mlp_num_params(int *padded, int *trainable);
+int ipf_mlp_set_params(ipf_mlp *mlp, const float *params_host);
+int ipf_mlp_get_params(ipf_mlp *mlp, float *params_host);
+int ipf_mlp_gradient(ipf_mlp *mlp, const float *feat_dev, const int *branch_dev, const float *target_dev, int64_t n,
+                     float *grad_host, double *loss);
+int ipf_mlp_train_step(ipf_mlp *mlp, const float *feat_dev, const int *branch_dev, const float *target_dev,
+                       int64_t n, float lr, double *loss);
+int ipf_mlp_predict(ipf_mlp *mlp, const float *feat_dev, int64_t n, int *branch_dev, float *out_dev);
+
 /* ---- tangent, linear solve and Newton (the caller side of the path) ---- */
 
 /* Symmetry / Dirichlet DoFs (1 = fixed) used by the solver entry points,

@@ -21,7 +21,7 @@ import os
 
 import numpy as np
 
-__all__ = ["lib", "library_path", "CompMesh", "StructMatrix", "DeviceContext", "IPFError", "build"]
+__all__ = ["lib", "library_path", "CompMesh", "StructMatrix", "DeviceContext", "MLP", "IPFError", "build"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libintpointsfem.so")
@@ -99,6 +99,17 @@ def lib():
         "ipf_cg_solve": (_int, [_vp, _vp, _vp, _d, _int, _ip, _dp]),
         "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
         "ipf_get_state": (_int, [_vp, _dp, _dp, _ip, _ip]),
+        "ipf_point_features": (_int, [_vp, _vp, _vp, _vp, _vp]),
+        "ipf_branch_order": (_int, [_vp, _vp, _vp]),
+        "ipf_compute_sigma_ordered": (_int, [_vp, _vp, _vp, _vp]),
+        "ipf_mlp_create": (_int, [_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+        "ipf_mlp_destroy": (_int, [_vp]),
+        "ipf_mlp_num_params": (_int, [_ip, _ip]),
+        "ipf_mlp_set_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+        "ipf_mlp_get_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+        "ipf_mlp_gradient": (_int, [_vp, _vp, _vp, _vp, _i64, ctypes.POINTER(ctypes.c_float), _dp]),
+        "ipf_mlp_train_step": (_int, [_vp, _vp, _vp, _vp, _i64, ctypes.c_float, _dp]),
+        "ipf_mlp_predict": (_int, [_vp, _vp, _i64, _vp, _vp]),
         "ipf_set_state": (_int, [_vp, _dp, _dp]),
         # front-end
         "ipf_cmesh_quarter_annulus": (_int, [_int, _int, _int, _d, _d, _d, ctypes.POINTER(_vp)]),
@@ -386,6 +397,19 @@ class DeviceContext:
         keys = ("iterations", "converged", "rel_residual", "r0", "r", "cg_iterations", "cg_worst_relres")
         return dict(zip(keys, (float(v) for v in st[:7])))
 
+    def point_features(self, u, feat, branch, target):
+        """Training pairs of the exact return map at the trial state of u
+        (device tensors: feat float32 [npts, 8], branch int32 [npts], target float32 [npts])."""
+        self._c(lib().ipf_point_features(self._h, _ptr(u), _ptr(feat), _ptr(branch), _ptr(target)),
+                "ipf_point_features")
+
+    def branch_order(self, branch, order):
+        self._c(lib().ipf_branch_order(self._h, _ptr(branch), _ptr(order)), "ipf_branch_order")
+
+    def compute_sigma_ordered(self, strain, wsigma, order):
+        self._c(lib().ipf_compute_sigma_ordered(self._h, _ptr(strain), _ptr(wsigma), _ptr(order)),
+                "ipf_compute_sigma_ordered")
+
     def get_state(self):
         n = self.info()["npts"]
         ep = np.zeros(6 * n)
@@ -399,6 +423,57 @@ class DeviceContext:
         ep = np.ascontiguousarray(plastic_strain6, dtype=np.float64).reshape(-1)
         al = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
         self._c(lib().ipf_set_state(self._h, _p(ep, _dp), _p(al, _dp)), "ipf_set_state")
+
+
+class MLP:
+    """The learned return-branch predictor (``ipf_mlp``): 7 -> 64 -> 64 -> 6,
+    GELU, BF16 matrix-core operands, FP32 accumulation, Adam."""
+
+    def __init__(self, device=0, seed=0):
+        h = _vp()
+        _check(lib().ipf_mlp_create(device, seed, ctypes.byref(h)), what="ipf_mlp_create")
+        self._h = h
+        a = ctypes.c_int()
+        b = ctypes.c_int()
+        lib().ipf_mlp_num_params(ctypes.byref(a), ctypes.byref(b))
+        self.npar, self.ntrainable = a.value, b.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.ipf_mlp_destroy(h)
+            self._h = None
+
+    def params(self):
+        p = np.zeros(self.npar, dtype=np.float32)
+        _check(lib().ipf_mlp_get_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               what="ipf_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().ipf_mlp_set_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               what="ipf_mlp_set_params")
+
+    def gradient(self, feat, branch, target, n=None):
+        n = len(branch) if n is None else n
+        g = np.zeros(self.npar, dtype=np.float32)
+        loss = np.zeros(1)
+        _check(lib().ipf_mlp_gradient(self._h, _ptr(feat), _ptr(branch), _ptr(target), int(n),
+                                      g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _p(loss, _dp)),
+               what="ipf_mlp_gradient")
+        return float(loss[0]), g
+
+    def train_step(self, feat, branch, target, lr, n=None, want_loss=True):
+        n = len(branch) if n is None else n
+        loss = np.zeros(1)
+        _check(lib().ipf_mlp_train_step(self._h, _ptr(feat), _ptr(branch), _ptr(target), int(n), float(lr),
+                                        _p(loss, _dp) if want_loss else None), what="ipf_mlp_train_step")
+        return float(loss[0]) if want_loss else None
+
+    def predict(self, feat, branch=None, out=None, n=None):
+        n = (feat.shape[0] if hasattr(feat, "shape") else 0) if n is None else n
+        _check(lib().ipf_mlp_predict(self._h, _ptr(feat), int(n), _ptr(branch), _ptr(out)), what="ipf_mlp_predict")
 
 
 class StructMatrix:

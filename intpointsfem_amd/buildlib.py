@@ -18,6 +18,8 @@ SOURCES = [
     "ipf_solver.hip",
     "ipf_kernels_fmt.hip",
     "ipf_assemble.hip",
+    "ipf_learned.hip",
+    "ipf_mlp.hip",
     "ipf_device.hip",
     "TPZCompMesh.cpp",
     "TPZIntPoints.cpp",

@@ -18,6 +18,7 @@
 
 #include "../../include/intpointsfem.h"
 #include "ipf_kernels.h"
+#include "ipf_learned.h"
 
 namespace ipf {
 
@@ -174,6 +175,9 @@ struct ipf_ctx {
     std::vector<int> elem_mat_next;
     int max_mat = 0;              // largest material id of the transferred data
     int* dTmeta = nullptr;
+    int64_t* dNewOf = nullptr;    // reference point -> new point index
+    int* dPmat = nullptr;         // material id per new point
+    int* dHist = nullptr;         // counting-sort histograms
 
     std::vector<Group> groups;
 
@@ -286,6 +290,9 @@ void release_data(ipf_ctx* ctx) {
     for (int f = 1; f < FMT_COUNT; f++) dev_free(ctx->dBf[f]);
     dev_free(ctx->dLidx);
     dev_free(ctx->dTmeta);
+    dev_free(ctx->dNewOf);
+    dev_free(ctx->dPmat);
+    dev_free(ctx->dHist);
     ctx->max_mat = 0;
     ctx->nnz_k = 0;
     ctx->have_pattern = false;
@@ -705,7 +712,8 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
         (rc = dev_alloc(ctx, &ctx->dMtype, npad)) || (rc = dev_alloc(ctx, &ctx->dBranch, npad)) ||
         (rc = dev_alloc(ctx, &ctx->dOrig, npad)) || (rc = dev_alloc(ctx, &ctx->dFlin, neq)) ||
         (rc = dev_alloc(ctx, &ctx->dU, neq)) || (rc = dev_alloc(ctx, &ctx->dR, neq)) ||
-        (rc = dev_alloc(ctx, &ctx->dTmeta, ntile_all))) {
+        (rc = dev_alloc(ctx, &ctx->dTmeta, ntile_all)) || (rc = dev_alloc(ctx, &ctx->dNewOf, npts)) ||
+        (rc = dev_alloc(ctx, &ctx->dPmat, npad))) {
         release_data(ctx);
         return rc;
     }
@@ -729,6 +737,22 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
         (rc = zero(ctx->dBranch, npad)) || (rc = zero(ctx->dFlin, sizeof(double) * neq))) {
         release_data(ctx);
         return rc;
+    }
+    {
+        std::vector<int64_t> new_of(npts, -1);
+        std::vector<int> pmat(npad, 0);
+        for (int64_t P = 0; P < npad; P++)
+            if (hOrig[P] >= 0) new_of[hOrig[P]] = P;
+        for (const auto& g : groups) {
+            const int TP = shape_tp(g.shape);
+            for (int64_t t = 0; t < g.ntiles; t++)
+                for (int q = 0; q < TP; q++) pmat[g.pt0 + t * TP + q] = g.tmeta[t] >> 8;
+        }
+        if ((rc = up(ctx->dNewOf, new_of.data(), sizeof(int64_t) * npts)) ||
+            (rc = up(ctx->dPmat, pmat.data(), sizeof(int) * npad))) {
+            release_data(ctx);
+            return rc;
+        }
     }
     for (const auto& g : groups)
         if ((rc = up(ctx->dTmeta + g.tile0, g.tmeta.data(), sizeof(int) * g.tmeta.size()))) {
@@ -911,6 +935,42 @@ int ipf_compute_sigma(ipf_ctx* ctx, const double* strain_dev, double* wsigma_dev
         launch_sigma(false, ctx->update_mem, g.shape, group_dev(ctx, g), ctx->npad, strain_dev, wsigma_dev, nullptr,
                      ctx->dOrig, point_state(ctx), ctx->stream);
     return launch_status(ctx, "compute_sigma");
+}
+
+int ipf_point_features(ipf_ctx* ctx, const double* u_dev, float* feat_dev, int* branch_dev, float* target_dev) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!u_dev || !feat_dev || !branch_dev || !target_dev)
+        return fail(ctx, IPF_EINVAL, "ipf_point_features: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_solver_buffers(ctx))) return rc;
+    for (const auto& g : ctx->groups)
+        launch_multiply(g.shape, group_dev(ctx, g), u_dev, ctx->dStrain, ctx->dOrig, ctx->stream);
+    for (const auto& g : ctx->groups)
+        launch_point_features(g.shape, group_dev(ctx, g), ctx->dStrain, ctx->dOrig, point_state(ctx), feat_dev,
+                              branch_dev, target_dev, ctx->stream);
+    return launch_status(ctx, "point_features");
+}
+
+int ipf_branch_order(ipf_ctx* ctx, const int* branch_dev, int* order_dev) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!branch_dev || !order_dev) return fail(ctx, IPF_EINVAL, "ipf_branch_order: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->dHist && (rc = dev_alloc(ctx, &ctx->dHist, (int64_t)key_sort_blocks(ctx->npts) * IPF_NBRANCH)))
+        return rc;
+    launch_branch_order(branch_dev, ctx->npts, ctx->dHist, order_dev, ctx->stream);
+    return launch_status(ctx, "branch_order");
+}
+
+int ipf_compute_sigma_ordered(ipf_ctx* ctx, const double* strain_dev, double* wsigma_dev, const int* order_dev) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!strain_dev || !wsigma_dev || !order_dev) return fail(ctx, IPF_EINVAL, "ipf_compute_sigma_ordered: NULL");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    launch_sigma_order(ctx->npts, order_dev, ctx->dNewOf, ctx->dPmat, ctx->dMats, strain_dev, wsigma_dev,
+                       point_state(ctx), ctx->update_mem, ctx->stream);
+    return launch_status(ctx, "compute_sigma_ordered");
 }
 
 int ipf_compute_sigma_dep(ipf_ctx* ctx, const double* strain_dev, double* wsigma_dev, double* wdep_dev) {
