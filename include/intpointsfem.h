@@ -208,6 +208,9 @@ int ipf_compute_sigma_ordered(ipf_ctx *ctx, const double *strain_dev, double *ws
 typedef struct ipf_mlp ipf_mlp;
 int ipf_mlp_create(int device, uint64_t seed, ipf_mlp **out);
 int ipf_mlp_destroy(ipf_mlp *mlp);
+/* caller stream (hipStream_t as void*; NULL: the object's own stream, a
+ * blocking stream that orders itself with the legacy default stream) */
+int ipf_mlp_set_stream(ipf_mlp *mlp, void *hip_stream);
 int ipf_mlp_num_params(int *padded, int *trainable);
 int ipf_mlp_set_params(ipf_mlp *mlp, const float *params_host);
 int ipf_mlp_get_params(ipf_mlp *mlp, float *params_host);

@@ -104,6 +104,7 @@ def lib():
         "ipf_compute_sigma_ordered": (_int, [_vp, _vp, _vp, _vp]),
         "ipf_mlp_create": (_int, [_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
         "ipf_mlp_destroy": (_int, [_vp]),
+        "ipf_mlp_set_stream": (_int, [_vp, _vp]),
         "ipf_mlp_num_params": (_int, [_ip, _ip]),
         "ipf_mlp_set_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
         "ipf_mlp_get_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
@@ -443,6 +444,9 @@ class MLP:
         if h and _lib is not None:
             _lib.ipf_mlp_destroy(h)
             self._h = None
+
+    def set_stream(self, stream_ptr):
+        _check(lib().ipf_mlp_set_stream(self._h, stream_ptr), what="ipf_mlp_set_stream")
 
     def params(self):
         p = np.zeros(self.npar, dtype=np.float32)

@@ -18,7 +18,8 @@ extern "C" int ipf_internal_set_error(const char* msg);
 
 struct ipf_mlp {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;  // own_stream or the caller's
     float* par = nullptr;     // FP32 master parameters [NPAR]
     float* m1 = nullptr;      // Adam moments
     float* m2 = nullptr;
@@ -100,7 +101,7 @@ int ipf_mlp_create(int device, uint64_t seed, ipf_mlp** out) {
         return rc;
     };
     int rc;
-    if ((rc = hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate")) ||
+    if ((rc = hip_check(hipStreamCreate(&m->own_stream), "hipStreamCreate")) ||
         (rc = hip_check(hipMalloc((void**)&m->par, sizeof(float) * NPAR), "hipMalloc")) ||
         (rc = hip_check(hipMalloc((void**)&m->m1, sizeof(float) * NPAR), "hipMalloc")) ||
         (rc = hip_check(hipMalloc((void**)&m->m2, sizeof(float) * NPAR), "hipMalloc")) ||
@@ -109,6 +110,7 @@ int ipf_mlp_create(int device, uint64_t seed, ipf_mlp** out) {
         (rc = hip_check(hipMalloc((void**)&m->slots, sizeof(float) * NPAR * 4 * kMaxBlocks), "hipMalloc")) ||
         (rc = hip_check(hipMalloc((void**)&m->loss_slots, sizeof(double) * 4 * kMaxBlocks), "hipMalloc")))
         return bad(rc);
+    m->stream = m->own_stream;
     std::vector<float> p;
     init_params(p, seed);
     if ((rc = ipf_mlp_set_params(m, p.data()))) return bad(rc);
@@ -123,8 +125,14 @@ int ipf_mlp_destroy(ipf_mlp* m) {
     for (void* p : {(void*)m->par, (void*)m->m1, (void*)m->m2, (void*)m->wbf, (void*)m->grad, (void*)m->slots,
                     (void*)m->loss_slots})
         if (p) (void)hipFree(p);
-    if (m->stream) (void)hipStreamDestroy(m->stream);
+    if (m->own_stream) (void)hipStreamDestroy(m->own_stream);
     delete m;
+    return IPF_OK;
+}
+
+int ipf_mlp_set_stream(ipf_mlp* m, void* stream) {
+    if (!m) return mlp_fail(IPF_EINVAL, "ipf_mlp_set_stream: NULL mlp");
+    m->stream = stream ? (hipStream_t)stream : m->own_stream;
     return IPF_OK;
 }
 
