@@ -9,11 +9,14 @@ whose results are bit-identical to the dense blocks.
 
 Prints ONE JSON line on stdout (progress goes to stderr):
   value       Mint-pts/s = points * steps / timed seconds / 1e6
-  roofline    the fused per-color residual kernel against HBM: the plain FP64
-              algorithmic bytes (dense B once, 4-byte DoF index, weight + 4
-              plastic-strain components per point, u read + r written once)
-              over the kernel's average duration, from HIP events around each
-              launch; also the same rate on the bytes the format stores
+  roofline    the fused per-color residual kernel against HBM: its compulsory
+              bytes -- the stored B values once (compact FP64: dN/dx, dN/dy per
+              node and point), a 4-byte DoF index per element column, weight +
+              4 plastic-strain components per point, u read + r written once --
+              over the kernel's average duration from HIP events around each
+              launch; `traffic` = HBM bytes per launch measured by rocprofv3
+              (profiles/), `dense_fp64_bytes_rate` = the rate on the dense
+              blocks' byte count, bytes this kernel does not move
   cpu_baseline the CPU oracle (same algorithm, OpenMP) on the host cores
   b_format_variants  the same residual with the other B formats (dense FP64,
               dense FP32, compact FP32; ipf_set_b_format): time, rate on each
@@ -101,6 +104,68 @@ def launch_times(ctx, u, r, nrep):
     return acc / nrep, pts
 
 
+def event_ms(fn, stream, reps=10):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def measure_learned(ctx, u, r, npts, stream, dev, fused_ms):
+    """Trains the branch predictor on pairs from a batch of small instances,
+    then times, at the bench state, the reference's unfused steps with and
+    without branch sorting (ComputeSigma on points grouped by predicted
+    branch); the results of both are identical."""
+    import numpy as np
+    import torch
+
+    from intpointsfem_amd import learned as L
+    t0 = time.time()
+    f_tr, b_tr, t_tr = L.training_set(n_instances=24, seed=1)
+    mlp = L.new_mlp(seed=0)
+    hist = L.train(mlp, f_tr, b_tr, t_tr, steps=3000, batch=4096, lr=3e-3, log_every=1000)
+    train_s = time.time() - t0
+    del f_tr, b_tr, t_tr
+    feat, br_exact, tg = L.point_features(ctx, u, npts, dev)
+    strain = torch.zeros(3 * npts, dtype=torch.float64, device=dev)
+    ws = torch.zeros(3 * npts, dtype=torch.float64, device=dev)
+    pred = torch.zeros(npts, dtype=torch.int32, device=dev)
+    order = torch.zeros(npts, dtype=torch.int32, device=dev)
+    ctx.set_update_mem(False)
+    t = {
+        "multiply_ms": event_ms(lambda: ctx.multiply(u, strain), stream),
+        "features_ms": event_ms(lambda: ctx.point_features(u, feat, br_exact, tg), stream),
+        "predict_ms": event_ms(lambda: mlp.predict(feat, branch=pred, n=npts), stream),
+        "sort_ms": event_ms(lambda: ctx.branch_order(pred, order), stream),
+        "compute_sigma_ms": event_ms(lambda: ctx.compute_sigma(strain, ws), stream),
+        "compute_sigma_sorted_ms": event_ms(lambda: ctx.compute_sigma_ordered(strain, ws, order), stream),
+        "multiply_transpose_ms": event_ms(lambda: ctx.multiply_transpose(ws, r), stream),
+    }
+    acc = float((pred == br_exact).float().mean().item())
+    be = br_exact.cpu().numpy()
+    div_nat = L.divergence(be, 63)
+    div_sorted = L.divergence(be[order.cpu().numpy()], 64)
+    plain = t["multiply_ms"] + t["compute_sigma_ms"] + t["multiply_transpose_ms"]
+    learned = (t["multiply_ms"] + t["features_ms"] + t["predict_ms"] + t["sort_ms"] + t["compute_sigma_sorted_ms"]
+               + t["multiply_transpose_ms"])
+    log(f"[bench] learned: trained in {train_s:.1f}s (loss {hist[0][1]:.3f} -> {hist[-1][1]:.4f}), branch accuracy "
+        f"{acc:.4f}; mixed-branch groups {div_nat:.3f} -> {div_sorted:.4f}; ComputeSigma {t['compute_sigma_ms']:.3f} "
+        f"-> sorted {t['compute_sigma_sorted_ms']:.3f} ms; unfused residual plain {plain:.3f} ms, learned "
+        f"{learned:.3f} ms, fused {fused_ms:.3f} ms")
+    return dict({k: round(v, 4) for k, v in t.items()}, train_seconds=round(train_s, 1),
+                branch_accuracy=round(acc, 5), mixed_branch_groups_natural=round(div_nat, 4),
+                mixed_branch_groups_sorted=round(div_sorted, 5), unfused_residual_plain_ms=round(plain, 4),
+                unfused_residual_learned_ms=round(learned, 4), fused_residual_ms=round(fused_ms, 4),
+                learned_value=round(npts / learned / 1e3, 2))
+
+
 def run_batch(args):
     """The 64 uneven wellbore instances (tests/golden/batch64_instances.json,
     drawn with seed 0) in one context: one launch per (color, shape) for the
@@ -147,12 +212,12 @@ def run_batch(args):
     roof = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt0)
     log(f"[bench] launches {', '.join(f'{x:.4f}' for x in roof['color_ms'])} ms; {roof['achieved']:.0f} GB/s on "
         f"plain FP64 bytes, {roof['achieved_stored']:.0f} GB/s on stored bytes")
-    roofline = {"bound": "hbm", "achieved": round(roof["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(roof["achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
+    roofline = {"bound": "hbm", "achieved": round(roof["achieved_stored"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_residual_fmt, one launch per (color, element order) over all instances",
+                "algorithmic_bytes_per_point": round(roof["stored_bytes"] / npts, 1),
                 "avg_launch_ms": round(roof["avg_launch_ms"], 5),
-                "achieved_on_stored_bytes": round(roof["achieved_stored"], 1),
-                "frac_on_stored_bytes": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4)}
+                "dense_fp64_bytes_rate": round(roof["achieved"], 1)}
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -198,6 +263,7 @@ def main():
                     help="800x800-q2 (default), 800x800-q3, 40x40-q2, or batch64 (64 uneven instances)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-formats", action="store_true", help="skip the compact / FP32 B format variants")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned branch-predictor measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's timed work")
     ap.add_argument("--state", choices=("lame", "newton"), default="lame",
                     help="lame: thick-cylinder displacement of the last load step, zero plastic strain; "
@@ -304,14 +370,14 @@ def main():
             tk = json.load(f)["kernels"]
         if kname in tk:
             traffic = int(tk[kname]["total_bytes"])
-    roofline = {"bound": "hbm", "achieved": round(roof["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(roof["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+    roofline = {"bound": "hbm", "achieved": round(roof["achieved_stored"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": f"{kname} (one launch per element color)",
-                "bytes_per_launch": int(roof["bytes_per_launch"]),
+                "algorithmic_bytes_per_launch": int(roof["stored_bytes_per_launch"]),
+                "algorithmic_bytes_per_point": round(roof["stored_bytes"] / npts, 1),
                 "avg_launch_ms": round(roof["avg_launch_ms"], 5),
-                "stored_bytes_per_launch": int(roof["stored_bytes_per_launch"]),
-                "achieved_on_stored_bytes": round(roof["achieved_stored"], 1),
-                "frac_on_stored_bytes": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4)}
+                "measured_traffic_rate": None if traffic is None else round(traffic / (roof["avg_launch_ms"] / 1e3) / 1e9, 1),
+                "dense_fp64_bytes_rate": round(roof["achieved"], 1)}
 
     # the other B formats (same state, same u): timing, roofline on their own
     # stored bytes, and the deviation from the FP64 residual
@@ -342,6 +408,8 @@ def main():
                 f"{fr['achieved_stored']:.0f} GB/s on its stored bytes, "
                 f"|r - r_fp64| / max|f_int| = {err:.2e}")
         ctx.set_b_format(fmt0)
+
+    learned_info = None if args.no_learned else measure_learned(ctx, u, r, npts, stream, dev, ms_per_step)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -387,6 +455,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "b_format_variants": variants,
+        "learned": learned_info,
     }
     print(json.dumps(out), flush=True)
 

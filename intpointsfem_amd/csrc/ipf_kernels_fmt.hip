@@ -143,23 +143,22 @@ __device__ __forceinline__ void bt_cols(const typename FC<SHAPE, FMT>::T* b, con
 
 // Per element and column: the sum of its points' partials in point order,
 // added negated to r (TPZNumericalIntegrator::MultiplyTranspose's
-// y = -B^T x; the color's elements share no DoF, so no atomics).
+// y = -B^T x; the color's elements share no DoF, so no atomics).  dsx holds
+// the tile's DoF indexes (staged in LDS by the kernel), nv its valid elements.
 template <int SHAPE, int FMT, int J0, int J1>
-__device__ __forceinline__ void reduce_cols(const typename FC<SHAPE, FMT>::T* L, const GroupDev& g, int64_t tile,
-                                            int lane, double* __restrict__ r) {
+__device__ __forceinline__ void reduce_cols(const typename FC<SHAPE, FMT>::T* L, const int* dsx, int nv, int lane,
+                                              double* __restrict__ r) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
     constexpr int W = J1 - J0;
-    const int nv = tile_nvalid(g, tile);
     for (int task = lane; task < C::ET * W; task += kWave) {
         const int e2 = task / W;
         const int jj = task - e2 * W;
-        const int64_t elem = tile * C::ET + e2;
         if (e2 < nv) {
             T f = L[(e2 * C::NIP) * W + jj];
 #pragma unroll
             for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e2 * C::NIP + ip) * W + jj];
-            const int d = g.dof[elem * C::CE + J0 + jj];
+            const int d = dsx[e2 * C::CE + J0 + jj];
             r[d] = r[d] + (-double(f));
         }
     }
@@ -172,6 +171,8 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
     __shared__ T lds[C::TP * (C::CE - C::CE / 2)];
+    __shared__ double us[C::ET * C::CE];
+    __shared__ int dsx[C::ET * C::CE];
     const int lane = threadIdx.x;
     const int64_t tile = blockIdx.x;
     const int el = lane / C::NIP;
@@ -185,17 +186,20 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
 #pragma unroll
         for (int k = 0; k < C::NV; k++) b[k] = T(0);
     }
-    double ue[C::CE];
-    if (valid) {
-        const int* d = g.dof + elem * C::CE;
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) ue[j] = u[ld_stream(&d[j])];
-    } else {
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) ue[j] = 0.0;
+    // the tile's DoF indexes and u values, gathered once per element into LDS
+    // (the lanes of an element read them from there: no per-lane copy of the
+    // element's u in registers, no repeated gathers)
+    const int nv = tile_nvalid(g, tile);
+    for (int t = lane; t < C::ET * C::CE; t += kWave) {
+        if (t / C::CE < nv) {
+            const int d = ld_stream(&g.dof[tile * (int64_t)(C::ET * C::CE) + t]);
+            dsx[t] = d;
+            us[t] = u[d];
+        }
     }
-    double e3[3];
-    strain_fmt<SHAPE, FMT>(b, ue, e3);
+    __syncthreads();
+    double e3[3] = {0.0, 0.0, 0.0};
+    if (valid) strain_fmt<SHAPE, FMT>(b, us + el * C::CE, e3);
     double ws[3] = {0.0, 0.0, 0.0};
     if (valid) {
         const int64_t P = g.pt0 + tile * C::TP + lane;
@@ -216,11 +220,11 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     constexpr int H = C::CE / 2;
     if (slot) bt_cols<SHAPE, FMT, 0, H>(b, ws, lds + lane * H);
     __syncthreads();
-    reduce_cols<SHAPE, FMT, 0, H>(lds, g, tile, lane, r);
+    reduce_cols<SHAPE, FMT, 0, H>(lds, dsx, nv, lane, r);
     __syncthreads();
     if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, ws, lds + lane * (C::CE - H));
     __syncthreads();
-    reduce_cols<SHAPE, FMT, H, C::CE>(lds, g, tile, lane, r);
+    reduce_cols<SHAPE, FMT, H, C::CE>(lds, dsx, nv, lane, r);
 }
 
 // y += sum_e B_e^T (w Dep) B_e v_e with B in format FMT (FP32 formats: FP32
@@ -232,6 +236,8 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
     __shared__ T lds[C::TP * (C::CE - C::CE / 2)];
+    __shared__ double vs[C::ET * C::CE];
+    __shared__ int dsx[C::ET * C::CE];
     const int lane = threadIdx.x;
     const int64_t tile = blockIdx.x;
     const int el = lane / C::NIP;
@@ -245,17 +251,17 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
 #pragma unroll
         for (int k = 0; k < C::NV; k++) b[k] = T(0);
     }
-    double ve[C::CE];
-    if (valid) {
-        const int* d = g.dof + elem * C::CE;
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) ve[j] = v[ld_stream(&d[j])];
-    } else {
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) ve[j] = 0.0;
+    const int nv = tile_nvalid(g, tile);
+    for (int t = lane; t < C::ET * C::CE; t += kWave) {
+        if (t / C::CE < nv) {
+            const int d = ld_stream(&g.dof[tile * (int64_t)(C::ET * C::CE) + t]);
+            dsx[t] = d;
+            vs[t] = v[d];
+        }
     }
-    double e3[3];
-    strain_fmt<SHAPE, FMT>(b, ve, e3);
+    __syncthreads();
+    double e3[3] = {0.0, 0.0, 0.0};
+    if (valid) strain_fmt<SHAPE, FMT>(b, vs + el * C::CE, e3);
     double x[3] = {0.0, 0.0, 0.0};
     if (valid) {
         const int64_t P = g.pt0 + tile * C::TP + lane;
@@ -270,11 +276,11 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     constexpr int H = C::CE / 2;
     if (slot) bt_cols<SHAPE, FMT, 0, H>(b, x, lds + lane * H);
     __syncthreads();
-    reduce_cols<SHAPE, FMT, 0, H>(lds, g, tile, lane, y);
+    reduce_cols<SHAPE, FMT, 0, H>(lds, dsx, nv, lane, y);
     __syncthreads();
     if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, x, lds + lane * (C::CE - H));
     __syncthreads();
-    reduce_cols<SHAPE, FMT, H, C::CE>(lds, g, tile, lane, y);
+    reduce_cols<SHAPE, FMT, H, C::CE>(lds, dsx, nv, lane, y);
 }
 
 // Builds format FMT from the dense FP64 tile layout of one group (device side).

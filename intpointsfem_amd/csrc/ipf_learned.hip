@@ -451,24 +451,51 @@ __global__ void __launch_bounds__(kSortBlock) k_key_hist(const int* __restrict__
     if (threadIdx.x < NK) hist[(int64_t)blockIdx.x * NK + threadIdx.x] = h[threadIdx.x];
 }
 
-// exclusive offsets: key-major, then block order (one block of NK threads)
+// exclusive offsets: key-major, then block order.  One block of 1024
+// threads; thread t owns a contiguous chunk of the block histograms.
+constexpr int kScanThreads = 1024;
 template <int NK>
-__global__ void k_key_scan(int* __restrict__ hist, int nblocks) {
-    __shared__ long long tot[NK];
-    const int k = threadIdx.x;
-    if (k < NK) {
-        long long s = 0;
-        for (int b = 0; b < nblocks; b++) s += hist[(int64_t)b * NK + k];
-        tot[k] = s;
+__global__ void __launch_bounds__(kScanThreads) k_key_scan(int* __restrict__ hist, int nblocks) {
+    __shared__ long long part[kScanThreads][NK];
+    __shared__ long long base[NK];
+    const int t = threadIdx.x;
+    const int chunk = (nblocks + kScanThreads - 1) / kScanThreads;
+    const int b0 = min(nblocks, t * chunk), b1 = min(nblocks, b0 + chunk);
+    long long sum[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++) sum[k] = 0;
+    for (int b = b0; b < b1; b++)
+#pragma unroll
+        for (int k = 0; k < NK; k++) sum[k] += hist[(int64_t)b * NK + k];
+#pragma unroll
+    for (int k = 0; k < NK; k++) part[t][k] = sum[k];
+    __syncthreads();
+    if (t < NK) {  // per key: exclusive prefix over the chunks, then the key totals
+        long long run = 0;
+        for (int i = 0; i < kScanThreads; i++) {
+            const long long c = part[i][t];
+            part[i][t] = run;
+            run += c;
+        }
+        base[t] = run;
     }
     __syncthreads();
-    if (k < NK) {
-        long long base = 0;
-        for (int j = 0; j < k; j++) base += tot[j];
-        for (int b = 0; b < nblocks; b++) {
+    if (t == 0) {
+        long long acc = 0;
+        for (int k = 0; k < NK; k++) {
+            const long long c = base[k];
+            base[k] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+        long long run = base[k] + part[t][k];
+        for (int b = b0; b < b1; b++) {
             const int c = hist[(int64_t)b * NK + k];
-            hist[(int64_t)b * NK + k] = (int)base;
-            base += c;
+            hist[(int64_t)b * NK + k] = (int)run;
+            run += c;
         }
     }
 }
@@ -541,7 +568,7 @@ void launch_branch_order(const int* key, int64_t n, int* hist, int* order, hipSt
     if (n == 0) return;
     const int nb = key_sort_blocks(n);
     hipLaunchKernelGGL(k_key_hist<IPF_NBRANCH>, dim3(nb), dim3(kSortBlock), 0, s, key, n, hist);
-    hipLaunchKernelGGL(k_key_scan<IPF_NBRANCH>, dim3(1), dim3(64), 0, s, hist, nb);
+    hipLaunchKernelGGL(k_key_scan<IPF_NBRANCH>, dim3(1), dim3(kScanThreads), 0, s, hist, nb);
     hipLaunchKernelGGL(k_key_scatter<IPF_NBRANCH>, dim3(nb), dim3(kSortBlock), 0, s, key, n, hist, order);
 }
 
