@@ -19,6 +19,8 @@
 
 #include <cmath>
 
+#define IPF_HD __host__ __device__ __forceinline__
+
 namespace ipf {
 
 enum { XX = 0, XY = 1, XZ = 2, YY = 3, YZ = 4, ZZ = 5 };
@@ -49,7 +51,7 @@ struct MatConst {
 // ---------------------------------------------------------------- spectral
 
 // FunctionsSpectralDecomp.h:7-20
-__device__ __forceinline__ double normalize6(double* s) {
+IPF_HD double normalize6(double* s) {
     double maxel = s[0];
 #pragma unroll
     for (int i = 1; i < 6; i++)
@@ -62,7 +64,7 @@ __device__ __forceinline__ double normalize6(double* s) {
 }
 
 // FunctionsSpectralDecomp.h:25-56 (row 3 uses sxy + syz as the reference does)
-__device__ __forceinline__ void gershgorin(const double* s, double& upper_out, double& lower_out) {
+IPF_HD void gershgorin(const double* s, double& upper_out, double& lower_out) {
     const double row_1 = s[XY] + s[XZ];
     const double lo0 = s[XX] - fabs(row_1), up0 = s[XX] + fabs(row_1);
     const double row_2 = s[XY] + s[YZ];
@@ -79,7 +81,7 @@ __device__ __forceinline__ void gershgorin(const double* s, double& upper_out, d
 }
 
 // Cubic of FunctionsSpectralDecomp.h:75-77, evaluated left to right.
-__device__ __forceinline__ double cubic(double x, const double* s, double sxy2, double sxz2, double syz2) {
+IPF_HD double cubic(double x, const double* s, double sxy2, double sxz2, double syz2) {
     return -x * x * x - sxz2 * s[YY] - sxy2 * s[ZZ] - syz2 * s[XX] + 2 * s[XY] * s[XZ] * s[YZ] +
            s[XX] * s[YY] * s[ZZ] + x * x * (s[XX] + s[YY] + s[ZZ]) +
            x * (sxy2 + sxz2 + syz2 - s[XX] * s[YY] - s[XX] * s[ZZ] - s[YY] * s[ZZ]);
@@ -87,7 +89,7 @@ __device__ __forceinline__ double cubic(double x, const double* s, double sxy2, 
 
 // FunctionsSpectralDecomp.h:61-111: Newton from each Gershgorin bound
 // (20 iterations, |f| > 1e-11), third root from the trace, descending sort.
-__device__ __forceinline__ void eigenvalues_newton(double up, double lo, const double* s, double maxel, double* ev) {
+IPF_HD void eigenvalues_newton(double up, double lo, const double* s, double maxel, double* ev) {
     const double sxy2 = s[XY] * s[XY];
     const double sxz2 = s[XZ] * s[XZ];
     const double syz2 = s[YZ] * s[YZ];
@@ -118,7 +120,7 @@ __device__ __forceinline__ void eigenvalues_newton(double up, double lo, const d
 }
 
 // FunctionsSpectralDecomp.h:116-150
-__device__ __forceinline__ void eigvec_simple(const double* s, double lam, double* vec) {
+IPF_HD void eigvec_simple(const double* s, double lam, double* vec) {
     const double d0 = (s[XX] - lam) * (s[YY] - lam) - s[XY] * s[XY];
     const double d1 = (s[XX] - lam) * (s[ZZ] - lam) - s[XZ] * s[XZ];
     const double d2 = (s[YY] - lam) * (s[ZZ] - lam) - s[YZ] * s[YZ];
@@ -146,7 +148,7 @@ __device__ __forceinline__ void eigvec_simple(const double* s, double lam, doubl
 }
 
 // FunctionsSpectralDecomp.h:155-212 (second norm keeps the v2[1]*v1[1] term of :203)
-__device__ __forceinline__ void eigvec_double(const double* s, double lam, double* vec1, double* vec2) {
+IPF_HD void eigvec_double(const double* s, double lam, double* vec1, double* vec2) {
     const double x0 = s[XX] - lam, x1 = s[YY] - lam, x2 = s[ZZ] - lam;
     double maxx = fabs(x0);
     if (fabs(x1) > fabs(maxx)) maxx = fabs(x1);
@@ -170,7 +172,7 @@ __device__ __forceinline__ void eigvec_double(const double* s, double lam, doubl
 
 // FunctionsSpectralDecomp.h:217-256 followed by the normalise/scale-back of
 // :261-268.  s is normalised on entry and scaled back by maxel here.
-__device__ __forceinline__ void eigenvectors6(double* s, const double* ev, double* vec, double maxel) {
+IPF_HD void eigenvectors6(double* s, const double* ev, double* vec, double maxel) {
 #pragma unroll
     for (int i = 0; i < 6; i++) s[i] *= maxel;
     if ((ev[0] == ev[1]) && (ev[1] == ev[2])) {
@@ -199,7 +201,7 @@ __device__ __forceinline__ void eigenvectors6(double* s, const double* ev, doubl
 }
 
 // FunctionsSpectralDecomp.h:261-268 (s is modified in place like the reference)
-__device__ __forceinline__ void spectral(double* s, double* ev, double* vec) {
+IPF_HD void spectral(double* s, double* ev, double* vec) {
     const double maxel = normalize6(s);
     double up, lo;
     gershgorin(s, up, lo);
@@ -207,14 +209,158 @@ __device__ __forceinline__ void spectral(double* s, double* ev, double* vec) {
     eigenvectors6(s, ev, vec, maxel);
 }
 
+// ---------------------------------------------------------------- plane strain
+//
+// spectral_ps: spectral() for a tensor whose XZ and YZ components are +0 (the
+// trial stresses and elastic strains of plane strain: mu * 0.0, 0.0).  The
+// results equal spectral()'s bit for bit up to the sign of exact zeros; only
+// work on the zero components is left out:
+//   * normalize: +0 / maxel is a zero with the sign of maxel;
+//   * Gershgorin rows: |sxy + (+-0)| = |sxy|;
+//   * cubic and derivative: sxz^2 = syz^2 = +0 and 2 sxy sxz syz = +-0.
+//     Adding or subtracting a zero changes a partial sum only when that sum
+//     is itself zero, and then only its sign, so the cubic is the same
+//     nonzero value or zero in both forms (it enters only |f| and, when
+//     nonzero, f / f'); the derivative's partial sum -3x^2 + sxy^2 is never
+//     -0, so adding +0 to it is exact;
+//   * eigenvectors: after the scale-back sxz = syz = +0, so in the branch of
+//     the xy minor the vector is (zero, zero, 1) with norm exactly 1, and in
+//     the other two branches its z component is a zero that the division by
+//     the (finite, >= 1) norm leaves unchanged.
+// Non-finite or degenerate cases (a zero minor, NaN) take the general code.
+
+IPF_HD double cubic_ps(double x, const double* s, double sxy2) {
+    return -x * x * x - sxy2 * s[ZZ] + s[XX] * s[YY] * s[ZZ] + x * x * (s[XX] + s[YY] + s[ZZ]) +
+           x * (sxy2 - s[XX] * s[YY] - s[XX] * s[ZZ] - s[YY] * s[ZZ]);
+}
+
+IPF_HD void eigenvalues_newton_ps(double up, double lo, const double* s, double maxel, double* ev) {
+    const double sxy2 = s[XY] * s[XY];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        double x = (i == 0) ? up : lo;
+        double f = cubic_ps(x, s, sxy2);
+        double res = fabs(f);
+        int it = 0;
+        while (it < 20 && res > 10e-12) {
+            const double df = -3 * x * x + sxy2 - s[XX] * s[YY] - s[XX] * s[ZZ] - s[YY] * s[ZZ] +
+                              2 * x * (s[XX] + s[YY] + s[ZZ]);
+            x -= f / df;
+            f = cubic_ps(x, s, sxy2);
+            res = fabs(f);
+            it++;
+        }
+        ev[i] = x;
+    }
+    ev[2] = s[XX] + s[YY] + s[ZZ] - ev[0] - ev[1];
+    ev[0] *= maxel;
+    ev[1] *= maxel;
+    ev[2] *= maxel;
+    if (ev[0] < ev[1]) { const double a = ev[0]; ev[0] = ev[1]; ev[1] = a; }
+    if (ev[0] < ev[2]) { const double a = ev[0]; ev[0] = ev[2]; ev[2] = a; }
+    if (ev[1] < ev[2]) { const double a = ev[1]; ev[1] = ev[2]; ev[2] = a; }
+}
+
+IPF_HD void eigvec_simple_ps(const double* s, double lam, double* vec) {
+    const double d0 = (s[XX] - lam) * (s[YY] - lam) - s[XY] * s[XY];
+    const double d1 = (s[XX] - lam) * (s[ZZ] - lam) - s[XZ] * s[XZ];
+    const double d2 = (s[YY] - lam) * (s[ZZ] - lam) - s[YZ] * s[YZ];
+    double maxdet = fabs(d0);
+    if (fabs(d1) > fabs(maxdet)) maxdet = fabs(d1);
+    if (fabs(d2) > fabs(maxdet)) maxdet = fabs(d2);
+    double v0, v1, v2;
+    if (maxdet == fabs(d0)) {
+        const double t0 = -(s[YY] - lam) * s[XZ] + s[XY] * s[YZ];
+        const double t1 = -(s[XX] - lam) * s[YZ] + s[XY] * s[XZ];
+        if ((d0 > 0.0 || d0 < 0.0) && t0 == 0.0 && t1 == 0.0) {
+            // (1/d0) t: a zero with sign(d0) * sign(t); norm sqrt(0 + 0 + 1) = 1
+            vec[0] = d0 > 0.0 ? t0 : -t0;
+            vec[1] = d0 > 0.0 ? t1 : -t1;
+            vec[2] = 1.0;
+            return;
+        }
+        v0 = 1 / d0 * t0;
+        v1 = 1 / d0 * t1;
+        v2 = 1;
+    } else if (maxdet == fabs(d1)) {
+        v0 = 1 / d1 * (-(s[ZZ] - lam) * s[XY] + s[XZ] * s[YZ]);
+        v1 = 1;
+        v2 = 1 / d1 * (-(s[XX] - lam) * s[YZ] + s[XY] * s[XZ]);
+    } else {
+        v0 = 1;
+        v1 = 1 / d2 * (-(s[ZZ] - lam) * s[XY] + s[XZ] * s[YZ]);
+        v2 = 1 / d2 * (-(s[YY] - lam) * s[XZ] + s[XY] * s[YZ]);
+    }
+    const double norm = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    vec[0] = v0 / norm;
+    vec[1] = v1 / norm;
+    // v2 / norm for a zero v2 and a finite norm (>= 1 here) is v2 itself
+    vec[2] = (v2 == 0.0 && norm >= 1.0 && norm <= 1.7976931348623157e308) ? v2 : v2 / norm;
+}
+
+IPF_HD void eigenvectors6_ps(double* s, const double* ev, double* vec, double maxel) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) s[i] *= maxel;
+    if ((ev[0] == ev[1]) && (ev[1] == ev[2])) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) vec[i] = 0.;
+        vec[0] = 1.;
+        vec[4] = 1.;
+        vec[8] = 1.;
+        return;
+    }
+    if (ev[0] != ev[1] && ev[0] != ev[2]) {
+        eigvec_simple_ps(s, ev[0], &vec[0]);
+    } else if (ev[0] == ev[1]) {
+        eigvec_double(s, ev[0], &vec[0], &vec[3]);
+    } else if (ev[0] == ev[2]) {
+        eigvec_double(s, ev[0], &vec[0], &vec[6]);
+    }
+    if (ev[1] != ev[0] && ev[1] != ev[2]) {
+        eigvec_simple_ps(s, ev[1], &vec[3]);
+    } else if (ev[1] == ev[2]) {
+        eigvec_double(s, ev[1], &vec[3], &vec[6]);
+    }
+    if (ev[2] != ev[0] && ev[2] != ev[1]) {
+        eigvec_simple_ps(s, ev[2], &vec[6]);
+    }
+}
+
+// s[XZ] and s[YZ] must be +0 on entry.
+IPF_HD void spectral_ps(double* s, double* ev, double* vec) {
+    // normalize6: the zero components never win the max-abs search
+    double maxel = s[XX];
+    if (fabs(s[XY]) > fabs(maxel)) maxel = s[XY];
+    if (fabs(s[YY]) > fabs(maxel)) maxel = s[YY];
+    if (fabs(s[ZZ]) > fabs(maxel)) maxel = s[ZZ];
+    if (maxel != 0) {
+        s[XX] /= maxel;
+        s[XY] /= maxel;
+        s[YY] /= maxel;
+        s[ZZ] /= maxel;
+        const double z = maxel > 0 ? 0.0 : (maxel < 0 ? -0.0 : s[XZ] / maxel);
+        s[XZ] = z;
+        s[YZ] = z;
+    }
+    // Gershgorin (FunctionsSpectralDecomp.h:25-56) with |sxy + sxz| = |sxy + syz| = |sxy|
+    const double a = fabs(s[XY]);
+    double up = s[XX] + a, lo = s[XX] - a;
+    if (s[YY] + a > up) up = s[YY] + a;
+    if (s[YY] - a < lo) lo = s[YY] - a;
+    if (s[ZZ] + a > up) up = s[ZZ] + a;
+    if (s[ZZ] - a < lo) lo = s[ZZ] - a;
+    eigenvalues_newton_ps(up, lo, s, maxel, ev);
+    eigenvectors6_ps(s, ev, vec, maxel);
+}
+
 // ---------------------------------------------------------------- projection
 
-__device__ __forceinline__ bool ordered(const double* e) {
+IPF_HD bool ordered(const double* e) {
     return (e[0] > e[1] || fabs(e[0] - e[1]) < 1.e-12) && (e[1] > e[2] || fabs(e[1] - e[2]) < 1.e-12);
 }
 
 // FunctionsSigmaProjection.h:25-63 (H = 0)
-__device__ __forceinline__ bool main_plane(const double* tr, double* sp, double& hard, const MatConst& m) {
+IPF_HD bool main_plane(const double* tr, double* sp, double& hard, const MatConst& m) {
     double e0 = tr[0], e1 = tr[1], e2 = tr[2];
     double phi = e0 - e2 + (e0 + e2) * m.sinphi - m.phi_c;
     double gamma = 0;
@@ -237,7 +383,7 @@ __device__ __forceinline__ bool main_plane(const double* tr, double* sp, double&
 
 // 2x2 Newton of the edge returns, FunctionsSigmaProjection.h:119-146 / :223-250.
 // gamma starts at {0, 0} (the reference leaves it uninitialised, :103, :205).
-__device__ __forceinline__ void edge_newton(double sb0, double sb1, double ab0, double ab1, double ji00, double ji01,
+IPF_HD void edge_newton(double sb0, double sb1, double ab0, double ab1, double ji00, double ji01,
                                             double ji10, double ji11, double edge_c, double& g0, double& g1) {
     double phi0 = sb0 - edge_c;
     double phi1 = sb1 - edge_c;
@@ -254,7 +400,7 @@ __device__ __forceinline__ void edge_newton(double sb0, double sb1, double ab0, 
 }
 
 // FunctionsSigmaProjection.h:93-158
-__device__ __forceinline__ bool right_edge(const double* tr, double* sp, double& hard, const MatConst& m) {
+IPF_HD bool right_edge(const double* tr, double* sp, double& hard, const MatConst& m) {
     double e0 = tr[0], e1 = tr[1], e2 = tr[2];
     const double sb0 = e0 - e2 + (e0 + e2) * m.sinphi;
     const double sb1 = e0 - e1 + (e0 + e1) * m.sinphi;
@@ -271,7 +417,7 @@ __device__ __forceinline__ bool right_edge(const double* tr, double* sp, double&
 }
 
 // FunctionsSigmaProjection.h:193-257
-__device__ __forceinline__ bool left_edge(const double* tr, double* sp, double& hard, const MatConst& m) {
+IPF_HD bool left_edge(const double* tr, double* sp, double& hard, const MatConst& m) {
     double e0 = tr[0], e1 = tr[1], e2 = tr[2];
     const double sb0 = e0 - e2 + (e0 + e2) * m.sinphi;
     const double sb1 = e1 - e2 + (e1 + e2) * m.sinphi;
@@ -288,7 +434,7 @@ __device__ __forceinline__ bool left_edge(const double* tr, double* sp, double& 
 }
 
 // FunctionsSigmaProjection.h:292-323
-__device__ __forceinline__ void apex(const double* tr, double* sp, double& hard, const MatConst& m) {
+IPF_HD void apex(const double* tr, double* sp, double& hard, const MatConst& m) {
     double ptr = 0.;
     ptr += tr[0];
     ptr += tr[1];
@@ -312,7 +458,7 @@ __device__ __forceinline__ void apex(const double* tr, double* sp, double& hard,
 // FunctionsSigmaProjection.h:337-379.  The gradient (principal-space
 // d sigma / d sigma_trial) is produced only when WANT_GRAD.
 template <bool WANT_GRAD>
-__device__ __forceinline__ void project(const double* ev, double* sp, int& mtype, int& branch, double& hard,
+IPF_HD void project(const double* ev, double* sp, int& mtype, int& branch, double& hard,
                                         const MatConst& m, double* grad) {
     const double phi = ev[0] - ev[2] + (ev[0] + ev[2]) * m.sinphi - m.phi_c;
     sp[0] = ev[0];
@@ -370,7 +516,7 @@ __device__ __forceinline__ void project(const double* ev, double* sp, int& mtype
 
 // FunctionsTangentOperator.h:37-123, reduced to the 3x3 [xx, xy, yy] block of
 // FunctionsStressStrain.h:103-119 (TranslateDep) and scaled by w.
-__device__ __forceinline__ void tangent3(const double* grad, const double* sev, const double* eev, const double* v,
+IPF_HD void tangent3(const double* grad, const double* sev, const double* eev, const double* v,
                                          double G, double lambda, double w, double* dep9) {
     const int kival[6] = {0, 0, 0, 1, 1, 2};
     const int kjval[6] = {0, 1, 2, 1, 2, 2};
@@ -467,7 +613,7 @@ __device__ __forceinline__ void tangent3(const double* grad, const double* sev, 
 //   updated in place; alpha: committed alpha, updated (alpha_n + d_alpha).
 //   ws3: w * [sxx, sxy, syy]; sig4 (optional): [XX, XY, YY, ZZ] stress.
 template <bool WANT_DEP>
-__device__ __forceinline__ void point_sigma(const double* e3, double* ep4, double& alpha, int& mtype, int& branch,
+IPF_HD void point_sigma(const double* e3, double* ep4, double& alpha, int& mtype, int& branch,
                                             const MatConst& m, double w, double* ws3, double* wdep9) {
     // ComposeStrain + ElasticStrain + ComputeTrialStress (FunctionsStressStrain.h:10-55)
     double el[6];
@@ -485,12 +631,12 @@ __device__ __forceinline__ void point_sigma(const double* e3, double* ep4, doubl
     s[XZ] = m.mu * el[XZ];
     s[YZ] = m.mu * el[YZ];
     double ev[3], vec[9];
-    spectral(s, ev, vec);
+    spectral_ps(s, ev, vec);  // s[XZ] = s[YZ] = mu * 0.0 = +0
     double sp[3];
     if (WANT_DEP) {
         double eev[3], evec[9], grad[9];
         el[XY] = el[XY] / 2;
-        spectral(el, eev, evec);
+        spectral_ps(el, eev, evec);  // el[XZ] = el[YZ] = +0
         project<true>(ev, sp, mtype, branch, alpha, m, grad);
         tangent3(grad, ev, eev, evec, m.G, m.lambda, w, wdep9);
     } else {

@@ -1,0 +1,71 @@
+"""The production sizes (800 x 800 Q2: 5.76 M points, 5.1 M DoF; 800 x 800 Q3:
+10.24 M points, 11.5 M DoF) through properties that do not need the oracle:
+
+  * compact FP64 B and dense FP64 B give bit-identical residuals, branch
+    flags and plastic strains (the compact products only skip exact zeros);
+  * the fused residual equals the reference's separate steps
+    (Multiply -> ComputeSigma -> MultiplyTranspose + f_lin) to 1e-12 of
+    max |f_int| (same per-element sums, same color order);
+  * partition of unity: sum_a dN_a/dx = sum_a dN_a/dy = 0 per point, so the
+    internal forces of each direction sum to zero: |sum_x f_int| and
+    |sum_y f_int| <= 1e-10 * sum |f_int|.
+"""
+import numpy as np
+import pytest
+
+import intpointsfem_amd as ipf
+from intpointsfem_amd import wellbore as wb
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mesh", ["800x800-q2", "800x800-q3"])
+def test_production_mesh_properties(gpu, mesh):
+    import torch
+    m = wb.make_mesh(mesh)
+    sm = ipf.StructMatrix(m, device=0)
+    ctx = sm.context()
+    s = sm.sizes()
+    neq, npts = s["neq"], s["npts"]
+    print(f"{mesh}: {s['nblocks']} elements, {npts} points, neq {neq}", flush=True)
+    f_lin = sm.export_indexes()["rhs_linear"]
+    u = torch.as_tensor(wb.lame_displacement(m.nodes(), wb.normal_stress(10), scale=2.0), device=gpu)
+
+    def residual(fmt, commit):
+        ctx.set_b_format(fmt)
+        r = torch.zeros(neq, dtype=torch.float64, device=gpu)
+        ctx.set_update_mem(commit)
+        ctx.residual(u, r)
+        ctx.set_update_mem(False)
+        ctx.synchronize()
+        return r.cpu().numpy()
+
+    r_c = residual("compact-fp64", True)
+    st_c = ctx.get_state()
+    ctx.set_state(np.zeros(6 * npts))
+    r_d = residual("dense-fp64", True)
+    st_d = ctx.get_state()
+    assert (r_c == r_d).all()
+    assert np.array_equal(st_c[2], st_d[2]) and np.array_equal(st_c[3], st_d[3])
+    assert (st_c[0] == st_d[0]).all() and (st_c[1] == st_d[1]).all()
+    plastic = int((st_c[2] != 0).sum())
+    assert plastic > 0
+    # the reference's separate steps from the zero state
+    ctx.set_state(np.zeros(6 * npts))
+    strain = torch.zeros(3 * npts, dtype=torch.float64, device=gpu)
+    ws = torch.zeros(3 * npts, dtype=torch.float64, device=gpu)
+    res = torch.zeros(neq, dtype=torch.float64, device=gpu)
+    ctx.multiply(u, strain)
+    ctx.compute_sigma(strain, ws)
+    ctx.multiply_transpose(ws, res)
+    ctx.synchronize()
+    f_int = r_c - f_lin
+    scale = np.abs(f_int).max()
+    r_steps = res.cpu().numpy() + f_lin
+    err = np.abs(r_steps - r_c).max() / scale
+    print(f"{mesh}: {plastic} plastic points; separate steps vs fused {err:.2e} of max|f_int|", flush=True)
+    assert err <= 1e-12
+    sx, sy = f_int[0::2].sum(), f_int[1::2].sum()
+    tot = np.abs(f_int).sum()
+    print(f"{mesh}: sum f_int x {sx:.3e}, y {sy:.3e}, sum |f_int| {tot:.3e}", flush=True)
+    assert abs(sx) <= 1e-10 * tot and abs(sy) <= 1e-10 * tot
