@@ -1155,18 +1155,17 @@ int ipf_cg_solve(ipf_ctx* ctx, const double* b_dev, double* x_dev, double rtol, 
     double rel = 0.0;
     if (bnorm > 0.0) {
         const int check_every = 10;
+        launch_fill(n, ctx->dCq, 0.0, s);
         for (it = 1; it <= maxit; it++) {
-            // q = K p on the free DoFs
-            launch_fill(n, ctx->dCq, 0.0, s);
+            // q = K p on the free DoFs (q was cleared by the previous direction update)
             for (const auto& g : ctx->groups)
                 launch_group_tangent(ctx, g, ctx->dCp, ctx->dCq);
-            launch_cg_mask(n, ctx->dMask, ctx->dCq, s);
-            launch_dot(n, ctx->dCp, ctx->dCq, ctx->dPartial, nb, ctx->dScal + 1, s);
+            launch_cg_mask_dot(n, ctx->dMask, ctx->dCp, ctx->dCq, ctx->dPartial, nb, ctx->dScal + 1, s);
             launch_cg_scalars(ctx->dScal, 0, s);
-            launch_cg_update(n, ctx->dScal, ctx->dCp, ctx->dCq, x_dev, ctx->dCr, ctx->dDiag, ctx->dCz, s);
-            launch_dot(n, ctx->dCr, ctx->dCz, ctx->dPartial, nb, ctx->dScal + 2, s);
+            launch_cg_update_dot(n, ctx->dScal, ctx->dCp, ctx->dCq, x_dev, ctx->dCr, ctx->dDiag, ctx->dCz,
+                                 ctx->dPartial, nb, ctx->dScal + 2, s);
             launch_cg_scalars(ctx->dScal, 1, s);
-            launch_cg_direction(n, ctx->dScal, ctx->dCz, ctx->dCp, s);
+            launch_cg_direction_clear(n, ctx->dScal, ctx->dCz, ctx->dCp, ctx->dCq, s);
             if (it % check_every == 0 || it == maxit) {
                 launch_dot(n, ctx->dCr, ctx->dCr, ctx->dPartial, nb, ctx->dScal + 3, s);
                 double rr;

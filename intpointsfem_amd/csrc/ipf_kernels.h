@@ -122,6 +122,13 @@ void launch_cg_update(int64_t n, const double* scal, const double* p, const doub
                       const double* diag, double* z, hipStream_t stream);
 void launch_cg_direction(int64_t n, const double* scal, const double* z, double* p, hipStream_t stream);
 void launch_cg_scalars(double* scal, int stage, hipStream_t stream);
+// fused forms: q masked + p.q into *out; update + r.z into *out; direction + q = 0
+void launch_cg_mask_dot(int64_t n, const signed char* mask, const double* p, double* q, double* partial, int nb,
+                        double* out, hipStream_t stream);
+void launch_cg_update_dot(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
+                          const double* diag, double* z, double* partial, int nb, double* out, hipStream_t stream);
+void launch_cg_direction_clear(int64_t n, const double* scal, const double* z, double* p, double* q,
+                               hipStream_t stream);
 void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t stream);
 void launch_mask_copy(int64_t n, const signed char* mask, const double* src, double* dst, hipStream_t stream);
 int dot_blocks(int64_t n);

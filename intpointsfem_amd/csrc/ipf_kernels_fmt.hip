@@ -204,7 +204,9 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     if (valid) {
         const int64_t P = g.pt0 + tile * C::TP + lane;
         double ep[4] = {ld_stream(&st.ep0[P]), ld_stream(&st.ep1[P]), ld_stream(&st.ep2[P]), ld_stream(&st.ep3[P])};
-        double alpha = ld_stream(&st.alpha[P]);
+        // alpha only accumulates the plastic increment (H = 0: it does not
+        // enter the stress); without a commit it is not needed
+        double alpha = COMMIT ? ld_stream(&st.alpha[P]) : 0.0;
         int mt, br;
         point_sigma<false>(e3, ep, alpha, mt, br, tile_material(g, tile), ld_stream(&st.w[P]), ws, nullptr);
         if (COMMIT) {

@@ -75,6 +75,53 @@ __global__ void k_cg_update(int64_t n, const double* __restrict__ scal, const do
     z[i] = precond(ri, diag[i]);
 }
 
+// q = 0 on fixed DoFs, and the block partials of p.q (k_dot_partial's order)
+__global__ void __launch_bounds__(kDotThreads)
+k_cg_mask_dot(int64_t n, const signed char* __restrict__ mask, const double* __restrict__ p, double* __restrict__ q,
+              double* __restrict__ partial) {
+    __shared__ double sh[kDotThreads];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kDotThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kDotThreads) {
+        double qi = q[i];
+        if (mask[i]) {
+            qi = 0.0;
+            q[i] = 0.0;
+        }
+        s = s + p[i] * qi;
+    }
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// k_cg_update and the block partials of r.z
+__global__ void __launch_bounds__(kDotThreads)
+k_cg_update_dot(int64_t n, const double* __restrict__ scal, const double* __restrict__ p,
+                const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
+                const double* __restrict__ diag, double* __restrict__ z, double* __restrict__ partial) {
+    __shared__ double sh[kDotThreads];
+    const double alpha = scal[4];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kDotThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kDotThreads) {
+        x[i] = x[i] + alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const double zi = precond(ri, diag[i]);
+        z[i] = zi;
+        s = s + ri * zi;
+    }
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// p = z + beta p, and q = 0 for the next product
+__global__ void k_cg_direction_clear(int64_t n, const double* __restrict__ scal, const double* __restrict__ z,
+                                     double* __restrict__ p, double* __restrict__ q) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    p[i] = z[i] + scal[5] * p[i];
+    q[i] = 0.0;
+}
+
 __global__ void k_cg_direction(int64_t n, const double* __restrict__ scal, const double* __restrict__ z,
                                double* __restrict__ p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -107,6 +154,23 @@ __global__ void k_mask_copy(int64_t n, const signed char* __restrict__ mask, con
 inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
+
+void launch_cg_mask_dot(int64_t n, const signed char* mask, const double* p, double* q, double* partial, int nb,
+                        double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_mask_dot, dim3(nb), dim3(kDotThreads), 0, s, n, mask, p, q, partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, nb, partial, out);
+}
+
+void launch_cg_update_dot(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
+                          const double* diag, double* z, double* partial, int nb, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_update_dot, dim3(nb), dim3(kDotThreads), 0, s, n, scal, p, q, x, r, diag, z, partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, nb, partial, out);
+}
+
+void launch_cg_direction_clear(int64_t n, const double* scal, const double* z, double* p, double* q, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_cg_direction_clear, grid1(n), dim3(256), 0, s, n, scal, z, p, q);
+}
 
 int dot_blocks(int64_t n) {
     int64_t b = (n + kDotThreads - 1) / kDotThreads;
