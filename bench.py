@@ -369,7 +369,8 @@ def main():
     kname = kernel_name(fmt0, shape)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"k_residual_{args.mesh}_traffic.json")
-    if args.state == "lame" and os.path.exists(tfile):
+    # bytes per launch do not depend on the state (same loads and stores at every point)
+    if os.path.exists(tfile):
         with open(tfile) as f:
             tk = json.load(f)["kernels"]
         if kname in tk:

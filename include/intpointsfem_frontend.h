@@ -38,6 +38,11 @@ int ipf_cmesh_material(const ipf_cmesh *m, double out[5]);
 int ipf_cmesh_nodes(const ipf_cmesh *m, double *xy /* [2*nnodes] */);
 /* connect (node) lists per element: conn_ptr [nelem+1], conn [conn_ptr[nelem]] */
 int ipf_cmesh_connectivity(const ipf_cmesh *m, int64_t *conn_ptr, int64_t *conn);
+/* Physical coordinates of the integration points [2*npts] (x, y), in the
+ * reference's point order (element by element, eta fastest): the locations
+ * of the per-point stresses and strains (post-processing, e.g. the plot over
+ * y = 0 of RK_Verification). */
+int ipf_cmesh_point_coords(const ipf_cmesh *m, double *xy);
 /* 1 for DoFs fixed by the symmetry conditions (u_y on theta=0, u_x on theta=pi/2) */
 int ipf_cmesh_dirichlet_mask(const ipf_cmesh *m, signed char *mask /* [neq] */);
 /* the material's UpdateMem switch (main.cpp:487-522) */

@@ -123,6 +123,7 @@ def lib():
         "ipf_cmesh_connectivity": (_int, [_vp, _lp, _lp]),
         "ipf_cmesh_dirichlet_mask": (_int, [_vp, _bp]),
         "ipf_cmesh_set_update_mem": (_int, [_vp, _int]),
+        "ipf_cmesh_point_coords": (_int, [_vp, _dp]),
         "ipf_sm_create": (_int, [_vp, _int, ctypes.POINTER(_vp)]),
         "ipf_sm_destroy": (_int, [_vp]),
         "ipf_sm_context": (_vp, [_vp]),
@@ -227,6 +228,13 @@ class CompMesh:
         xy = np.zeros(2 * n)
         _check(lib().ipf_cmesh_nodes(self._h, _p(xy, _dp)), what="ipf_cmesh_nodes")
         return xy.reshape(n, 2)
+
+    def point_coords(self):
+        """Integration-point coordinates [npts, 2] in the reference point order."""
+        inf = self.info()
+        xy = np.zeros(2 * inf["nelem"] * inf["npts_per_element"])
+        _check(lib().ipf_cmesh_point_coords(self._h, _p(xy, _dp)), what="ipf_cmesh_point_coords")
+        return xy.reshape(-1, 2)
 
     def connectivity(self):
         inf = self.info()

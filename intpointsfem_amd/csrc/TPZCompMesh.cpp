@@ -125,6 +125,17 @@ void TPZCompMesh::NodeCoords(int64_t node, double& x, double& y) const {
     y = N[0] * X[0][1] + N[1] * X[1][1] + N[2] * X[2][1] + N[3] * X[3][1];
 }
 
+void TPZCompMesh::PointCoords(int64_t e, int ip, double& x, double& y) const {
+    const int n1 = fP + 1;
+    const double xi = fGx[ip / n1], eta = fGx[ip % n1];
+    double X[4][2];
+    CornerCoords(e, X);
+    const double N[4] = {(1 - xi) * (1 - eta) / 4, (1 + xi) * (1 - eta) / 4, (1 + xi) * (1 + eta) / 4,
+                         (1 - xi) * (1 + eta) / 4};
+    x = N[0] * X[0][0] + N[1] * X[1][0] + N[2] * X[2][0] + N[3] * X[3][0];
+    y = N[0] * X[0][1] + N[1] * X[1][1] + N[2] * X[2][1] + N[3] * X[3][1];
+}
+
 double TPZCompMesh::Lagrange(int k, double x) const {
     double v = 1.0;
     const double xk = -1.0 + 2.0 * k / fP;

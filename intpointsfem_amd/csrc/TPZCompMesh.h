@@ -46,6 +46,8 @@ public:
     void CornerCoords(int64_t e, double X[4][2]) const;
     // Node coordinates (Q1 map of the owning cell at the node's reference position).
     void NodeCoords(int64_t node, double& x, double& y) const;
+    // Coordinates of integration point ip (eta fastest) of element e.
+    void PointCoords(int64_t e, int ip, double& x, double& y) const;
 
     // Reference-element data for the element order: local node a has grid
     // position (ga[a], gb[a]) in 0..p along (xi, eta).

@@ -134,6 +134,15 @@ int ipf_cmesh_dirichlet_mask(const ipf_cmesh* m, signed char* mask) {
     return IPF_OK;
 }
 
+int ipf_cmesh_point_coords(const ipf_cmesh* m, double* xy) {
+    if (!m || !xy) return null_arg("ipf_cmesh_point_coords");
+    const TPZCompMesh& c = *m->mesh;
+    const int nip = c.NIntPoints();
+    for (int64_t e = 0; e < c.NElements(); e++)
+        for (int ip = 0; ip < nip; ip++) c.PointCoords(e, ip, xy[2 * (e * nip + ip)], xy[2 * (e * nip + ip) + 1]);
+    return IPF_OK;
+}
+
 int ipf_cmesh_set_update_mem(ipf_cmesh* m, int update_mem) {
     if (!m) return null_arg("ipf_cmesh_set_update_mem");
     m->mesh->SetUpdateMem(update_mem != 0);
