@@ -13,7 +13,9 @@ Bars:
   * training on pairs from a batch of instances: held-out branch accuracy
     >= 99 % and far above the majority-class rate;
   * branch-sorted ComputeSigma: identical results (stresses, flags, plastic
-    strain) to the unsorted one; the order groups points by predicted branch.
+    strain) to the unsorted one; the order groups points by predicted branch;
+    with corrupted (NaN) weights the order is still a permutation and the
+    results are still identical.
 """
 import numpy as np
 import pytest
@@ -166,3 +168,29 @@ def test_branch_sorted_compute_sigma_is_exact(gpu):
     d_natural = learned.divergence(be)
     print(f"mixed-branch wavefronts: natural order {d_natural:.4f}, sorted by exact branch {d_sorted:.4f}")
     assert d_sorted * (npts // 64) <= 4
+
+
+def test_corrupted_weights_keep_results_exact(gpu):
+    """NaN weights: the predictions are garbage, the order is still a
+    permutation and ComputeSigma's results are unchanged."""
+    import torch
+    m, sm, ctx, B, ix = _setup()
+    npts = sm.sizes()["npts"]
+    u = torch.as_tensor(wb.lame_displacement(m.nodes(), wb.normal_stress(10), scale=6.0), device=gpu)
+    feat, _, _ = learned.point_features(ctx, u, npts, gpu)
+    strain = torch.zeros(3 * npts, dtype=torch.float64, device=gpu)
+    ctx.multiply(u, strain)
+    ws0 = torch.zeros(3 * npts, dtype=torch.float64, device=gpu)
+    ctx.compute_sigma(strain, ws0)
+    ctx.synchronize()
+    mlp = learned.new_mlp(seed=1)
+    p = mlp.params()
+    p[M.OFF_W2:M.OFF_B2] = np.nan
+    mlp.set_params(p)
+    order = torch.zeros(npts, dtype=torch.int32, device=gpu)
+    ws1 = torch.zeros(3 * npts, dtype=torch.float64, device=gpu)
+    torch.cuda.synchronize()
+    learned.compute_sigma_sorted(ctx, mlp, strain, ws1, feat, order)
+    ctx.synchronize()
+    assert np.array_equal(np.sort(order.cpu().numpy()), np.arange(npts))
+    assert (ws1.cpu().numpy() == ws0.cpu().numpy()).all()
