@@ -4,24 +4,28 @@ A step is one full residual evaluation r = f_lin - sum_e B_e^T (w sigma(B_e u_e)
 (TPZElastoPlasticIntPointsStructMatrix::Assemble(rhs)) on the 800x800 Q2
 quarter-annulus wellbore (640,000 elements, 5.76 M integration points,
 5,126,402 DoF), plain FP64, with u and r resident in HBM, at the converged
-Newton state of the 10 load steps (||r||/||r0|| <= 1e-8; the solve -- mixed
-Newton, FP32 tangent products, FP64 residuals -- runs first, untimed).  The library's B
-format for these blocks is compact FP64 (dN/dx, dN/dy per node and point),
-whose results are bit-identical to the dense blocks.
+Newton state of the 10 load steps (||r||/||r0|| <= 1e-8; the solve -- Newton
+with Jacobi-PCG inner solves, FP64 throughout -- runs first, untimed).  The
+data is set up on the GPU from the element corners, and the library's B
+format is the geometric one: B and the weights recomputed inside the kernel
+from 8 corner coordinates per element, bit-identical to the stored dense and
+compact FP64 blocks built from the same geometry.
 
 Prints ONE JSON line on stdout (progress goes to stderr):
   value       Mint-pts/s = points * steps / timed seconds / 1e6
   roofline    the fused per-color residual kernel against HBM: its compulsory
-              bytes -- the stored B values once (compact FP64: dN/dx, dN/dy per
-              node and point), a 4-byte DoF index per element column, weight +
-              4 plastic-strain components per point, u read + r written once --
-              over the kernel's average duration from HIP events around each
-              launch; `traffic` = HBM bytes per launch measured by rocprofv3
-              (profiles/), `dense_fp64_bytes_rate` = the rate on the dense
-              blocks' byte count, bytes this kernel does not move
+              bytes -- what the B format stores (geometric: 8 corner
+              coordinates per element; compact: dN/dx, dN/dy per node and
+              point, + a weight per point), a 4-byte DoF index per element
+              column, 4 plastic-strain components per point, u read + r
+              read-modify-written once -- over the kernel's average duration
+              from HIP events around each launch; `traffic` = HBM bytes per
+              launch measured by rocprofv3 (profiles/), `dense_fp64_bytes_rate`
+              = the rate on the dense blocks' byte count, bytes this kernel
+              does not move
   cpu_baseline the CPU oracle (same algorithm, OpenMP) on the host cores
-  b_format_variants  the same residual with the other B formats (dense FP64,
-              dense FP32, compact FP32; ipf_set_b_format): time, rate on each
+  b_format_variants  the same residual with the other B formats (dense and
+              compact FP64 and FP32; ipf_set_b_format): time, rate on each
               format's stored bytes, deviation from the FP64 residual
 """
 import argparse
@@ -43,13 +47,17 @@ def log(*a):
 
 
 def algorithmic_bytes(blocks, npts, neq, fmt="dense-fp64"):
-    """Bytes one residual must move: the stored B values once (dense: every
-    block entry; compact: dN/dx, dN/dy per node and point), a 4-byte DoF index
-    per block column, weight + 4 plastic-strain components per point, u read
-    and r read-modify-written once."""
+    """Bytes one residual must move: the B data once (dense: every block
+    entry; compact: dN/dx, dN/dy per node and point; geometric: 8 corner
+    coordinates per element), the weight per point (not for the geometric
+    form, which computes it), a 4-byte DoF index per block column, 4
+    plastic-strain components per point, u read and r read-modify-written
+    once."""
     import numpy as np
     R = blocks["row_sizes"].astype(np.int64)
     C = blocks["col_sizes"].astype(np.int64)
+    if fmt == "geometry-fp64":
+        return int((64 + 4 * C).sum()) + npts * 32 + neq * (8 + 8)
     word = 4 if fmt.endswith("fp32") else 8
     nval = (R // 3) * C if fmt.startswith("compact") else R * C
     return int((word * nval + 4 * C).sum()) + npts * (8 + 32) + neq * (8 + 8)
@@ -74,7 +82,8 @@ def kernel_name(fmt, shape):
     """Name rocprofv3 shows for the residual kernel of a B format and element shape."""
     if fmt == "dense-fp64":
         return f"void k_residual<{shape}, false>"
-    return f"void k_residual_fmt<{shape}, {('dense-fp64', 'compact-fp64', 'dense-fp32', 'compact-fp32').index(fmt)}, false>"
+    import intpointsfem_amd as ipf
+    return f"void k_residual_fmt<{shape}, {ipf.B_FORMATS.index(fmt)}, false>"
 
 
 def launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt):
@@ -275,8 +284,9 @@ def main():
     ap.add_argument("--cg-rtol", type=float, default=1e-1,
                     help="inner Jacobi-PCG tolerance of each Newton step (the Newton loop converges linearly "
                          "with the reference's tangent, so a loose inner solve costs few extra Newton steps)")
-    ap.add_argument("--solve-tangent-format", default="compact-fp32",
-                    help="B format of the CG matvecs of the --state newton solve (the residual stays FP64)")
+    ap.add_argument("--solve-tangent-format", default=None,
+                    help="B format of the CG matvecs of the --state newton solve (default: the library's, "
+                         "geometric FP64; the residual stays FP64 whatever this is)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the path runs on a single "
@@ -323,8 +333,9 @@ def main():
         # 10 load steps of Newton + Jacobi-PCG on the GPU; the state is committed after each
         ctx.set_dirichlet_mask(mesh.dirichlet_mask())
         fmt_default = ctx.b_format()
-        # mixed Newton: FP64 residual, tangent products in the chosen format
-        ctx.set_b_format(fmt_default[0], tangent=args.solve_tangent_format)
+        # FP64 residual; tangent products in the chosen format (FP32: mixed Newton)
+        solve_tf = args.solve_tangent_format or fmt_default[1]
+        ctx.set_b_format(fmt_default[0], tangent=solve_tf)
         f_unit = sm.export_indexes()["rhs_linear"] / wb.normal_stress(wb.NSTEPS)
         u = torch.zeros(neq, dtype=torch.float64, device=dev)
         t1 = time.time()
@@ -338,7 +349,7 @@ def main():
         ctx.set_b_format(*fmt_default)
         u_host = u.cpu().numpy()
         _, _, mt, br = ctx.get_state()
-        newton_info = {"solve_tangent_format": args.solve_tangent_format, "load_steps": wb.NSTEPS, "newton_iterations": [int(s["iterations"]) for s in steps],
+        newton_info = {"solve_tangent_format": solve_tf, "load_steps": wb.NSTEPS, "newton_iterations": [int(s["iterations"]) for s in steps],
                        "final_rel_residual": steps[-1]["rel_residual"],
                        "converged": all(s["converged"] == 1.0 for s in steps),
                        "cg_iterations": int(sum(s["cg_iterations"] for s in steps)),
@@ -349,7 +360,7 @@ def main():
     log(f"[bench] branches: {branch_counts}")
     ep_state, al_state, _, _ = ctx.get_state()  # the committed state the timed residuals start from
 
-    fmt0 = ctx.b_format()[0]  # the library's choice for these blocks (compact FP64 when it applies)
+    fmt0 = ctx.b_format()[0]  # the library's choice for these data (geometric FP64 after the device setup)
     shape = info["porder"] - 1
     total_ms = time_residuals(ctx, u, r, stream, args.steps, args.warmup)
     ms_per_step = total_ms / args.steps

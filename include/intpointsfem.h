@@ -85,6 +85,34 @@ int ipf_transfer_data(ipf_ctx *ctx, int64_t nblocks, const int *row_sizes, const
                       const double *storage, const int *dof_indexes, const int *color_indexes, int64_t ncolor,
                       int64_t neq, const double *weight);
 
+/* Setup on the device: the data of ipf_transfer_data with B and the weights
+ * computed on the GPU from each element's geometry instead of uploaded.
+ * Replaces the host loops of TPZNumericalIntegrator::SetUpIrregularBlocksData
+ * (TPZNumericalIntegrator.cpp:272-388: B = J^-1 dphi per point, written as
+ * B(3ip + c + j, 2a + c) = dN_a/dx_j) and the weights of SetUpIndexes
+ * (.cpp:390-426: w |det J|), followed by TransferDataToGPU.
+ *   nshape, shape_sizes [2*nshape]  (n_ip, n_sh) of each element shape;
+ *   shape_tables   per shape, concatenated, n_ip*(2*n_sh + 3) doubles:
+ *                  dphi[n_ip][n_sh][2] = (dN_a/dxi, dN_a/deta) at each point,
+ *                  then xi[n_ip], eta[n_ip], w_q[n_ip] (the quadrature);
+ *   elem_shape [nblocks]   shape index of each element;
+ *   corners [8*nblocks]    the 4 corners of each element's bilinear map (x0,
+ *                  y0, ..., x3, y3; counter-clockwise from (xi,eta) = (-1,-1));
+ *   dof_indexes, color_indexes, ncolor, neq as in ipf_transfer_data, the
+ *                  element's columns being (x, y) of each shape function.
+ * Per point: J from the corners, dN_a/dx = a dN_a/dxi + b dN_a/deta,
+ * dN_a/dy = c dN_a/dxi + d dN_a/deta with (a, b, c, d) the entries of J^-1
+ * (reciprocal of det J, then products), weight w_q |det J|.  Blocks are
+ * numbered and laid out as the reference's (row sizes 3 n_ip, column sizes
+ * 2 n_sh, points in element order).  Selects IPF_B_GEOMETRY_FP64 for the
+ * residual and the tangent product; the stored forms stay available.
+ * An element whose det J vanishes at a point gets non-finite values there.
+ * Fails like ipf_transfer_data, and with IPF_EINVAL for an unknown shape
+ * index, two shapes of equal sizes or a non-finite corner coordinate. */
+int ipf_transfer_geometry(ipf_ctx *ctx, int nshape, const int *shape_sizes, const double *shape_tables,
+                          int64_t nblocks, const int *elem_shape, const double *corners, const int *dof_indexes,
+                          const int *color_indexes, int64_t ncolor, int64_t neq);
+
 /* Material of TPZConstitutiveLawProcessor::SetMaterial (.cpp:127-129); the
  * reference reads lambda, mu, phi, psi, c from it per call (.cpp:152-158).
  * Angles in radians. */
@@ -154,7 +182,10 @@ enum {
     IPF_B_DENSE_FP64 = 0,   /* every stored value of the reference's blocks (default) */
     IPF_B_COMPACT_FP64 = 1, /* (dN/dx, dN/dy) per node and point; same results */
     IPF_B_DENSE_FP32 = 2,   /* FP32 products with FP32 accumulation */
-    IPF_B_COMPACT_FP32 = 3
+    IPF_B_COMPACT_FP32 = 3,
+    IPF_B_GEOMETRY_FP64 = 4 /* nothing stored per point: the compact FP64 values and
+                               the weight recomputed in the kernel from the element's
+                               corners (after ipf_transfer_geometry); compact FP64 results */
 };
 
 /* Selects the B representation of the fused residual (ipf_residual,
@@ -169,11 +200,14 @@ enum {
  * B^T(w sigma) in FP32 with FP32 accumulation, everything else in FP64.  An
  * FP64 residual with an FP32 tangent makes ipf_newton an iterative refinement:
  * the FP32 products only steer the correction, the FP64 residual decides
- * convergence.  No reference counterpart: the reference stores dense FP64
- * blocks only (TPZIrregularBlocksMatrix.h:20). */
+ * convergence.  The geometric form (after ipf_transfer_geometry) reads 8
+ * corner coordinates per element instead of any per-point B value or weight
+ * and gives the compact FP64 results bit for bit.  No reference counterpart:
+ * the reference stores dense FP64 blocks only (TPZIrregularBlocksMatrix.h:20). */
 int ipf_set_b_format(ipf_ctx *ctx, int residual_format, int tangent_format);
 /* The formats in use.  ipf_transfer_data selects IPF_B_COMPACT_FP64 for both
- * when every block has the 2D elasticity pattern, IPF_B_DENSE_FP64 otherwise. */
+ * when every block has the 2D elasticity pattern, IPF_B_DENSE_FP64 otherwise;
+ * ipf_transfer_geometry selects IPF_B_GEOMETRY_FP64. */
 int ipf_get_b_format(const ipf_ctx *ctx, int *residual_format, int *tangent_format);
 
 /* ---- learned return-branch predictor (no reference counterpart) ---- */

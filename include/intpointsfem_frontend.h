@@ -43,6 +43,15 @@ int ipf_cmesh_connectivity(const ipf_cmesh *m, int64_t *conn_ptr, int64_t *conn)
  * of the per-point stresses and strains (post-processing, e.g. the plot over
  * y = 0 of RK_Verification). */
 int ipf_cmesh_point_coords(const ipf_cmesh *m, double *xy);
+/* Geometry input of ipf_transfer_geometry for this mesh: sizes = (n_ip,
+ * n_sh) and, when tab is not NULL, the shape table (n_ip*(2 n_sh + 3)
+ * doubles, layout of ipf_transfer_geometry) -- the data NeoPZ's integration
+ * rule and shape functions give the reference's setup
+ * (TPZNumericalIntegrator.cpp:344-381). */
+int ipf_cmesh_shape_table(const ipf_cmesh *m, int sizes[2], double *tab);
+/* The 4 corners of each element's bilinear map, [8 * nelements], element
+ * order (gel->Jacobian's input at TPZNumericalIntegrator.cpp:424). */
+int ipf_cmesh_corners(const ipf_cmesh *m, double *corners);
 /* 1 for DoFs fixed by the symmetry conditions (u_y on theta=0, u_x on theta=pi/2) */
 int ipf_cmesh_dirichlet_mask(const ipf_cmesh *m, signed char *mask /* [neq] */);
 /* the material's UpdateMem switch (main.cpp:487-522) */

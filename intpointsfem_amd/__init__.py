@@ -28,7 +28,7 @@ _LIB_PATH = os.path.join(_HERE, "libintpointsfem.so")
 
 IPF_OK = 0
 BRANCH_NAMES = ("elastic", "main", "right_edge", "left_edge", "apex")
-B_FORMATS = ("dense-fp64", "compact-fp64", "dense-fp32", "compact-fp32")
+B_FORMATS = ("dense-fp64", "compact-fp64", "dense-fp32", "compact-fp32", "geometry-fp64")
 
 
 class IPFError(RuntimeError):
@@ -75,6 +75,7 @@ def lib():
         "ipf_get_stream": (_vp, [_vp]),
         "ipf_synchronize": (_int, [_vp]),
         "ipf_transfer_data": (_int, [_vp, _i64, _ip, _ip, _lp, _ip, _ip, _dp, _ip, _ip, _i64, _i64, _dp]),
+        "ipf_transfer_geometry": (_int, [_vp, _int, _ip, _dp, _i64, _ip, _dp, _ip, _ip, _i64, _i64]),
         "ipf_set_material": (_int, [_vp, _d, _d, _d, _d, _d]),
         "ipf_set_material_table": (_int, [_vp, _int, _dp]),
         "ipf_set_element_materials": (_int, [_vp, _i64, _ip]),
@@ -124,6 +125,8 @@ def lib():
         "ipf_cmesh_dirichlet_mask": (_int, [_vp, _bp]),
         "ipf_cmesh_set_update_mem": (_int, [_vp, _int]),
         "ipf_cmesh_point_coords": (_int, [_vp, _dp]),
+        "ipf_cmesh_shape_table": (_int, [_vp, _ip, _dp]),
+        "ipf_cmesh_corners": (_int, [_vp, _dp]),
         "ipf_sm_create": (_int, [_vp, _int, ctypes.POINTER(_vp)]),
         "ipf_sm_destroy": (_int, [_vp]),
         "ipf_sm_context": (_vp, [_vp]),
@@ -243,6 +246,21 @@ class CompMesh:
         _check(lib().ipf_cmesh_connectivity(self._h, _p(ptr, _lp), _p(conn, _lp)), what="ipf_cmesh_connectivity")
         return ptr, conn
 
+    def shape_table(self):
+        """((n_ip, n_sh), table) -- the shape table of ipf_transfer_geometry."""
+        sz = np.zeros(2, dtype=np.int32)
+        _check(lib().ipf_cmesh_shape_table(self._h, _p(sz, _ip), None), what="ipf_cmesh_shape_table")
+        nip, nsh = int(sz[0]), int(sz[1])
+        tab = np.zeros(nip * (2 * nsh + 3))
+        _check(lib().ipf_cmesh_shape_table(self._h, _p(sz, _ip), _p(tab, _dp)), what="ipf_cmesh_shape_table")
+        return (nip, nsh), tab
+
+    def corners(self):
+        """[nelem, 8] corners of each element's bilinear map (x0, y0, ..., x3, y3)."""
+        c = np.zeros(8 * self.info()["nelem"])
+        _check(lib().ipf_cmesh_corners(self._h, _p(c, _dp)), what="ipf_cmesh_corners")
+        return c.reshape(-1, 8)
+
     def dirichlet_mask(self):
         m = np.zeros(self.info()["neq"], dtype=np.int8)
         _check(lib().ipf_cmesh_dirichlet_mask(self._h, _p(m, _bp)), what="ipf_cmesh_dirichlet_mask")
@@ -283,6 +301,23 @@ class DeviceContext:
         self._c(lib().ipf_transfer_data(self._h, a[0].size, _p(a[0], _ip), _p(a[1], _ip), _p(a[2], _lp),
                                         _p(a[3], _ip), _p(a[4], _ip), _p(a[5], _dp), _p(a[6], _ip),
                                         _p(a[7], _ip), int(ncolor), int(neq), _p(a[8], _dp)), "ipf_transfer_data")
+
+    def transfer_geometry(self, shape_sizes, shape_tables, elem_shape, corners, dof_indexes, color_indexes, ncolor,
+                          neq):
+        """Device setup from geometry (ipf_transfer_geometry): shape_sizes
+        [(n_ip, n_sh), ...], shape_tables the tables concatenated, elem_shape
+        [nblocks] shape index per element, corners [nblocks, 8]."""
+        sz = np.ascontiguousarray(shape_sizes, dtype=np.int32).reshape(-1, 2)
+        tab = np.ascontiguousarray(shape_tables, dtype=np.float64)
+        es = np.ascontiguousarray(elem_shape, dtype=np.int32)
+        cr = np.ascontiguousarray(corners, dtype=np.float64).reshape(-1)
+        d = np.ascontiguousarray(dof_indexes, dtype=np.int32)
+        c = np.ascontiguousarray(color_indexes, dtype=np.int32)
+        if tab.size != sum(int(n) * (2 * int(m) + 3) for n, m in sz) or cr.size != 8 * es.size:
+            raise IPFError("transfer_geometry: table or corner array of the wrong size")
+        self._c(lib().ipf_transfer_geometry(self._h, sz.shape[0], _p(sz, _ip), _p(tab, _dp), es.size, _p(es, _ip),
+                                            _p(cr, _dp), _p(d, _ip), _p(c, _ip), int(ncolor), int(neq)),
+                "ipf_transfer_geometry")
 
     def set_material(self, lam, mu, phi, psi, cohesion):
         self._c(lib().ipf_set_material(self._h, lam, mu, phi, psi, cohesion), "ipf_set_material")
@@ -356,8 +391,9 @@ class DeviceContext:
 
     def set_b_format(self, fmt, tangent=None):
         """B format of the fused residual and (``tangent``, default: the same)
-        of the tangent product: 0 dense FP64 (default), 1 compact FP64,
-        2 dense FP32, 3 compact FP32, or the names in B_FORMATS."""
+        of the tangent product: 0 dense FP64, 1 compact FP64, 2 dense FP32,
+        3 compact FP32, 4 geometric FP64 (after transfer_geometry), or the
+        names in B_FORMATS."""
         def code(f):
             return B_FORMATS.index(f) if isinstance(f, str) else int(f)
         fr = code(fmt)

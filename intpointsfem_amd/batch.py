@@ -107,6 +107,58 @@ def concat_instances(meshes):
         eq_offsets=eq0, pt_offsets=pt0, block_offsets=off(nb))
 
 
+def concat_geometry(meshes):
+    """Host side of a batch for the device setup (ipf_transfer_geometry): the
+    instances' element corners, one shape table per element order, the shape
+    index per element, DoF / color indexes and f_lin stacked with offsets, the
+    per-block material id and the material table.  No B values on the host."""
+    parts = []
+    for m in meshes:
+        sm = StructMatrix(m, device=-1)
+        parts.append((m, sm.sizes(), sm.export_indexes(), m.shape_table(), m.corners()))
+    n = len(parts)
+    nb = np.array([p[1]["nblocks"] for p in parts], dtype=np.int64)
+    neq = np.array([p[1]["neq"] for p in parts], dtype=np.int64)
+    npt = np.array([p[1]["npts"] for p in parts], dtype=np.int64)
+    rows = np.array([p[1]["rows"] for p in parts], dtype=np.int64)
+    cols = np.array([p[1]["cols"] for p in parts], dtype=np.int64)
+
+    def off(a):
+        return np.concatenate([[0], np.cumsum(a)]).astype(np.int64)
+
+    eq0, r0, c0, pt0 = off(neq), off(rows), off(cols), off(npt)
+    ncolor = max(p[1]["ncolor"] for p in parts)
+    NEQ = int(eq0[-1])
+    if r0[-1] >= 2 ** 31 or c0[-1] >= 2 ** 31 or NEQ * ncolor >= 2 ** 31:
+        raise ValueError("batch too large for the int32 row / column / color indexes of the block format")
+    sizes, tables, shape_of = [], [], []
+    for p in parts:
+        key = p[3][0]
+        if key not in sizes:
+            sizes.append(key)
+            tables.append(p[3][1])
+        shape_of.append(sizes.index(key))
+    dof = np.concatenate([p[2]["dof_indexes"].astype(np.int64) + eq0[i] for i, p in enumerate(parts)])
+    colr = np.concatenate([(p[2]["color_indexes"].astype(np.int64) - p[2]["dof_indexes"]) // neq[i]
+                           for i, p in enumerate(parts)])
+    params = []
+    for m, *_ in parts:
+        d = m.material()
+        params.append([d["lambda"], d["mu"], d["phi"], d["psi"], d["cohesion"]])
+    return dict(
+        shape_sizes=np.array(sizes, dtype=np.int32), shape_tables=np.concatenate(tables),
+        elem_shape=np.repeat(np.array(shape_of, dtype=np.int32), nb),
+        corners=np.concatenate([p[4] for p in parts]),
+        dof_indexes=dof.astype(np.int32), color_indexes=(dof + colr * NEQ).astype(np.int32),
+        ncolor=ncolor, neq=NEQ,
+        rhs_linear=np.concatenate([p[2]["rhs_linear"] for p in parts]),
+        element_material=np.repeat(np.arange(n, dtype=np.int32), nb),
+        material_table=np.array(params),
+        eq_offsets=eq0, pt_offsets=pt0, block_offsets=off(nb),
+        row_sizes=np.repeat(np.array([3 * s[0] for s in sizes], dtype=np.int32)[shape_of], nb),
+        col_sizes=np.repeat(np.array([2 * s[1] for s in sizes], dtype=np.int32)[shape_of], nb))
+
+
 class Batch:
     """Independent instances in one DeviceContext (see the module docstring).
 
@@ -114,13 +166,22 @@ class Batch:
     ctx, n, neq (total), npts, eq_offsets [n+1], pt_offsets [n+1] (each
     instance's points in its reference order, stacked), rhs_linear."""
 
-    def __init__(self, meshes, device=0):
-        h = concat_instances(meshes)
+    def __init__(self, meshes, device=0, setup="device"):
+        """setup "device": B and the weights computed on the GPU from the
+        element corners (ipf_transfer_geometry); "host": the host-built
+        blocks uploaded (ipf_transfer_data).  Same data either way."""
+        if setup not in ("device", "host"):
+            raise ValueError(f"setup must be 'device' or 'host', not {setup!r}")
+        h = concat_geometry(meshes) if setup == "device" else concat_instances(meshes)
         self.ctx = DeviceContext(device=device)
         self.ctx.set_element_materials(h["element_material"])
-        self.ctx.transfer_data(h["row_sizes"], h["col_sizes"], h["matrix_position"], h["row_first"], h["col_first"],
-                               h["storage"], h["dof_indexes"], h["color_indexes"], h["ncolor"], h["neq"],
-                               h["weight"])
+        if setup == "device":
+            self.ctx.transfer_geometry(h["shape_sizes"], h["shape_tables"], h["elem_shape"], h["corners"],
+                                       h["dof_indexes"], h["color_indexes"], h["ncolor"], h["neq"])
+        else:
+            self.ctx.transfer_data(h["row_sizes"], h["col_sizes"], h["matrix_position"], h["row_first"],
+                                   h["col_first"], h["storage"], h["dof_indexes"], h["color_indexes"], h["ncolor"],
+                                   h["neq"], h["weight"])
         self.ctx.set_material_table(h["material_table"])
         self.ctx.set_rhs_linear(h["rhs_linear"])
         self.n = len(meshes)

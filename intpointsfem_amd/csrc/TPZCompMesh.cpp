@@ -164,6 +164,32 @@ double TPZCompMesh::LagrangeDeriv(int k, double x) const {
     return s;
 }
 
+void TPZCompMesh::ShapeTable(std::vector<double>& tab) const {
+    const int n1 = fP + 1, nip = NIntPoints(), nsh = (int)NShapeF();
+    std::vector<double> L(n1 * n1), dL(n1 * n1);  // [q][k]: basis k at Gauss point q
+    for (int q = 0; q < n1; q++)
+        for (int k = 0; k < n1; k++) {
+            L[q * n1 + k] = Lagrange(k, fGx[q]);
+            dL[q * n1 + k] = LagrangeDeriv(k, fGx[q]);
+        }
+    tab.assign((size_t)nip * (2 * nsh + 3), 0.0);
+    double* dphi = tab.data();
+    double* xi = dphi + (size_t)nip * 2 * nsh;
+    double* eta = xi + nip;
+    double* wq = eta + nip;
+    for (int qi = 0; qi < n1; qi++)
+        for (int qj = 0; qj < n1; qj++) {
+            const int ip = qi * n1 + qj;  // eta fastest
+            xi[ip] = fGx[qi];
+            eta[ip] = fGx[qj];
+            wq[ip] = fGw[qi] * fGw[qj];
+            for (int a = 0; a < nsh; a++) {
+                dphi[((size_t)ip * nsh + a) * 2 + 0] = dL[qi * n1 + fGa[a]] * L[qj * n1 + fGb[a]];
+                dphi[((size_t)ip * nsh + a) * 2 + 1] = L[qi * n1 + fGa[a]] * dL[qj * n1 + fGb[a]];
+            }
+        }
+}
+
 void TPZCompMesh::SetMaterial(double E, double nu, double c, double phi, double psi) {
     if (!(E > 0) || !(nu > -1.0) || !(nu < 0.5)) throw std::invalid_argument("TPZCompMesh: bad E / nu");
     fMu = E / (2. * (1. + nu));

@@ -61,6 +61,12 @@ public:
     double Lagrange(int k, double x) const;
     double LagrangeDeriv(int k, double x) const;
 
+    // Shape table of the element order (layout of ipf_geometry.h, size
+    // NIntPoints() * (2 NShapeF() + 3)): dN_a/dxi = L'_ga(xi) L_gb(eta),
+    // dN_a/deta = L_ga(xi) L'_gb(eta) at each point, the points' (xi, eta)
+    // and the tensor Gauss weights w_i w_j.
+    void ShapeTable(std::vector<double>& tab) const;
+
     // Material (TElastoPlasticData): Young's modulus, Poisson ratio, cohesion,
     // friction and dilation angles (radians).  lambda and mu follow the
     // standard isotropic formulas (ElasticElementaryContribution.nb:1561-1575).

@@ -35,6 +35,12 @@ public:
     // Setup (TPZNumericalIntegrator.cpp:272-536)
     void SetElementIndexes(TPZVec<int> element_indexes) { fElementIndex = element_indexes; }
     void SetUpIrregularBlocksData(TPZCompMesh* cmesh);
+    // The two halves of SetUpIrregularBlocksData: the blocks' sizes and
+    // prefix sums, and their values on the host (a no-op once filled).  The
+    // device setup (TransferDataToGPU) needs only the sizes; the host values
+    // are filled on first use (export, ComputeTangentMatrix).
+    void SetUpIrregularBlocksSizes(TPZCompMesh* cmesh);
+    void FillHostBlocks();
     int StressRateVectorSize(int dim);
     void SetUpIndexes(TPZCompMesh* cmesh);
     void SetUpColoredIndexes(TPZCompMesh* cmesh);
@@ -42,8 +48,10 @@ public:
     int64_t me(TPZVec<int64_t>& IA, TPZVec<int64_t>& JA, int64_t& i_dest, int64_t& j_dest);
     bool isBuilt() { return fBlockMatrix.Rows() != 0; }
 
-    // Creates the device context (if needed) and uploads blocks, indexes,
-    // weights and material (TPZNumericalIntegrator.cpp:218-233).
+    // Creates the device context (if needed) and sets up the data there
+    // (TPZNumericalIntegrator.cpp:218-233): element corners, shape table,
+    // DoF and color indexes go up; B and the weights are computed on the GPU
+    // (ipf_transfer_geometry) with the formulas of the host setup.
     void TransferDataToGPU(int device);
     std::shared_ptr<ipf_ctx> Context() { return fCtx; }
 

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/intpointsfem.h"
+#include "ipf_geometry.h"
 #include "ipf_kernels.h"
 #include "ipf_learned.h"
 
@@ -146,7 +147,8 @@ struct Group {
     int64_t pt0 = 0;
     int64_t b_off = 0;
     int64_t dof_off = 0;
-    int64_t f_off[FMT_COUNT] = {0, 0, 0, 0};  // first value of the group in each B format
+    int64_t slot0 = 0;                         // first element slot of the group (geometry rows)
+    int64_t f_off[FMT_COUNT] = {0, 0, 0, 0, 0};  // first value of the group in each stored B format
     int64_t l_off = 0;                         // first CSR position index of the group (tangent pattern)
 };
 
@@ -212,8 +214,14 @@ struct ipf_ctx {
     double* dScal = nullptr;    // CG scalars
     bool have_mask = false;
 
-    // alternative B formats (index FMT_*; FMT_D64 is dB)
-    void* dBf[FMT_COUNT] = {nullptr, nullptr, nullptr, nullptr};
+    // alternative B formats (index FMT_*; FMT_D64 is dB, FMT_G64 stores nothing)
+    void* dBf[FMT_COUNT] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // geometry of ipf_transfer_geometry: corners per element slot (8 doubles,
+    // group slot order) and the shape tables (offset per shape, -1 if absent)
+    double* dGeo = nullptr;
+    double* dTab = nullptr;
+    int64_t tab_off[SHAPE_COUNT] = {-1, -1, -1};
+    bool have_geometry = false;
     int rfmt = FMT_D64;  // B format of the fused residual
     int tfmt = FMT_D64;  // B format of the tangent product
     bool compact_ok = false;  // every block has the 2D elasticity pattern
@@ -288,6 +296,10 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dPartial);
     dev_free(ctx->dScal);
     for (int f = 1; f < FMT_COUNT; f++) dev_free(ctx->dBf[f]);
+    dev_free(ctx->dGeo);
+    dev_free(ctx->dTab);
+    for (auto& o : ctx->tab_off) o = -1;
+    ctx->have_geometry = false;
     dev_free(ctx->dLidx);
     dev_free(ctx->dTmeta);
     dev_free(ctx->dNewOf);
@@ -358,6 +370,7 @@ GroupDev group_dev(const ipf_ctx* ctx, const Group& g, bool columns = false) {
     d.mats = ctx->dMats;
     d.ntiles = g.ntiles;
     d.pt0 = g.pt0;
+    d.tab = (ctx->dTab && ctx->tab_off[g.shape] >= 0) ? ctx->dTab + ctx->tab_off[g.shape] : nullptr;
     return d;
 }
 
@@ -375,6 +388,7 @@ PointState point_state(ipf_ctx* ctx) {
 }
 
 const void* group_fmt_B(const ipf_ctx* ctx, const Group& g, int fmt) {
+    if (fmt == FMT_G64) return ctx->dGeo + 8 * g.slot0;
     return static_cast<const char*>(ctx->dBf[fmt]) + format_bytes(fmt) * (size_t)g.f_off[fmt];
 }
 
@@ -414,6 +428,10 @@ int launch_status(ipf_ctx* ctx, const char* what) {
 // Makes format `fmt` resident (converted on the device from the dense FP64 tiles).
 int ensure_format(ipf_ctx* ctx, int fmt) {
     if (fmt < 0 || fmt >= FMT_COUNT) return fail(ctx, IPF_EINVAL, "ipf_set_b_format: unknown format %d", fmt);
+    if (fmt == FMT_G64)
+        return ctx->have_geometry
+                   ? IPF_OK
+                   : fail(ctx, IPF_EINVAL, "geometric B needs the element geometry (ipf_transfer_geometry)");
     if (format_compact(fmt) && !ctx->compact_ok)
         return fail(ctx, IPF_EINVAL, "compact B needs every block in the 2D elasticity pattern; this data is not");
     if (fmt == FMT_D64 || ctx->dBf[fmt]) return IPF_OK;
@@ -505,17 +523,20 @@ int ipf_info(const ipf_ctx* ctx, int64_t out[8]) {
     return IPF_OK;
 }
 
-int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const int* col_sizes,
-                      const int64_t* matrix_position, const int* row_first, const int* col_first,
-                      const double* storage, const int* dof_indexes, const int* color_indexes, int64_t ncolor,
-                      int64_t neq, const double* weight) {
-    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
-    if (nblocks < 0 || ncolor < 1 || neq < 1)
-        return fail(ctx, IPF_EINVAL, "bad sizes: nblocks=%lld ncolor=%lld neq=%lld", (long long)nblocks,
-                    (long long)ncolor, (long long)neq);
-    if (nblocks > 0 && (!row_sizes || !col_sizes || !matrix_position || !row_first || !col_first || !storage ||
-                        !dof_indexes || !color_indexes || !weight))
-        return fail(ctx, IPF_EINVAL, "ipf_transfer_data: null array");
+// Element geometry of ipf_transfer_geometry: corners [8 * nblocks] and one
+// host shape table per kernel shape (null where no element has that shape).
+struct GeoInput {
+    const double* corners = nullptr;
+    const double* tables[SHAPE_COUNT] = {nullptr, nullptr, nullptr};
+};
+
+// The common part of ipf_transfer_data (storage and weight given, geo null)
+// and ipf_transfer_geometry (storage and weight null, geo given: B and the
+// weights are computed on the device).
+static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const int* col_sizes,
+                         const int64_t* matrix_position, const int* row_first, const int* col_first,
+                         const double* storage, const int* dof_indexes, const int* color_indexes, int64_t ncolor,
+                         int64_t neq, const double* weight, const GeoInput* geo) {
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     release_data(ctx);
@@ -641,9 +662,10 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
     // --- compact B possible? Every point of every block must have the 2D
     // elasticity pattern B(3ip + c + j, 2a + c) = dN_a/dx_j
     // (TPZNumericalIntegrator.cpp:371-381): row xx has zeros in the y columns,
-    // row yy zeros in the x columns, and row xy repeats the other two.
+    // row yy zeros in the x columns, and row xy repeats the other two.  Blocks
+    // built from geometry have it by construction.
     bool compact_ok = true;
-    for (int64_t e = 0; e < nblocks && compact_ok; e++) {
+    for (int64_t e = 0; storage && e < nblocks && compact_ok; e++) {
         const int C = col_sizes[e], nip = row_sizes[e] / 3;
         const double* blk = storage + matrix_position[e];
         for (int ip = 0; ip < nip && compact_ok; ip++) {
@@ -665,8 +687,15 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
     std::vector<int> hDof, hCol;
     std::vector<double> hW(npad, 0.0);
     std::vector<int64_t> hOrig(npad, -1);
+    std::vector<double> hGeo;
+    int64_t nslots = 0;
+    for (auto& g : groups) {
+        g.slot0 = nslots;
+        nslots += (int64_t)g.elems.size();
+    }
     try {
-        hB.assign(nb, 0.0);
+        if (storage) hB.assign(nb, 0.0);
+        if (geo) hGeo.assign(8 * nslots, 0.0);
         hDof.resize(nd);
         hCol.resize(nd);
     } catch (const std::bad_alloc&) {
@@ -678,22 +707,24 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
         const int64_t ne = (int64_t)g.elems.size();
         for (int64_t k = 0; k < ne; k++) {
             const int64_t e = g.elems[k];
-            if (e < 0) {  // padding slot: zero B, DoF 0 (never read), no point
+            if (e < 0) {  // padding slot: zero B and corners, DoF 0 (never read), no point
                 for (int j = 0; j < CE; j++) hDof[g.dof_off + k * CE + j] = hCol[g.dof_off + k * CE + j] = 0;
                 continue;
             }
             std::memcpy(&hDof[g.dof_off + k * CE], &dof_indexes[col_first[e]], sizeof(int) * CE);
             for (int j = 0; j < CE; j++) hCol[g.dof_off + k * CE + j] = col_first[e] + j;
+            if (geo) std::memcpy(&hGeo[8 * (g.slot0 + k)], geo->corners + 8 * e, sizeof(double) * 8);
             const int64_t t = k / ET;
             const int el = (int)(k % ET);
-            double* tileB = hB.data() + g.b_off + t * (int64_t)NB * TP;
-            const double* src = storage + matrix_position[e];
             for (int ip = 0; ip < NIP; ip++) {
                 const int q = el * NIP + ip;
                 const int64_t P = g.pt0 + t * TP + q;
                 const int64_t o = row_first[e] / 3 + ip;
                 hOrig[P] = o;
+                if (!storage) continue;
                 hW[P] = weight[o];
+                double* tileB = hB.data() + g.b_off + t * (int64_t)NB * TP;
+                const double* src = storage + matrix_position[e];
                 for (int m = 0; m < NB; m++) {
                     // m = k*CE + j <-> row 3*ip + k, column j of the row-major block
                     tileB[((int64_t)(m >> 1) * TP + q) * 2 + (m & 1)] = src[(int64_t)(3 * ip) * CE + m];
@@ -717,6 +748,18 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
         release_data(ctx);
         return rc;
     }
+    int64_t ntab = 0;
+    if (geo) {
+        for (int sh = 0; sh < SHAPE_COUNT; sh++) {
+            if (!geo->tables[sh]) continue;
+            ctx->tab_off[sh] = ntab;
+            ntab += (shape_table_size(shape_nip(sh), shape_ce(sh) / 2) + 31) / 32 * 32;  // 256-byte aligned rows
+        }
+        if ((rc = dev_alloc(ctx, &ctx->dGeo, 8 * nslots)) || (rc = dev_alloc(ctx, &ctx->dTab, ntab))) {
+            release_data(ctx);
+            return rc;
+        }
+    }
     auto up = [&](void* dst, const void* src, size_t bytes) -> int {
         if (bytes == 0) return IPF_OK;
         IPF_HIP(ctx, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
@@ -727,9 +770,9 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
         IPF_HIP(ctx, hipMemset(dst, 0, bytes));
         return IPF_OK;
     };
-    if ((rc = up(ctx->dB, hB.data(), sizeof(double) * nb)) || (rc = up(ctx->dDof, hDof.data(), sizeof(int) * nd)) ||
-        (rc = up(ctx->dCol, hCol.data(), sizeof(int) * nd)) ||
-        (rc = up(ctx->dW, hW.data(), sizeof(double) * npad)) ||
+    if ((storage && (rc = up(ctx->dB, hB.data(), sizeof(double) * nb))) ||
+        (rc = up(ctx->dDof, hDof.data(), sizeof(int) * nd)) || (rc = up(ctx->dCol, hCol.data(), sizeof(int) * nd)) ||
+        (storage && (rc = up(ctx->dW, hW.data(), sizeof(double) * npad))) ||
         (rc = up(ctx->dOrig, hOrig.data(), sizeof(int64_t) * npad)) ||
         (rc = zero(ctx->dEp[0], sizeof(double) * npad)) || (rc = zero(ctx->dEp[1], sizeof(double) * npad)) ||
         (rc = zero(ctx->dEp[2], sizeof(double) * npad)) || (rc = zero(ctx->dEp[3], sizeof(double) * npad)) ||
@@ -771,14 +814,102 @@ int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const
     ctx->groups = std::move(groups);
     ctx->compact_ok = compact_ok;
     ctx->rfmt = ctx->tfmt = FMT_D64;
+    if (geo) {
+        // corners and tables up, then B and the weights on the device
+        if ((rc = up(ctx->dGeo, hGeo.data(), sizeof(double) * 8 * nslots))) {
+            release_data(ctx);
+            return rc;
+        }
+        for (int sh = 0; sh < SHAPE_COUNT; sh++)
+            if (ctx->tab_off[sh] >= 0 &&
+                (rc = up(ctx->dTab + ctx->tab_off[sh], geo->tables[sh],
+                         sizeof(double) * shape_table_size(shape_nip(sh), shape_ce(sh) / 2)))) {
+                release_data(ctx);
+                return rc;
+            }
+        for (const auto& g : ctx->groups)
+            launch_build_B(g.shape, group_dev(ctx, g), ctx->dGeo + 8 * g.slot0, ctx->dB + g.b_off, ctx->dW,
+                           ctx->stream);
+        if ((rc = launch_status(ctx, "build_B"))) {
+            release_data(ctx);
+            return rc;
+        }
+        IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->have_geometry = true;
+    }
     ctx->have_data = true;
     // Blocks in the 2D elasticity pattern default to compact FP64: the same
-    // results from a third of the B bytes.
-    if (compact_ok) {
+    // results from a third of the B bytes; with the geometry at hand, to the
+    // geometric form (the same results from no per-point B bytes at all).
+    if (geo) {
+        ctx->rfmt = ctx->tfmt = FMT_G64;
+    } else if (compact_ok) {
         if ((rc = ensure_format(ctx, FMT_C64))) return rc;
         ctx->rfmt = ctx->tfmt = FMT_C64;
     }
     return IPF_OK;
+}
+
+int ipf_transfer_data(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, const int* col_sizes,
+                      const int64_t* matrix_position, const int* row_first, const int* col_first,
+                      const double* storage, const int* dof_indexes, const int* color_indexes, int64_t ncolor,
+                      int64_t neq, const double* weight) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    if (nblocks < 0 || ncolor < 1 || neq < 1)
+        return fail(ctx, IPF_EINVAL, "bad sizes: nblocks=%lld ncolor=%lld neq=%lld", (long long)nblocks,
+                    (long long)ncolor, (long long)neq);
+    if (nblocks > 0 && (!row_sizes || !col_sizes || !matrix_position || !row_first || !col_first || !storage ||
+                        !dof_indexes || !color_indexes || !weight))
+        return fail(ctx, IPF_EINVAL, "ipf_transfer_data: null array");
+    return transfer_impl(ctx, nblocks, row_sizes, col_sizes, matrix_position, row_first, col_first, storage,
+                         dof_indexes, color_indexes, ncolor, neq, weight, nullptr);
+}
+
+int ipf_transfer_geometry(ipf_ctx* ctx, int nshape, const int* shape_sizes, const double* shape_tables,
+                          int64_t nblocks, const int* elem_shape, const double* corners, const int* dof_indexes,
+                          const int* color_indexes, int64_t ncolor, int64_t neq) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    if (nblocks < 0 || ncolor < 1 || neq < 1 || nshape < 1 || nshape > SHAPE_COUNT)
+        return fail(ctx, IPF_EINVAL, "bad sizes: nshape=%d nblocks=%lld ncolor=%lld neq=%lld", nshape,
+                    (long long)nblocks, (long long)ncolor, (long long)neq);
+    if (!shape_sizes || !shape_tables || (nblocks > 0 && (!elem_shape || !corners || !dof_indexes || !color_indexes)))
+        return fail(ctx, IPF_EINVAL, "ipf_transfer_geometry: null array");
+    // shapes -> kernel shapes, tables by kernel shape
+    GeoInput geo;
+    geo.corners = corners;
+    std::vector<int> kshape(nshape);
+    int64_t toff = 0;
+    for (int i = 0; i < nshape; i++) {
+        const int nip = shape_sizes[2 * i], nsh = shape_sizes[2 * i + 1];
+        const int ks = shape_of(nip, 2 * nsh);
+        if (ks < 0)
+            return fail(ctx, IPF_EUNSUPPORTED, "shape %d (n_ip=%d, n_sh=%d) has no kernel (supported: Q1/Q2/Q3)", i,
+                        nip, nsh);
+        if (geo.tables[ks]) return fail(ctx, IPF_EINVAL, "shapes %d and another have the same sizes", i);
+        geo.tables[ks] = shape_tables + toff;
+        kshape[i] = ks;
+        toff += shape_table_size(nip, nsh);
+    }
+    // the blocks' sizes and prefix sums (TPZNumericalIntegrator.cpp:294-317)
+    std::vector<int> rs(nblocks), cs(nblocks), rf(nblocks + 1, 0), cf(nblocks + 1, 0);
+    std::vector<int64_t> mp(nblocks + 1, 0);
+    for (int64_t e = 0; e < nblocks; e++) {
+        const int i = elem_shape[e];
+        if (i < 0 || i >= nshape) return fail(ctx, IPF_EINVAL, "element %lld: shape %d not in [0,%d)", (long long)e, i, nshape);
+        rs[e] = 3 * shape_sizes[2 * i];
+        cs[e] = 2 * shape_sizes[2 * i + 1];
+        if ((int64_t)rf[e] + rs[e] > INT32_MAX || (int64_t)cf[e] + cs[e] > INT32_MAX)
+            return fail(ctx, IPF_EINVAL, "more than 2^31 rows or columns");
+        rf[e + 1] = rf[e] + rs[e];
+        cf[e + 1] = cf[e] + cs[e];
+        mp[e + 1] = mp[e] + (int64_t)rs[e] * cs[e];
+    }
+    for (int64_t e = 0; e < nblocks; e++)
+        for (int k = 0; k < 8; k++)
+            if (!std::isfinite(corners[8 * e + k]))
+                return fail(ctx, IPF_EINVAL, "element %lld: corner coordinate not finite", (long long)e);
+    return transfer_impl(ctx, nblocks, rs.data(), cs.data(), mp.data(), rf.data(), cf.data(), nullptr, dof_indexes,
+                         color_indexes, ncolor, neq, nullptr, &geo);
 }
 
 int ipf_set_material_table(ipf_ctx* ctx, int nmat, const double* params) {

@@ -143,6 +143,30 @@ int ipf_cmesh_point_coords(const ipf_cmesh* m, double* xy) {
     return IPF_OK;
 }
 
+int ipf_cmesh_shape_table(const ipf_cmesh* m, int sizes[2], double* tab) {
+    if (!m || !sizes) return null_arg("ipf_cmesh_shape_table");
+    const TPZCompMesh& c = *m->mesh;
+    sizes[0] = c.NIntPoints();
+    sizes[1] = (int)c.NShapeF();
+    if (tab) {
+        std::vector<double> t;
+        c.ShapeTable(t);
+        std::memcpy(tab, t.data(), sizeof(double) * t.size());
+    }
+    return IPF_OK;
+}
+
+int ipf_cmesh_corners(const ipf_cmesh* m, double* corners) {
+    if (!m || !corners) return null_arg("ipf_cmesh_corners");
+    const TPZCompMesh& c = *m->mesh;
+    for (int64_t e = 0; e < c.NElements(); e++) {
+        double X[4][2];
+        c.CornerCoords(e, X);
+        std::memcpy(corners + 8 * e, &X[0][0], sizeof(double) * 8);
+    }
+    return IPF_OK;
+}
+
 int ipf_cmesh_set_update_mem(ipf_cmesh* m, int update_mem) {
     if (!m) return null_arg("ipf_cmesh_set_update_mem");
     m->mesh->SetUpdateMem(update_mem != 0);
@@ -195,8 +219,13 @@ int ipf_sm_export_blocks(const ipf_sm* sm, int* row_sizes, int* col_sizes, int64
     if (matrix_position) std::memcpy(matrix_position, B.fMatrixPosition.data(), sizeof(int64_t) * (n + 1));
     if (row_first) std::memcpy(row_first, B.fRowFirstIndex.data(), sizeof(int) * (n + 1));
     if (col_first) std::memcpy(col_first, B.fColFirstIndex.data(), sizeof(int) * (n + 1));
-    if (storage) std::memcpy(storage, B.fStorage.data(), sizeof(double) * B.fStorage.size());
-    return IPF_OK;
+    if (!storage) return IPF_OK;
+    return guarded([&] {
+        // the host values are filled on first request (device setups compute B on the GPU)
+        const_cast<ipf_sm*>(sm)->sm->Integrator().FillHostBlocks();
+        std::memcpy(storage, B.fStorage.data(), sizeof(double) * B.fStorage.size());
+        return IPF_OK;
+    });
 }
 
 int ipf_sm_export_indexes(const ipf_sm* sm, int* dof_indexes, int* color_indexes, int* el_color_index,

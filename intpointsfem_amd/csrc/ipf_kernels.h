@@ -33,6 +33,7 @@ struct GroupDev {
     int64_t nelem;         // element slots of the group (ntiles * ET)
     int64_t ntiles;
     int64_t pt0;           // new-point index of the group's first slot
+    const double* tab;     // shape table of the group's shape (ipf_geometry.h layout), or null
 };
 
 __device__ __forceinline__ int tile_nvalid(const GroupDev& g, int64_t tile) { return g.tmeta[tile] & 255; }
@@ -64,10 +65,17 @@ constexpr int shape_tp(int s) { return shape_et(s) * shape_nip(s); }
 //   FMT_D64  dense FP64, every stored value of the reference's block
 //   FMT_C64  compact FP64, (dN_a/dx, dN_a/dy) per node and point
 //   FMT_D32  dense FP32, FMT_C32 compact FP32 (values padded to 4 per point)
-enum BFormat { FMT_D64 = 0, FMT_C64 = 1, FMT_D32 = 2, FMT_C32 = 3, FMT_COUNT = 4 };
+//   FMT_G64  geometric FP64: nothing per point; the kernels recompute the
+//            compact values (and the weight) from the element's 4 corners
+//            (8 doubles per element slot, tile order) and the shape table
+//            (ipf_geometry.h), bit-identical to the stored FP64 forms built
+//            from the same geometry
+enum BFormat { FMT_D64 = 0, FMT_C64 = 1, FMT_D32 = 2, FMT_C32 = 3, FMT_G64 = 4, FMT_COUNT = 5 };
 constexpr int format_bytes(int f) { return (f == FMT_D32 || f == FMT_C32) ? 4 : 8; }
-constexpr bool format_compact(int f) { return f == FMT_C64 || f == FMT_C32; }
-// values stored per integration point (padded to a whole 16-byte vector)
+constexpr bool format_compact(int f) { return f == FMT_C64 || f == FMT_C32 || f == FMT_G64; }
+constexpr bool format_stored(int f) { return f != FMT_G64; }
+// values per integration point (stored formats: padded to a whole 16-byte
+// vector; geometric: the compact values the kernel computes)
 constexpr int format_values(int s, int f) {
     return ((format_compact(f) ? shape_ce(s) : 3 * shape_ce(s)) + (16 / format_bytes(f)) - 1) /
            (16 / format_bytes(f)) * (16 / format_bytes(f));
@@ -103,13 +111,19 @@ void launch_assemble_K(int shape, const GroupDev& g, const double* dep_soa, int6
                        double* kg, hipStream_t stream);
 
 // Format variants (ipf_kernels_fmt.hip): Bv points at the group's first tile
-// in format fmt (FMT_C64, FMT_D32 or FMT_C32).
+// in format fmt (FMT_C64, FMT_D32 or FMT_C32), or for FMT_G64 at the corners
+// of the group's first element slot (with g.tab set).
 void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
                          double* r, const PointState& st, hipStream_t stream);
 void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, const double* v, double* y,
                         const double* dep_soa, int64_t npad, hipStream_t stream);
 // dense FP64 tiles of one group -> format fmt
 void launch_convert_B(int fmt, int shape, const double* dense, void* out, int64_t ntiles, hipStream_t stream);
+// Device setup of one group from geometry (corners: 8 doubles per element
+// slot, tile order; g.tab the shape table): the dense FP64 tiles of B (zeros
+// included) and the weights w[g.pt0 + ...] (0 on padding slots).
+void launch_build_B(int shape, const GroupDev& g, const double* corners, double* dense, double* w,
+                    hipStream_t stream);
 
 // Vector kernels of the preconditioned CG (ipf_solver.hip).  `scal` is a small
 // device array of scalars (see ipf_solver.hip for the slots).

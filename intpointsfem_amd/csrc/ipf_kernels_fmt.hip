@@ -9,9 +9,15 @@
 //            dense FP64 results (up to the sign of a zero).
 //   FMT_D32  dense FP32: the stored block in FP32, products accumulated in FP32.
 //   FMT_C32  compact FP32.
+//   FMT_G64  geometric FP64: the compact FP64 values and the weight computed
+//            in the kernel from the element's corners and the shape table
+//            (ipf_geometry.h) -- 8 doubles per element instead of 2 n_sh
+//            doubles (+ 1 weight) per point, the same bits as the stored
+//            FP64 forms built by the device setup from the same geometry.
 //
 // FP32 formats compute B.u and B^T(w sigma) in FP32 with FP32 accumulation;
 // the return map, the assembly into r and everything else stay FP64.
+#include "ipf_geometry.h"
 #include "ipf_kernels.h"
 
 namespace ipf {
@@ -23,13 +29,15 @@ constexpr int kWave = 64;
 template <int FMT>
 struct FmtTraits;
 template <>
-struct FmtTraits<FMT_D64> { using T = double; static constexpr bool COMPACT = false; static constexpr int VEC = 2; };
+struct FmtTraits<FMT_D64> { using T = double; static constexpr bool COMPACT = false, GEOM = false; static constexpr int VEC = 2; };
 template <>
-struct FmtTraits<FMT_C64> { using T = double; static constexpr bool COMPACT = true; static constexpr int VEC = 2; };
+struct FmtTraits<FMT_C64> { using T = double; static constexpr bool COMPACT = true, GEOM = false; static constexpr int VEC = 2; };
 template <>
-struct FmtTraits<FMT_D32> { using T = float; static constexpr bool COMPACT = false; static constexpr int VEC = 4; };
+struct FmtTraits<FMT_D32> { using T = float; static constexpr bool COMPACT = false, GEOM = false; static constexpr int VEC = 4; };
 template <>
-struct FmtTraits<FMT_C32> { using T = float; static constexpr bool COMPACT = true; static constexpr int VEC = 4; };
+struct FmtTraits<FMT_C32> { using T = float; static constexpr bool COMPACT = true, GEOM = false; static constexpr int VEC = 4; };
+template <>
+struct FmtTraits<FMT_G64> { using T = double; static constexpr bool COMPACT = true, GEOM = true; static constexpr int VEC = 2; };
 
 template <int SHAPE, int FMT>
 struct FC {
@@ -75,6 +83,33 @@ __device__ __forceinline__ void load_values(const void* __restrict__ Bg, int64_t
             b[4 * m + 3] = v.w;
         }
     }
+}
+
+// Geometric format: the compact values (dN_a/dx, dN_a/dy) of the lane's point
+// from its element's corners (Bg: 8 doubles per element slot of the group)
+// and the shape table; returns the weight w_q |det J|.
+template <int SHAPE>
+__device__ __forceinline__ double geom_values(const double* __restrict__ Bg, const double* __restrict__ tab,
+                                              int64_t tile, int lane, double* b) {
+    using C = FC<SHAPE, FMT_G64>;
+    const int el = lane / C::NIP, ip = lane - el * C::NIP;
+    const dv2* Xv = reinterpret_cast<const dv2*>(Bg) + (tile * C::ET + el) * 4;
+    double X[8];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const dv2 v = ld_stream(&Xv[c]);
+        X[2 * c] = v.x;
+        X[2 * c + 1] = v.y;
+    }
+    const double* xi = tab + C::NIP * C::CE;
+    const QuadMap m = quad_map(X, xi[ip], xi[C::NIP + ip]);
+    const dv2* dphi = reinterpret_cast<const dv2*>(tab) + ip * (C::CE / 2);
+#pragma unroll
+    for (int a = 0; a < C::CE / 2; a++) {
+        const dv2 d = dphi[a];
+        quad_grad(m, d.x, d.y, b[2 * a], b[2 * a + 1]);
+    }
+    return quad_weight(m, xi[2 * C::NIP + ip]);
 }
 
 // strain in the accumulation type T.  Dense: rows summed over columns j
@@ -180,7 +215,15 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     const bool slot = lane < C::TP;
     const bool valid = slot && el < tile_nvalid(g, tile);
     T b[C::NV];
-    if (slot) {
+    double wgeo = 0.0;
+    if constexpr (FmtTraits<FMT>::GEOM) {
+        if (valid) {
+            wgeo = geom_values<SHAPE>(static_cast<const double*>(Bv), g.tab, tile, lane, b);
+        } else {
+#pragma unroll
+            for (int k = 0; k < C::NV; k++) b[k] = T(0);
+        }
+    } else if (slot) {
         load_values<SHAPE, FMT>(Bv, tile, lane, b);
     } else {
 #pragma unroll
@@ -208,7 +251,8 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
         // enter the stress); without a commit it is not needed
         double alpha = COMMIT ? ld_stream(&st.alpha[P]) : 0.0;
         int mt, br;
-        point_sigma<false>(e3, ep, alpha, mt, br, tile_material(g, tile), ld_stream(&st.w[P]), ws, nullptr);
+        const double w = FmtTraits<FMT>::GEOM ? wgeo : ld_stream(&st.w[P]);
+        point_sigma<false>(e3, ep, alpha, mt, br, tile_material(g, tile), w, ws, nullptr);
         if (COMMIT) {
             st.ep0[P] = ep[0];
             st.ep1[P] = ep[1];
@@ -247,7 +291,14 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     const bool slot = lane < C::TP;
     const bool valid = slot && el < tile_nvalid(g, tile);
     T b[C::NV];
-    if (slot) {
+    if constexpr (FmtTraits<FMT>::GEOM) {
+        if (valid) {
+            (void)geom_values<SHAPE>(static_cast<const double*>(Bv), g.tab, tile, lane, b);
+        } else {
+#pragma unroll
+            for (int k = 0; k < C::NV; k++) b[k] = T(0);
+        }
+    } else if (slot) {
         load_values<SHAPE, FMT>(Bv, tile, lane, b);
     } else {
 #pragma unroll
@@ -312,6 +363,40 @@ __global__ void k_convert_B(const double* __restrict__ dense, void* __restrict__
     }
 }
 
+// Device setup of one group: dense FP64 tiles of B and the weights from the
+// corners (8 doubles per element slot) and the shape table g.tab.  Every value
+// of a tile is written, zeros and padding slots included.
+template <int SHAPE>
+__global__ void k_build_B(GroupDev g, const double* __restrict__ corners, double* __restrict__ dense,
+                          double* __restrict__ w) {
+    using C = FC<SHAPE, FMT_D64>;
+    const int64_t tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (tile >= g.ntiles || lane >= C::TP) return;
+    const int el = lane / C::NIP;
+    const bool valid = el < tile_nvalid(g, tile);
+    double cb[C::CE];
+    double wt = 0.0;
+    if (valid) {
+        wt = geom_values<SHAPE>(corners, g.tab, tile, lane, cb);
+    } else {
+#pragma unroll
+        for (int k = 0; k < C::CE; k++) cb[k] = 0.0;
+    }
+    double* dst = dense + tile * (int64_t)(3 * C::CE * C::TP);
+    // value m = k * CE + j: row 3 ip + k, column j = 2 a + c of the block
+#pragma unroll
+    for (int m = 0; m < 3 * C::CE; m++) {
+        const int k = m / C::CE, j = m % C::CE, a = j >> 1, c = j & 1;
+        double v = 0.0;
+        if (k == 0 && c == 0) v = cb[2 * a];          // xx: dN_a/dx
+        if (k == 1) v = c ? cb[2 * a] : cb[2 * a + 1];  // xy: dN_a/dy (x column), dN_a/dx (y column)
+        if (k == 2 && c == 1) v = cb[2 * a + 1];      // yy: dN_a/dy
+        dst[((int64_t)(m >> 1) * C::TP + lane) * 2 + (m & 1)] = v;
+    }
+    w[g.pt0 + tile * C::TP + lane] = wt;
+}
+
 template <int SHAPE, int FMT>
 void residual_fmt_shape(bool commit, const GroupDev& g, const void* Bv, const double* u, double* r,
                         const PointState& st, hipStream_t s) {
@@ -363,6 +448,7 @@ void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, con
     if (g.ntiles == 0) return;
     switch (fmt) {
         case FMT_C64: residual_fmt<FMT_C64>(shape, commit, g, Bv, u, r, st, s); break;
+        case FMT_G64: residual_fmt<FMT_G64>(shape, commit, g, Bv, u, r, st, s); break;
         case FMT_D32: residual_fmt<FMT_D32>(shape, commit, g, Bv, u, r, st, s); break;
         case FMT_C32: residual_fmt<FMT_C32>(shape, commit, g, Bv, u, r, st, s); break;
         default: break;
@@ -374,6 +460,7 @@ void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, c
     if (g.ntiles == 0) return;
     switch (fmt) {
         case FMT_C64: tangent_fmt<FMT_C64>(shape, g, Bv, v, y, dep, npad, s); break;
+        case FMT_G64: tangent_fmt<FMT_G64>(shape, g, Bv, v, y, dep, npad, s); break;
         case FMT_D32: tangent_fmt<FMT_D32>(shape, g, Bv, v, y, dep, npad, s); break;
         case FMT_C32: tangent_fmt<FMT_C32>(shape, g, Bv, v, y, dep, npad, s); break;
         default: break;
@@ -386,6 +473,17 @@ void launch_convert_B(int fmt, int shape, const double* dense, void* out, int64_
         case FMT_C64: convert_fmt<FMT_C64>(shape, dense, out, ntiles, s); break;
         case FMT_D32: convert_fmt<FMT_D32>(shape, dense, out, ntiles, s); break;
         case FMT_C32: convert_fmt<FMT_C32>(shape, dense, out, ntiles, s); break;
+        default: break;
+    }
+}
+
+void launch_build_B(int shape, const GroupDev& g, const double* corners, double* dense, double* w, hipStream_t s) {
+    if (g.ntiles == 0) return;
+    const dim3 grid((unsigned)g.ntiles), block(kWave);
+    switch (shape) {
+        case SHAPE_Q1: hipLaunchKernelGGL((k_build_B<SHAPE_Q1>), grid, block, 0, s, g, corners, dense, w); break;
+        case SHAPE_Q2: hipLaunchKernelGGL((k_build_B<SHAPE_Q2>), grid, block, 0, s, g, corners, dense, w); break;
+        case SHAPE_Q3: hipLaunchKernelGGL((k_build_B<SHAPE_Q3>), grid, block, 0, s, g, corners, dense, w); break;
         default: break;
     }
 }

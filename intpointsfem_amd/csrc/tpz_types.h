@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -15,6 +17,27 @@
 using REAL = double;
 using STATE = double;
 
+// Allocator whose value-less construct() default-initialises: resize(n) of a
+// vector of doubles leaves the new elements unwritten, so a large buffer is
+// first touched by the (parallel) code that fills it, not by a serial zero
+// fill.
+template <class T, class A = std::allocator<T>>
+struct DefaultInitAllocator : A {
+    template <class U>
+    struct rebind {
+        using other = DefaultInitAllocator<U, typename std::allocator_traits<A>::template rebind_alloc<U>>;
+    };
+    using A::A;
+    template <class U>
+    void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... Args>
+    void construct(U* p, Args&&... args) {
+        std::allocator_traits<A>::construct(static_cast<A&>(*this), p, std::forward<Args>(args)...);
+    }
+};
+
 template <class T>
 class TPZVec {
 public:
@@ -22,6 +45,8 @@ public:
     explicit TPZVec(int64_t n, const T& v = T()) : fData(n, v) {}
     int64_t size() const { return (int64_t)fData.size(); }
     int64_t NElements() const { return size(); }
+    // resize: new elements default-initialised (unwritten for arithmetic T);
+    // Resize: new elements set to v
     void resize(int64_t n) { fData.resize(n); }
     void Resize(int64_t n, const T& v = T()) { fData.resize(n, v); }
     void Fill(const T& v) { std::fill(fData.begin(), fData.end(), v); }
@@ -36,7 +61,7 @@ public:
     void push_back(const T& v) { fData.push_back(v); }
 
 private:
-    std::vector<T> fData;
+    std::vector<T, DefaultInitAllocator<T>> fData;
 };
 
 // Column-major dense matrix (NeoPZ's TPZFMatrix storage order).

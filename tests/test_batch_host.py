@@ -50,3 +50,20 @@ def test_concatenation_is_a_valid_data_set():
         assert len(np.unique(dc)) == len(dc)
     assert h["material_table"].shape == (len(meshes), 5)
     assert nb == sum(i["N"] ** 2 for i in SMALL)
+
+
+def test_geometry_concatenation_matches_blocks():
+    """The device-setup input of a batch (corners, shape tables) describes the
+    same data set as the host-built one: equal sizes, DoF / color indexes,
+    f_lin and materials; one table per element order."""
+    meshes = [batch.instance_mesh(i) for i in SMALL]
+    h = batch.concat_instances(meshes)
+    g = batch.concat_geometry(meshes)
+    for k in ("row_sizes", "col_sizes", "dof_indexes", "color_indexes", "rhs_linear", "element_material",
+              "material_table", "eq_offsets", "pt_offsets", "block_offsets"):
+        assert np.array_equal(h[k], g[k]), k
+    assert (h["ncolor"], h["neq"]) == (g["ncolor"], g["neq"])
+    assert sorted(map(tuple, g["shape_sizes"].tolist())) == [(4, 4), (9, 9), (16, 16)]
+    assert g["shape_tables"].size == sum(n * (2 * m + 3) for n, m in g["shape_sizes"])
+    assert g["corners"].shape == (len(h["row_sizes"]), 8)
+    assert np.array_equal(3 * g["shape_sizes"][g["elem_shape"], 0], h["row_sizes"])

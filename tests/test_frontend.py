@@ -218,3 +218,42 @@ def test_abi_errors_are_reported():
     m = ipf.CompMesh(2, 2, 1)
     with pytest.raises(ipf.IPFError, match="E / nu"):
         m.set_material(-1.0, 0.2, 1, 0.5, 0.5)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_blocks_from_corners_and_shape_table(p):
+    """The host blocks are the geometry formula applied to the mesh's corners
+    and shape table (the input of the device setup): per point J from the
+    bilinear map, inv = 1/det, (a, b, c, d) = (ye, -yx, -xe, xx) * inv,
+    dN/dx = a dN/dxi + b dN/deta, dN/dy = c dN/dxi + d dN/deta -- restated
+    here in numpy in the same operation order, so the bits must agree; the
+    weights are w_q |det|."""
+    m = ipf.CompMesh(4, 3, p, 0.1, 4.0, 1.2)
+    m.set_material(5000, 0.2, 10, 0.5, 0.5)
+    sm, b, ix = host_sm(m)
+    (nip, nsh), tab = m.shape_table()
+    X = m.corners()
+    dphi = tab[:nip * nsh * 2].reshape(nip, nsh, 2)
+    xi, eta, wq = tab[nip * nsh * 2:].reshape(3, nip)
+    C = 2 * nsh
+    for e in range(X.shape[0]):
+        blk = b["storage"][b["matrix_position"][e]:b["matrix_position"][e + 1]].reshape(3 * nip, C)
+        for ip in range(nip):
+            x, y = X[e, 0::2], X[e, 1::2]
+            dxi = np.array([-(1 - eta[ip]) / 4, (1 - eta[ip]) / 4, (1 + eta[ip]) / 4, -(1 + eta[ip]) / 4])
+            det_ = np.array([-(1 - xi[ip]) / 4, -(1 + xi[ip]) / 4, (1 + xi[ip]) / 4, (1 - xi[ip]) / 4])
+            xx = xe = yx = ye = 0.0
+            for c in range(4):
+                xx += dxi[c] * x[c]
+                xe += det_[c] * x[c]
+                yx += dxi[c] * y[c]
+                ye += det_[c] * y[c]
+            det = xx * ye - xe * yx
+            inv = 1.0 / det
+            A, Bc, Cc, D = ye * inv, -(yx * inv), -(xe * inv), xx * inv
+            dx = A * dphi[ip, :, 0] + Bc * dphi[ip, :, 1]
+            dy = Cc * dphi[ip, :, 0] + D * dphi[ip, :, 1]
+            assert np.array_equal(blk[3 * ip, 0::2], dx) and np.array_equal(blk[3 * ip + 2, 1::2], dy)
+            assert np.array_equal(blk[3 * ip + 1, 0::2], dy) and np.array_equal(blk[3 * ip + 1, 1::2], dx)
+            assert not blk[3 * ip, 1::2].any() and not blk[3 * ip + 2, 0::2].any()
+            assert ix["weight"][b["row_first"][e] // 3 + ip] == wq[ip] * abs(det)

@@ -174,8 +174,16 @@ def test_compact_rejected_for_general_blocks(gpu):
     ctx.synchronize()
 
 
-def test_compact_fp64_is_the_default(gpu):
+def test_default_formats(gpu):
+    """Device setup from geometry: the geometric form; host blocks in the 2D
+    pattern: compact FP64."""
     m, sm, ctx, B, ix = _setup((4, 3, 2, 1.2))
-    assert ctx.b_format() == ("compact-fp64", "compact-fp64")
+    assert ctx.b_format() == ("geometry-fp64", "geometry-fp64")
     ctx.set_b_format("dense-fp64", tangent="compact-fp32")
     assert ctx.b_format() == ("dense-fp64", "compact-fp32")
+    b = sm.export_blocks()
+    s = sm.sizes()
+    ch = ipf.DeviceContext(device=0)
+    ch.transfer_data(b["row_sizes"], b["col_sizes"], b["matrix_position"], b["row_first"], b["col_first"],
+                     b["storage"], ix["dof_indexes"], ix["color_indexes"], s["ncolor"], s["neq"], ix["weight"])
+    assert ch.b_format() == ("compact-fp64", "compact-fp64")

@@ -1,8 +1,9 @@
 """The production sizes (800 x 800 Q2: 5.76 M points, 5.1 M DoF; 800 x 800 Q3:
 10.24 M points, 11.5 M DoF) through properties that do not need the oracle:
 
-  * compact FP64 B and dense FP64 B give bit-identical residuals, branch
-    flags and plastic strains (the compact products only skip exact zeros);
+  * compact FP64 B, dense FP64 B and the geometric format (B recomputed from
+    the element corners) give bit-identical residuals, branch flags and
+    plastic strains (the compact products only skip exact zeros);
   * the fused residual equals the reference's separate steps
     (Multiply -> ComputeSigma -> MultiplyTranspose + f_lin) to 1e-12 of
     max |f_int| (same per-element sums, same color order);
@@ -48,6 +49,12 @@ def test_production_mesh_properties(gpu, mesh):
     assert (r_c == r_d).all()
     assert np.array_equal(st_c[2], st_d[2]) and np.array_equal(st_c[3], st_d[3])
     assert (st_c[0] == st_d[0]).all() and (st_c[1] == st_d[1]).all()
+    ctx.set_state(np.zeros(6 * npts))
+    r_g = residual("geometry-fp64", True)
+    st_g = ctx.get_state()
+    assert (r_c == r_g).all()
+    assert all(np.array_equal(a, b) for a, b in zip(st_c, st_g))
+    ctx.set_b_format("compact-fp64")
     plastic = int((st_c[2] != 0).sum())
     assert plastic > 0
     # the reference's separate steps from the zero state
