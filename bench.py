@@ -63,18 +63,47 @@ def algorithmic_bytes(blocks, npts, neq, fmt="dense-fp64"):
     return int((word * nval + 4 * C).sum()) + npts * (8 + 32) + neq * (8 + 8)
 
 
-def time_residuals(ctx, u, r, stream, steps, warmup):
+_ROCTX = None
+
+
+def roctx_range(name=None):
+    """Pushes (name given) or pops a roctx range, which rocprofv3
+    --marker-trace records: the timed region can then be selected from a
+    kernel trace of the whole run.  A no-op where the roctx library is absent."""
+    global _ROCTX
+    if _ROCTX is None:
+        import ctypes
+        _ROCTX = False
+        for lib in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1"):
+            try:
+                _ROCTX = ctypes.CDLL(lib)
+                break
+            except OSError:
+                pass
+    if not _ROCTX:
+        return
+    if name is None:
+        _ROCTX.roctxRangePop()
+    else:
+        _ROCTX.roctxRangePushA(name.encode())
+
+
+def time_residuals(ctx, u, r, stream, steps, warmup, roi=None):
     import torch
     for _ in range(warmup):
         ctx.residual(u, r)
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
+    if roi:
+        roctx_range(roi)
     start.record(stream)
     for _ in range(steps):
         ctx.residual(u, r)
     end.record(stream)
     torch.cuda.synchronize()
+    if roi:
+        roctx_range()
     return start.elapsed_time(end)
 
 
@@ -86,17 +115,27 @@ def kernel_name(fmt, shape):
     return f"void k_residual_fmt<{shape}, {ipf.B_FORMATS.index(fmt)}, false>"
 
 
-def launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt):
+def launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt, step_ms=None):
     """Per-launch HBM rates of the color kernels: each launch is credited with
     its points' share of the plain FP64 algorithmic bytes (dense B) and of the
-    bytes the format actually stores."""
+    bytes the format actually stores.  Events around every launch add the
+    dispatch gap that back-to-back launches hide; with `step_ms` (the
+    back-to-back residual time) that gap, (sum of the per-launch times -
+    step_ms) / launches, is taken off each launch, so the durations are those
+    of the timed region."""
     avg, pts = launch_times(ctx, u, r, nrep)
+    gap = 0.0
+    if step_ms is not None:
+        gap = max(0.0, (float(avg.sum()) - step_ms) / len(avg))
+    isolated = float(avg[1:].mean())
+    avg = avg - gap
     color_ms = avg[1:]
     share = pts[1:] / npts
     alg = algorithmic_bytes(blocks, npts, neq) * share
     sto = algorithmic_bytes(blocks, npts, neq, fmt) * share
     t = color_ms.sum() / 1e3
     return {"copy_ms": float(avg[0]), "color_ms": color_ms, "avg_launch_ms": float(color_ms.mean()),
+            "avg_launch_ms_isolated": isolated, "dispatch_gap_ms": gap,
             "bytes_per_launch": float(alg.mean()), "stored_bytes_per_launch": float(sto.mean()),
             "stored_bytes": algorithmic_bytes(blocks, npts, neq, fmt),
             "achieved": float(alg.sum() / t / 1e9), "achieved_stored": float(sto.sum() / t / 1e9)}
@@ -220,7 +259,7 @@ def run_batch(args):
     log(f"[bench] {args.steps} batched residuals ({fmt0} B): {ms_per_step:.4f} ms, {value:.1f} Mint-pts/s")
     blocks = {"row_sizes": B.row_sizes, "col_sizes": B.col_sizes}
     nrep = max(3, min(args.steps, 20))
-    roof = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt0)
+    roof = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt0, step_ms=ms_per_step)
     log(f"[bench] launches {', '.join(f'{x:.4f}' for x in roof['color_ms'])} ms; {roof['achieved']:.0f} GB/s on "
         f"plain FP64 bytes, {roof['achieved_stored']:.0f} GB/s on stored bytes")
     roofline = {"bound": "hbm", "achieved": round(roof["achieved_stored"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -284,6 +323,11 @@ def main():
     ap.add_argument("--cg-rtol", type=float, default=1e-1,
                     help="inner Jacobi-PCG tolerance of each Newton step (the Newton loop converges linearly "
                          "with the reference's tangent, so a loose inner solve costs few extra Newton steps)")
+    ap.add_argument("--save-state", default=None, metavar="FILE",
+                    help="--state newton: write the converged state (u, plastic strain, alpha) to FILE (.npz)")
+    ap.add_argument("--load-state", default=None, metavar="FILE",
+                    help="--state newton: start from a state written by --save-state for the same mesh instead of "
+                         "solving (profiler passes over the same state)")
     ap.add_argument("--solve-tangent-format", default=None,
                     help="B format of the CG matvecs of the --state newton solve (default: the library's, "
                          "geometric FP64; the residual stays FP64 whatever this is)")
@@ -329,6 +373,19 @@ def main():
         _, _, mt, br = ctx.get_state()
         ctx.set_state(np.zeros(6 * npts))
         state_desc = "u = linear-elastic thick-cylinder displacement of load step 10, zero plastic strain"
+    elif args.load_state:
+        z = np.load(args.load_state, allow_pickle=False)
+        if str(z["mesh"]) != args.mesh or int(z["neq"]) != neq:
+            raise SystemExit(f"{args.load_state} holds a state of {z['mesh']} (neq {int(z['neq'])}), not {args.mesh}")
+        u_host = z["u"]
+        u = torch.as_tensor(u_host, dtype=torch.float64, device=dev)
+        ctx.set_state(z["ep"], z["alpha"])
+        ctx.synchronize()
+        newton_info = json.loads(str(z["newton_info"]))
+        newton_info["loaded_from"] = os.path.basename(args.load_state)
+        br = z["branch"]
+        state_desc = (f"converged Newton solution of load step 10 (||r||/||r0|| <= {args.newton_rtol:g}), "
+                      "committed plastic strain of the 10 load steps (loaded from --save-state output)")
     else:
         # 10 load steps of Newton + Jacobi-PCG on the GPU; the state is committed after each
         ctx.set_dirichlet_mask(mesh.dirichlet_mask())
@@ -356,13 +413,18 @@ def main():
                        "solve_seconds": round(time.time() - t1, 1)}
         state_desc = (f"converged Newton solution of load step 10 (||r||/||r0|| <= {args.newton_rtol:g}), "
                       "committed plastic strain of the 10 load steps")
+        if args.save_state:
+            ep_s, al_s, _, _ = ctx.get_state()
+            np.savez(args.save_state, mesh=args.mesh, neq=neq, u=u_host, ep=ep_s, alpha=al_s, branch=br,
+                     newton_info=json.dumps(newton_info))
+            log(f"[bench] state saved to {args.save_state}")
     branch_counts = {ipf.BRANCH_NAMES[i]: int((br == i).sum()) for i in range(5)}
     log(f"[bench] branches: {branch_counts}")
     ep_state, al_state, _, _ = ctx.get_state()  # the committed state the timed residuals start from
 
     fmt0 = ctx.b_format()[0]  # the library's choice for these data (geometric FP64 after the device setup)
     shape = info["porder"] - 1
-    total_ms = time_residuals(ctx, u, r, stream, args.steps, args.warmup)
+    total_ms = time_residuals(ctx, u, r, stream, args.steps, args.warmup, roi="timed residuals")
     ms_per_step = total_ms / args.steps
     value = npts * args.steps / (total_ms / 1e3) / 1e6
     log(f"[bench] {args.steps} residuals ({fmt0} B): {ms_per_step:.4f} ms/residual, {value:.1f} Mint-pts/s")
@@ -371,7 +433,7 @@ def main():
     nrep = max(3, min(args.steps, 20))
     blocks = sm.export_blocks()
     nbytes = algorithmic_bytes(blocks, npts, neq)  # plain FP64 byte count: dense B
-    roof = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt0)
+    roof = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt0, step_ms=ms_per_step)
     log(f"[bench] copy kernel {roof['copy_ms']:.4f} ms; color kernels "
         f"{', '.join(f'{x:.4f}' for x in roof['color_ms'])} ms; {nbytes / npts:.1f} algorithmic B/pt -> "
         f"{roof['achieved']:.0f} GB/s; {roof['stored_bytes'] / npts:.1f} stored B/pt -> {roof['achieved_stored']:.0f} GB/s")
@@ -379,20 +441,25 @@ def main():
     # k_copy) for this workload, committed with the profile it comes from
     kname = kernel_name(fmt0, shape)
     traffic = None
+    valu_busy = None
     tfile = os.path.join(ROOT, "profiles", f"k_residual_{args.mesh}_traffic.json")
-    # bytes per launch do not depend on the state (same loads and stores at every point)
+    # bytes per launch hardly depend on the state (same loads and stores at every point)
     if os.path.exists(tfile):
         with open(tfile) as f:
             tk = json.load(f)["kernels"]
         if kname in tk:
-            traffic = int(tk[kname]["total_bytes"])
+            traffic = int(tk[kname]["total_bytes"]) if "total_bytes" in tk[kname] else None
+            valu_busy = tk[kname].get("valu_busy")
     roofline = {"bound": "hbm", "achieved": round(roof["achieved_stored"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": f"{kname} (one launch per element color)",
                 "algorithmic_bytes_per_launch": int(roof["stored_bytes_per_launch"]),
                 "algorithmic_bytes_per_point": round(roof["stored_bytes"] / npts, 1),
                 "avg_launch_ms": round(roof["avg_launch_ms"], 5),
+                "avg_launch_ms_isolated": round(roof["avg_launch_ms_isolated"], 5),
                 "measured_traffic_rate": None if traffic is None else round(traffic / (roof["avg_launch_ms"] / 1e3) / 1e9, 1),
+                # 4 cycles x VALU instructions / (1024 SIMDs x 2.4 GHz x duration), rocprofv3 PMC (profiles/)
+                "valu_busy": None if valu_busy is None else round(valu_busy, 3),
                 "dense_fp64_bytes_rate": round(roof["achieved"], 1)}
 
     # the other B formats (same state, same u): timing, roofline on their own
@@ -411,7 +478,7 @@ def main():
             ctx.residual(u, r)
             ctx.synchronize()
             err = float(np.abs(r.cpu().numpy() - r64).max()) / fint
-            fr = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt)
+            fr = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt, step_ms=tms / args.steps)
             variants[fmt] = {"ms_per_step": round(tms / args.steps, 5),
                              "value": round(npts * args.steps / (tms / 1e3) / 1e6, 2),
                              "kernel": kernel_name(fmt, shape),

@@ -5,7 +5,10 @@ Inputs: one ``--kernel-trace --stats`` run (``<trace>/<prefix>_kernel_stats.csv`
 and two PMC runs, one with FETCH_SIZE and one with WRITE_SIZE
 (``<dir>/<prefix>_counter_collection.csv``).  FETCH_SIZE / WRITE_SIZE (KiB per
 dispatch) are scaled by what they read for ``k_copy``, which reads and writes
-exactly neq doubles.
+exactly neq doubles.  An optional third PMC run (``--sq``) with SQ_INSTS_VALU
+and SQ_WAVES gives the VALU issue load: a wave64 vector instruction holds a
+16-lane SIMD for 4 cycles, so busy = 4 * SQ_INSTS_VALU / (SIMDs * clock *
+average duration) at the peak engine clock (``--simds``, ``--clock-ghz``).
 
   python -m intpointsfem_amd.rocprof_summary --name k_residual_800x800-q2 \\
       --trace gpurun_out/prof_trace --fetch gpurun_out/prof_fetch \\
@@ -31,6 +34,32 @@ def counters(path, cname):
     return {k: sum(v) / len(v) for k, v in agg.items()}
 
 
+def roi_stats(trace_dir, prefix, roi):
+    """Per-kernel statistics (rows shaped like rocprofv3's kernel_stats.csv)
+    of the dispatches that start and end inside the first occurrence of the
+    marker range `roi`; the kernel trace is streamed, not loaded."""
+    with open(os.path.join(trace_dir, f"{prefix}_marker_api_trace.csv")) as f:
+        rng = next(((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in csv.DictReader(f)
+                    if r["Function"] == roi), None)
+    if rng is None:
+        raise SystemExit(f"marker range {roi!r} not found")
+    agg = {}
+    with open(os.path.join(trace_dir, f"{prefix}_kernel_trace.csv")) as f:
+        for r in csv.DictReader(f):
+            t0, t1 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+            if t0 < rng[0] or t1 > rng[1]:
+                continue
+            d = t1 - t0
+            a = agg.setdefault(r["Kernel_Name"], [0, 0, d, d])
+            a[0] += 1
+            a[1] += d
+            a[2] = min(a[2], d)
+            a[3] = max(a[3], d)
+    total = sum(v[1] for v in agg.values()) or 1
+    return [{"Name": k, "Calls": v[0], "AverageNs": v[1] / v[0], "MinNs": v[2], "MaxNs": v[3],
+             "Percentage": 100.0 * v[1] / total} for k, v in sorted(agg.items(), key=lambda kv: -kv[1][1])]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--name", required=True)
@@ -39,15 +68,26 @@ def main():
     ap.add_argument("--write", default=None)
     ap.add_argument("--prefix", default="q2")
     ap.add_argument("--neq", type=int, default=0)
+    ap.add_argument("--roi", default=None,
+                    help="a marker range (rocprofv3 --marker-trace): kernel statistics over the dispatches that run "
+                         "inside it, from the kernel trace, instead of the whole-run statistics")
+    ap.add_argument("--sq", default=None, help="PMC run with SQ_INSTS_VALU and SQ_WAVES (optional)")
+    ap.add_argument("--simds", type=int, default=1024, help="SIMDs of the GPU (MI355X: 256 CUs x 4)")
+    ap.add_argument("--clock-ghz", type=float, default=2.4, help="peak engine clock (MI355X: 2.4 GHz)")
     ap.add_argument("--workload", required=True)
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                                                   "profiles"))
     a = ap.parse_args()
-    with open(os.path.join(a.trace, f"{a.prefix}_kernel_stats.csv")) as f:
-        stats = list(csv.DictReader(f))
-    lines = [f"# {a.name}: rocprofv3 summary", "", f"Workload: {a.workload}", "",
-             "Commands (run from /tmp with TMPDIR=/tmp, repository at $R):", "```"] + a.command + [
+    if a.roi:
+        stats = roi_stats(a.trace, a.prefix, a.roi)
+    else:
+        with open(os.path.join(a.trace, f"{a.prefix}_kernel_stats.csv")) as f:
+            stats = list(csv.DictReader(f))
+    lines = [f"# {a.name}: rocprofv3 summary", "", f"Workload: {a.workload}", ""] + (
+        [f"Kernel statistics over the dispatches inside the marker range \"{a.roi}\" (the timed region).", ""]
+        if a.roi else []) + [
+        "Commands (run from /tmp with TMPDIR=/tmp, repository at $R):", "```"] + a.command + [
         "```", "", "## Kernel statistics (kernel trace)", "",
         "| kernel | calls | average µs | min µs | max µs | share |", "|---|---|---|---|---|---|"]
     for r in stats:
@@ -80,6 +120,24 @@ def main():
             lines.append(f"| `{k}` | {fetch[k]:.0f} | {write[k]:.0f} | {rb / 1e6:.1f} | {wb / 1e6:.1f} | "
                          f"{(rb + wb) / 1e6:.1f} |")
             out["kernels"][k] = {"read_bytes": rb, "write_bytes": wb, "total_bytes": rb + wb}
+    if a.sq:
+        sqf = os.path.join(a.sq, f"{a.prefix}_counter_collection.csv")
+        valu = counters(sqf, "SQ_INSTS_VALU")
+        waves = counters(sqf, "SQ_WAVES")
+        avg_us = {short(r["Name"]): float(r["AverageNs"]) / 1e3 for r in stats}
+        lines += ["", "## VALU issue load (PMC)", "",
+                  f"busy = 4 cycles x SQ_INSTS_VALU / ({a.simds} SIMDs x {a.clock_ghz} GHz x average duration).", "",
+                  "| kernel | SQ_WAVES | SQ_INSTS_VALU | VALU instr / wave | average µs | VALU busy |",
+                  "|---|---|---|---|---|---|"]
+        if out is None:
+            out = {"kernels": {}}
+        for k in valu:
+            if k not in avg_us or k not in waves or valu[k] < 1e5:
+                continue
+            busy = 4.0 * valu[k] / (a.simds * a.clock_ghz * 1e3 * avg_us[k])
+            lines.append(f"| `{k}` | {waves[k]:.0f} | {valu[k]:.4g} | {valu[k] / waves[k]:.0f} | {avg_us[k]:.1f} | "
+                         f"{busy:.2f} |")
+            out["kernels"].setdefault(k, {}).update({"valu_insts": valu[k], "waves": waves[k], "valu_busy": busy})
     with open(os.path.join(a.out, f"{a.name}.md"), "w") as f:
         f.write("\n".join(lines) + "\n")
     if out is not None:
