@@ -24,14 +24,30 @@ typedef struct ipf_sm ipf_sm;
  * Stands in for the gmsh + NeoPZ mesh of main.cpp:447-464, 542-607. */
 int ipf_cmesh_quarter_annulus(int nr, int ntheta, int porder, double r_in, double r_out, double ratio,
                               ipf_cmesh **out);
+/* A Gmsh 4.1 ASCII mesh (TPZGmshReader::GeometricGmshMesh, main.cpp:447-464)
+ * with Lagrange elements of order porder in 1..3 on its triangles and
+ * quadrilaterals (CmeshElastoplasticity, main.cpp:542-607).  Physical groups:
+ * the lines "bc_wellbore" / "bc_farfield" carry the inner / outer normal
+ * tractions (ipf_cmesh_set_normal_stress), the points "fixed_x" / "fixed_y"
+ * fix u_x / u_y (main.cpp:394-411).  Domain elements keep the file order;
+ * DoF numbering: the file's nodes, then edge nodes, then element interiors.
+ * IPF_EINVAL for an unreadable file, a format other than 4.1 ASCII or an
+ * element type other than point, 2-node line, 3-node triangle, 4-node
+ * quadrilateral. */
+int ipf_cmesh_read_gmsh(const char *path, int porder, ipf_cmesh **out);
+/* What the Gmsh file held: out = nodes, elements, points, lines, triangles,
+ * quadrilaterals (the counts TPZGmshReader reports, Results/.../out-1.txt). */
+int ipf_cmesh_gmsh_counts(const ipf_cmesh *m, int64_t out[6]);
 int ipf_cmesh_destroy(ipf_cmesh *m);
 /* Material (TElastoPlasticData + TPZElasticResponse): E, nu, cohesion,
  * friction and dilation angles in radians. */
 int ipf_cmesh_set_material(ipf_cmesh *m, double E, double nu, double cohesion, double phi, double psi);
-/* Normal tractions sigma_n on the inner / outer arcs, t = sigma_n * n_out
- * (BC type 6 of main.cpp:397-405 in perturbation form). */
+/* Normal tractions sigma_n on the inner / outer boundary (arcs of the
+ * quarter annulus; bc_wellbore / bc_farfield of a Gmsh mesh),
+ * t = sigma_n * n_out (BC type 6 of main.cpp:397-405 in perturbation form). */
 int ipf_cmesh_set_normal_stress(ipf_cmesh *m, double inner, double outer);
-/* out[0..7] = nelem, nnodes, neq, porder, nr, ntheta, npts_per_element, nshape */
+/* out[0..7] = nelem, nnodes, neq, porder, nr, ntheta (0 for a Gmsh mesh),
+ * npts_per_element, nshape (-1 when triangles and quadrilaterals mix) */
 int ipf_cmesh_info(const ipf_cmesh *m, int64_t out[8]);
 /* out[0..4] = lambda, mu, phi, psi, cohesion */
 int ipf_cmesh_material(const ipf_cmesh *m, double out[5]);
@@ -43,16 +59,21 @@ int ipf_cmesh_connectivity(const ipf_cmesh *m, int64_t *conn_ptr, int64_t *conn)
  * of the per-point stresses and strains (post-processing, e.g. the plot over
  * y = 0 of RK_Verification). */
 int ipf_cmesh_point_coords(const ipf_cmesh *m, double *xy);
-/* Geometry input of ipf_transfer_geometry for this mesh: sizes = (n_ip,
- * n_sh) and, when tab is not NULL, the shape table (n_ip*(2 n_sh + 3)
+/* Geometry input of ipf_transfer_geometry for this mesh: for element type
+ * `type` (0 quadrilateral, 1 triangle, -1 the mesh's only type) sizes =
+ * (n_ip, n_sh) and, when tab is not NULL, the shape table (n_ip*(2 n_sh + 3)
  * doubles, layout of ipf_transfer_geometry) -- the data NeoPZ's integration
  * rule and shape functions give the reference's setup
- * (TPZNumericalIntegrator.cpp:344-381). */
-int ipf_cmesh_shape_table(const ipf_cmesh *m, int sizes[2], double *tab);
+ * (TPZNumericalIntegrator.cpp:344-381).  Triangles are tabulated in the
+ * reference square of the parallelogram their corners describe. */
+int ipf_cmesh_shape_table(const ipf_cmesh *m, int type, int sizes[2], double *tab);
+/* Element type per element: 0 quadrilateral, 1 triangle. */
+int ipf_cmesh_element_types(const ipf_cmesh *m, signed char *types /* [nelem] */);
 /* The 4 corners of each element's bilinear map, [8 * nelements], element
  * order (gel->Jacobian's input at TPZNumericalIntegrator.cpp:424). */
 int ipf_cmesh_corners(const ipf_cmesh *m, double *corners);
-/* 1 for DoFs fixed by the symmetry conditions (u_y on theta=0, u_x on theta=pi/2) */
+/* 1 for constrained DoFs: the quarter annulus's symmetry conditions (u_y on
+ * theta=0, u_x on theta=pi/2), a Gmsh mesh's fixed_x / fixed_y points */
 int ipf_cmesh_dirichlet_mask(const ipf_cmesh *m, signed char *mask /* [neq] */);
 /* the material's UpdateMem switch (main.cpp:487-522) */
 int ipf_cmesh_set_update_mem(ipf_cmesh *m, int update_mem);

@@ -125,7 +125,10 @@ def lib():
         "ipf_cmesh_dirichlet_mask": (_int, [_vp, _bp]),
         "ipf_cmesh_set_update_mem": (_int, [_vp, _int]),
         "ipf_cmesh_point_coords": (_int, [_vp, _dp]),
-        "ipf_cmesh_shape_table": (_int, [_vp, _ip, _dp]),
+        "ipf_cmesh_shape_table": (_int, [_vp, _int, _ip, _dp]),
+        "ipf_cmesh_element_types": (_int, [_vp, _bp]),
+        "ipf_cmesh_read_gmsh": (_int, [ctypes.c_char_p, _int, ctypes.POINTER(_vp)]),
+        "ipf_cmesh_gmsh_counts": (_int, [_vp, _lp]),
         "ipf_cmesh_corners": (_int, [_vp, _dp]),
         "ipf_sm_create": (_int, [_vp, _int, ctypes.POINTER(_vp)]),
         "ipf_sm_destroy": (_int, [_vp]),
@@ -185,7 +188,9 @@ def _ptr(x):
 
 
 class CompMesh:
-    """Structured quarter annulus with Q_p Lagrange elements (TPZCompMesh stand-in)."""
+    """The TPZCompMesh stand-in: a structured quarter annulus with Q_p
+    Lagrange elements (the constructor) or a Gmsh 4.1 mesh of triangles and
+    quadrilaterals (``CompMesh.from_gmsh``)."""
 
     def __init__(self, nr, ntheta, porder, r_in=0.1, r_out=4.0, ratio=1.0):
         L = lib()
@@ -195,6 +200,20 @@ class CompMesh:
         self._h = h
         self.nr, self.ntheta, self.porder = nr, ntheta, porder
         self.r_in, self.r_out, self.ratio = r_in, r_out, ratio
+
+    @classmethod
+    def from_gmsh(cls, path, porder):
+        """Elements of order porder on the triangles and quadrilaterals of a
+        Gmsh 4.1 ASCII file (ipf_cmesh_read_gmsh)."""
+        h = _vp()
+        _check(lib().ipf_cmesh_read_gmsh(os.fsencode(path), int(porder), ctypes.byref(h)), what="ipf_cmesh_read_gmsh")
+        m = cls.__new__(cls)
+        m._h = h
+        m.nr = m.ntheta = None
+        m.porder = int(porder)
+        m.r_in = m.r_out = m.ratio = None
+        m.path = os.fspath(path)
+        return m
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -246,14 +265,29 @@ class CompMesh:
         _check(lib().ipf_cmesh_connectivity(self._h, _p(ptr, _lp), _p(conn, _lp)), what="ipf_cmesh_connectivity")
         return ptr, conn
 
-    def shape_table(self):
-        """((n_ip, n_sh), table) -- the shape table of ipf_transfer_geometry."""
+    def shape_table(self, etype=-1):
+        """((n_ip, n_sh), table) -- the shape table of ipf_transfer_geometry
+        for element type etype (0 quadrilateral, 1 triangle, -1 the mesh's
+        only type)."""
         sz = np.zeros(2, dtype=np.int32)
-        _check(lib().ipf_cmesh_shape_table(self._h, _p(sz, _ip), None), what="ipf_cmesh_shape_table")
+        _check(lib().ipf_cmesh_shape_table(self._h, int(etype), _p(sz, _ip), None), what="ipf_cmesh_shape_table")
         nip, nsh = int(sz[0]), int(sz[1])
         tab = np.zeros(nip * (2 * nsh + 3))
-        _check(lib().ipf_cmesh_shape_table(self._h, _p(sz, _ip), _p(tab, _dp)), what="ipf_cmesh_shape_table")
+        _check(lib().ipf_cmesh_shape_table(self._h, int(etype), _p(sz, _ip), _p(tab, _dp)),
+               what="ipf_cmesh_shape_table")
         return (nip, nsh), tab
+
+    def element_types(self):
+        """[nelem] int8: 0 quadrilateral, 1 triangle."""
+        t = np.zeros(self.info()["nelem"], dtype=np.int8)
+        _check(lib().ipf_cmesh_element_types(self._h, _p(t, _bp)), what="ipf_cmesh_element_types")
+        return t
+
+    def gmsh_counts(self):
+        """What the Gmsh file held: nodes, elements, points, lines, triangles, quads."""
+        out = np.zeros(6, dtype=np.int64)
+        _check(lib().ipf_cmesh_gmsh_counts(self._h, _p(out, _lp)), what="ipf_cmesh_gmsh_counts")
+        return dict(zip(("nodes", "elements", "points", "lines", "triangles", "quads"), (int(v) for v in out)))
 
     def corners(self):
         """[nelem, 8] corners of each element's bilinear map (x0, y0, ..., x3, y3)."""

@@ -96,12 +96,10 @@ void launch_assemble_K(int shape, const GroupDev& g, const double* dep, int64_t 
                        hipStream_t s) {
     if (g.nelem == 0) return;
     const dim3 grid((unsigned)g.nelem);
-    switch (shape) {
-        case SHAPE_Q1: hipLaunchKernelGGL(k_assemble_K<SHAPE_Q1>, grid, dim3(kWave), 0, s, g, dep, npad, lidx, kg); break;
-        case SHAPE_Q2: hipLaunchKernelGGL(k_assemble_K<SHAPE_Q2>, grid, dim3(kWave), 0, s, g, dep, npad, lidx, kg); break;
-        case SHAPE_Q3: hipLaunchKernelGGL(k_assemble_K<SHAPE_Q3>, grid, dim3(kWave), 0, s, g, dep, npad, lidx, kg); break;
-        default: break;
-    }
+    dispatch_shape(shape, [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        hipLaunchKernelGGL(k_assemble_K<S>, grid, dim3(kWave), 0, s, g, dep, npad, lidx, kg);
+    });
 }
 
 }  // namespace ipf

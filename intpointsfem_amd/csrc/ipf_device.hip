@@ -220,7 +220,7 @@ struct ipf_ctx {
     // group slot order) and the shape tables (offset per shape, -1 if absent)
     double* dGeo = nullptr;
     double* dTab = nullptr;
-    int64_t tab_off[SHAPE_COUNT] = {-1, -1, -1};
+    int64_t tab_off[SHAPE_COUNT] = {-1, -1, -1, -1, -1, -1};
     bool have_geometry = false;
     int rfmt = FMT_D64;  // B format of the fused residual
     int tfmt = FMT_D64;  // B format of the tangent product
@@ -527,7 +527,7 @@ int ipf_info(const ipf_ctx* ctx, int64_t out[8]) {
 // host shape table per kernel shape (null where no element has that shape).
 struct GeoInput {
     const double* corners = nullptr;
-    const double* tables[SHAPE_COUNT] = {nullptr, nullptr, nullptr};
+    const double* tables[SHAPE_COUNT] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 // The common part of ipf_transfer_data (storage and weight given, geo null)
@@ -555,7 +555,7 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
             return fail(ctx, IPF_EINVAL, "block %lld: inconsistent first-index / position prefix sums", (long long)e);
         const int s = shape_of(R / 3, C);
         if (s < 0)
-            return fail(ctx, IPF_EUNSUPPORTED, "block %lld: shape (n_ip=%d, C_e=%d) has no kernel (supported: Q1/Q2/Q3)",
+            return fail(ctx, IPF_EUNSUPPORTED, "block %lld: shape (n_ip=%d, C_e=%d) has no kernel (supported: Q1/Q2/Q3, T1/T2/T3)",
                         (long long)e, R / 3, C);
         elem_shape[e] = s;
     }
@@ -883,7 +883,7 @@ int ipf_transfer_geometry(ipf_ctx* ctx, int nshape, const int* shape_sizes, cons
         const int nip = shape_sizes[2 * i], nsh = shape_sizes[2 * i + 1];
         const int ks = shape_of(nip, 2 * nsh);
         if (ks < 0)
-            return fail(ctx, IPF_EUNSUPPORTED, "shape %d (n_ip=%d, n_sh=%d) has no kernel (supported: Q1/Q2/Q3)", i,
+            return fail(ctx, IPF_EUNSUPPORTED, "shape %d (n_ip=%d, n_sh=%d) has no kernel (supported: Q1/Q2/Q3, T1/T2/T3)", i,
                         nip, nsh);
         if (geo.tables[ks]) return fail(ctx, IPF_EINVAL, "shapes %d and another have the same sizes", i);
         geo.tables[ks] = shape_tables + toff;

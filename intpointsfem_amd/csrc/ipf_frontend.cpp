@@ -48,6 +48,11 @@ int null_arg(const char* fn) {
     return IPF_EINVAL;
 }
 
+int fail_status(int code, const char* msg) {
+    ipf_internal_set_error(msg);
+    return code;
+}
+
 }  // namespace
 
 extern "C" {
@@ -137,22 +142,59 @@ int ipf_cmesh_dirichlet_mask(const ipf_cmesh* m, signed char* mask) {
 int ipf_cmesh_point_coords(const ipf_cmesh* m, double* xy) {
     if (!m || !xy) return null_arg("ipf_cmesh_point_coords");
     const TPZCompMesh& c = *m->mesh;
-    const int nip = c.NIntPoints();
-    for (int64_t e = 0; e < c.NElements(); e++)
-        for (int ip = 0; ip < nip; ip++) c.PointCoords(e, ip, xy[2 * (e * nip + ip)], xy[2 * (e * nip + ip) + 1]);
+    int64_t k = 0;
+    for (int64_t e = 0; e < c.NElements(); e++) {
+        const int nip = c.NIntPoints(c.ElementType(e));
+        for (int ip = 0; ip < nip; ip++, k++) c.PointCoords(e, ip, xy[2 * k], xy[2 * k + 1]);
+    }
     return IPF_OK;
 }
 
-int ipf_cmesh_shape_table(const ipf_cmesh* m, int sizes[2], double* tab) {
+int ipf_cmesh_shape_table(const ipf_cmesh* m, int type, int sizes[2], double* tab) {
     if (!m || !sizes) return null_arg("ipf_cmesh_shape_table");
     const TPZCompMesh& c = *m->mesh;
-    sizes[0] = c.NIntPoints();
-    sizes[1] = (int)c.NShapeF();
-    if (tab) {
-        std::vector<double> t;
-        c.ShapeTable(t);
-        std::memcpy(tab, t.data(), sizeof(double) * t.size());
+    if (type < 0) {
+        if (c.Mixed()) return fail_status(IPF_EINVAL, "ipf_cmesh_shape_table: the mesh mixes element types; name one");
+        type = c.HasType(TPZCompMesh::EQuad) ? 0 : 1;
     }
+    if (type > 1) return fail_status(IPF_EINVAL, "ipf_cmesh_shape_table: type must be -1, 0 or 1");
+    const auto t = (TPZCompMesh::ElType)type;
+    sizes[0] = c.NIntPoints(t);
+    sizes[1] = c.NShapeF(t);
+    if (tab) {
+        std::vector<double> v;
+        c.ShapeTable(t, v);
+        std::memcpy(tab, v.data(), sizeof(double) * v.size());
+    }
+    return IPF_OK;
+}
+
+int ipf_cmesh_element_types(const ipf_cmesh* m, signed char* types) {
+    if (!m || !types) return null_arg("ipf_cmesh_element_types");
+    for (int64_t e = 0; e < m->mesh->NElements(); e++) types[e] = (signed char)m->mesh->ElementType(e);
+    return IPF_OK;
+}
+
+int ipf_cmesh_read_gmsh(const char* path, int porder, ipf_cmesh** out) {
+    if (!path || !out) return null_arg("ipf_cmesh_read_gmsh");
+    *out = nullptr;
+    return guarded([&] {
+        auto m = std::make_unique<ipf_cmesh>();
+        m->mesh = TPZCompMesh::ReadGmsh(path, porder);
+        *out = m.release();
+        return IPF_OK;
+    });
+}
+
+int ipf_cmesh_gmsh_counts(const ipf_cmesh* m, int64_t out[6]) {
+    if (!m || !out) return null_arg("ipf_cmesh_gmsh_counts");
+    const auto& k = m->mesh->Counts();
+    out[0] = k.nodes;
+    out[1] = k.elements;
+    out[2] = k.points;
+    out[3] = k.lines;
+    out[4] = k.triangles;
+    out[5] = k.quads;
     return IPF_OK;
 }
 

@@ -15,6 +15,7 @@
 #pragma once
 
 #include <cstdint>
+#include <type_traits>
 
 #include "ipf_pointmath.h"
 
@@ -53,13 +54,33 @@ struct PointState {
 };
 
 // Supported block shapes (n_ip, C_e): Q1, Q2, Q3 quadrilaterals with
-// (p+1)^2 Gauss points.
-enum Shape { SHAPE_Q1 = 0, SHAPE_Q2 = 1, SHAPE_Q3 = 2, SHAPE_COUNT = 3 };
+// (p+1)^2 Gauss points; T1, T2, T3 triangles (P_p, 3, 6, 10 shape functions)
+// with the 3-, 6- and 12-point rules of degree 2p.
+enum Shape { SHAPE_Q1 = 0, SHAPE_Q2 = 1, SHAPE_Q3 = 2, SHAPE_T1 = 3, SHAPE_T2 = 4, SHAPE_T3 = 5, SHAPE_COUNT = 6 };
 
-constexpr int shape_nip(int s) { return s == SHAPE_Q1 ? 4 : s == SHAPE_Q2 ? 9 : 16; }
-constexpr int shape_ce(int s) { return s == SHAPE_Q1 ? 8 : s == SHAPE_Q2 ? 18 : 32; }
+constexpr int shape_nip(int s) {
+    return s == SHAPE_Q1 ? 4 : s == SHAPE_Q2 ? 9 : s == SHAPE_Q3 ? 16 : s == SHAPE_T1 ? 3 : s == SHAPE_T2 ? 6 : 12;
+}
+constexpr int shape_ce(int s) {
+    return s == SHAPE_Q1 ? 8 : s == SHAPE_Q2 ? 18 : s == SHAPE_Q3 ? 32 : s == SHAPE_T1 ? 6 : s == SHAPE_T2 ? 12 : 20;
+}
 constexpr int shape_et(int s) { return 64 / shape_nip(s); }
 constexpr int shape_tp(int s) { return shape_et(s) * shape_nip(s); }
+
+// Calls f(std::integral_constant<int, S>()) for the runtime shape s: the
+// kernels are templates on the shape.
+template <class F>
+inline void dispatch_shape(int s, F&& f) {
+    switch (s) {
+        case SHAPE_Q1: f(std::integral_constant<int, SHAPE_Q1>()); break;
+        case SHAPE_Q2: f(std::integral_constant<int, SHAPE_Q2>()); break;
+        case SHAPE_Q3: f(std::integral_constant<int, SHAPE_Q3>()); break;
+        case SHAPE_T1: f(std::integral_constant<int, SHAPE_T1>()); break;
+        case SHAPE_T2: f(std::integral_constant<int, SHAPE_T2>()); break;
+        case SHAPE_T3: f(std::integral_constant<int, SHAPE_T3>()); break;
+        default: break;
+    }
+}
 
 // Storage formats of B (same tile order, point-interleaved 16-byte vectors):
 //   FMT_D64  dense FP64, every stored value of the reference's block

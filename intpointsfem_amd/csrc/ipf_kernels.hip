@@ -391,51 +391,24 @@ void residual_shape(bool commit, const GroupDev& g, const double* u, double* r, 
 
 void launch_residual(int shape, bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
                      hipStream_t s) {
-    switch (shape) {
-        case SHAPE_Q1: residual_shape<SHAPE_Q1>(commit, g, u, r, st, s); break;
-        case SHAPE_Q2: residual_shape<SHAPE_Q2>(commit, g, u, r, st, s); break;
-        case SHAPE_Q3: residual_shape<SHAPE_Q3>(commit, g, u, r, st, s); break;
-        default: break;
-    }
+    dispatch_shape(shape, [&](auto sc) { residual_shape<decltype(sc)::value>(commit, g, u, r, st, s); });
 }
 
 void launch_multiply(int shape, const GroupDev& g, const double* u, double* strain, const int64_t* orig,
                      hipStream_t s) {
     if (g.ntiles == 0) return;
-    switch (shape) {
-        case SHAPE_Q1:
-            hipLaunchKernelGGL(k_multiply<SHAPE_Q1>, tile_grid<SHAPE_Q1>(g), dim3(Cfg<SHAPE_Q1>::WPB * kWave), 0, s,
-                               g, u, strain, orig);
-            break;
-        case SHAPE_Q2:
-            hipLaunchKernelGGL(k_multiply<SHAPE_Q2>, tile_grid<SHAPE_Q2>(g), dim3(Cfg<SHAPE_Q2>::WPB * kWave), 0, s,
-                               g, u, strain, orig);
-            break;
-        case SHAPE_Q3:
-            hipLaunchKernelGGL(k_multiply<SHAPE_Q3>, tile_grid<SHAPE_Q3>(g), dim3(Cfg<SHAPE_Q3>::WPB * kWave), 0, s,
-                               g, u, strain, orig);
-            break;
-        default: break;
-    }
+    dispatch_shape(shape, [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        hipLaunchKernelGGL(k_multiply<S>, tile_grid<S>(g), dim3(Cfg<S>::WPB * kWave), 0, s, g, u, strain, orig);
+    });
 }
 
 void launch_transpose(int shape, const GroupDev& g, const double* x, double* r, const int64_t* orig, hipStream_t s) {
     if (g.ntiles == 0) return;
-    switch (shape) {
-        case SHAPE_Q1:
-            hipLaunchKernelGGL(k_transpose<SHAPE_Q1>, tile_grid<SHAPE_Q1>(g), dim3(Cfg<SHAPE_Q1>::WPB * kWave), 0,
-                               s, g, x, r, orig);
-            break;
-        case SHAPE_Q2:
-            hipLaunchKernelGGL(k_transpose<SHAPE_Q2>, tile_grid<SHAPE_Q2>(g), dim3(Cfg<SHAPE_Q2>::WPB * kWave), 0,
-                               s, g, x, r, orig);
-            break;
-        case SHAPE_Q3:
-            hipLaunchKernelGGL(k_transpose<SHAPE_Q3>, tile_grid<SHAPE_Q3>(g), dim3(Cfg<SHAPE_Q3>::WPB * kWave), 0,
-                               s, g, x, r, orig);
-            break;
-        default: break;
-    }
+    dispatch_shape(shape, [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        hipLaunchKernelGGL(k_transpose<S>, tile_grid<S>(g), dim3(Cfg<S>::WPB * kWave), 0, s, g, x, r, orig);
+    });
 }
 
 void launch_sigma(bool dep, bool commit, int shape, const GroupDev& g, int64_t npad, const double* strain,
@@ -455,41 +428,19 @@ void launch_sigma(bool dep, bool commit, int shape, const GroupDev& g, int64_t n
 void launch_tangent(int shape, const GroupDev& g, const double* v, double* y, const double* dep, int64_t npad,
                     hipStream_t s) {
     if (g.ntiles == 0) return;
-    switch (shape) {
-        case SHAPE_Q1:
-            hipLaunchKernelGGL(k_tangent<SHAPE_Q1>, tile_grid<SHAPE_Q1>(g), dim3(Cfg<SHAPE_Q1>::WPB * kWave), 0, s,
-                               g, v, y, dep, npad);
-            break;
-        case SHAPE_Q2:
-            hipLaunchKernelGGL(k_tangent<SHAPE_Q2>, tile_grid<SHAPE_Q2>(g), dim3(Cfg<SHAPE_Q2>::WPB * kWave), 0, s,
-                               g, v, y, dep, npad);
-            break;
-        case SHAPE_Q3:
-            hipLaunchKernelGGL(k_tangent<SHAPE_Q3>, tile_grid<SHAPE_Q3>(g), dim3(Cfg<SHAPE_Q3>::WPB * kWave), 0, s,
-                               g, v, y, dep, npad);
-            break;
-        default: break;
-    }
+    dispatch_shape(shape, [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        hipLaunchKernelGGL(k_tangent<S>, tile_grid<S>(g), dim3(Cfg<S>::WPB * kWave), 0, s, g, v, y, dep, npad);
+    });
 }
 
 void launch_tangent_diag(int shape, const GroupDev& g, double* diag, const double* dep, int64_t npad,
                          hipStream_t s) {
     if (g.ntiles == 0) return;
-    switch (shape) {
-        case SHAPE_Q1:
-            hipLaunchKernelGGL(k_tangent_diag<SHAPE_Q1>, tile_grid<SHAPE_Q1>(g), dim3(Cfg<SHAPE_Q1>::WPB * kWave),
-                               0, s, g, diag, dep, npad);
-            break;
-        case SHAPE_Q2:
-            hipLaunchKernelGGL(k_tangent_diag<SHAPE_Q2>, tile_grid<SHAPE_Q2>(g), dim3(Cfg<SHAPE_Q2>::WPB * kWave),
-                               0, s, g, diag, dep, npad);
-            break;
-        case SHAPE_Q3:
-            hipLaunchKernelGGL(k_tangent_diag<SHAPE_Q3>, tile_grid<SHAPE_Q3>(g), dim3(Cfg<SHAPE_Q3>::WPB * kWave),
-                               0, s, g, diag, dep, npad);
-            break;
-        default: break;
-    }
+    dispatch_shape(shape, [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        hipLaunchKernelGGL(k_tangent_diag<S>, tile_grid<S>(g), dim3(Cfg<S>::WPB * kWave), 0, s, g, diag, dep, npad);
+    });
 }
 
 void launch_copy(int64_t n, const double* src, double* dst, hipStream_t s) {

@@ -410,35 +410,26 @@ void residual_fmt_shape(bool commit, const GroupDev& g, const void* Bv, const do
 template <int FMT>
 void residual_fmt(int shape, bool commit, const GroupDev& g, const void* Bv, const double* u, double* r,
                   const PointState& st, hipStream_t s) {
-    switch (shape) {
-        case SHAPE_Q1: residual_fmt_shape<SHAPE_Q1, FMT>(commit, g, Bv, u, r, st, s); break;
-        case SHAPE_Q2: residual_fmt_shape<SHAPE_Q2, FMT>(commit, g, Bv, u, r, st, s); break;
-        case SHAPE_Q3: residual_fmt_shape<SHAPE_Q3, FMT>(commit, g, Bv, u, r, st, s); break;
-        default: break;
-    }
+    dispatch_shape(shape, [&](auto sc) { residual_fmt_shape<decltype(sc)::value, FMT>(commit, g, Bv, u, r, st, s); });
 }
 
 template <int FMT>
 void tangent_fmt(int shape, const GroupDev& g, const void* Bv, const double* v, double* y, const double* dep,
                  int64_t npad, hipStream_t s) {
     const dim3 grid((unsigned)g.ntiles), block(kWave);
-    switch (shape) {
-        case SHAPE_Q1: hipLaunchKernelGGL((k_tangent_fmt<SHAPE_Q1, FMT>), grid, block, 0, s, g, Bv, v, y, dep, npad); break;
-        case SHAPE_Q2: hipLaunchKernelGGL((k_tangent_fmt<SHAPE_Q2, FMT>), grid, block, 0, s, g, Bv, v, y, dep, npad); break;
-        case SHAPE_Q3: hipLaunchKernelGGL((k_tangent_fmt<SHAPE_Q3, FMT>), grid, block, 0, s, g, Bv, v, y, dep, npad); break;
-        default: break;
-    }
+    dispatch_shape(shape, [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        hipLaunchKernelGGL((k_tangent_fmt<S, FMT>), grid, block, 0, s, g, Bv, v, y, dep, npad);
+    });
 }
 
 template <int FMT>
 void convert_fmt(int shape, const double* dense, void* out, int64_t ntiles, hipStream_t s) {
     const dim3 grid((unsigned)ntiles), block(kWave);
-    switch (shape) {
-        case SHAPE_Q1: hipLaunchKernelGGL((k_convert_B<SHAPE_Q1, FMT>), grid, block, 0, s, dense, out, ntiles); break;
-        case SHAPE_Q2: hipLaunchKernelGGL((k_convert_B<SHAPE_Q2, FMT>), grid, block, 0, s, dense, out, ntiles); break;
-        case SHAPE_Q3: hipLaunchKernelGGL((k_convert_B<SHAPE_Q3, FMT>), grid, block, 0, s, dense, out, ntiles); break;
-        default: break;
-    }
+    dispatch_shape(shape, [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        hipLaunchKernelGGL((k_convert_B<S, FMT>), grid, block, 0, s, dense, out, ntiles);
+    });
 }
 
 }  // namespace
@@ -480,12 +471,10 @@ void launch_convert_B(int fmt, int shape, const double* dense, void* out, int64_
 void launch_build_B(int shape, const GroupDev& g, const double* corners, double* dense, double* w, hipStream_t s) {
     if (g.ntiles == 0) return;
     const dim3 grid((unsigned)g.ntiles), block(kWave);
-    switch (shape) {
-        case SHAPE_Q1: hipLaunchKernelGGL((k_build_B<SHAPE_Q1>), grid, block, 0, s, g, corners, dense, w); break;
-        case SHAPE_Q2: hipLaunchKernelGGL((k_build_B<SHAPE_Q2>), grid, block, 0, s, g, corners, dense, w); break;
-        case SHAPE_Q3: hipLaunchKernelGGL((k_build_B<SHAPE_Q3>), grid, block, 0, s, g, corners, dense, w); break;
-        default: break;
-    }
+    dispatch_shape(shape, [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        hipLaunchKernelGGL((k_build_B<S>), grid, block, 0, s, g, corners, dense, w);
+    });
 }
 
 }  // namespace ipf
