@@ -254,16 +254,22 @@ class CompMesh:
     def point_coords(self):
         """Integration-point coordinates [npts, 2] in the reference point order."""
         inf = self.info()
-        xy = np.zeros(2 * inf["nelem"] * inf["npts_per_element"])
+        if inf["npts_per_element"] >= 0:
+            npts = inf["nelem"] * inf["npts_per_element"]
+        else:  # triangles and quadrilaterals
+            p = inf["porder"]
+            t = self.element_types()
+            npts = int((t == 1).sum()) * {1: 3, 2: 6, 3: 12}[p] + int((t == 0).sum()) * (p + 1) ** 2
+        xy = np.zeros(2 * npts)
         _check(lib().ipf_cmesh_point_coords(self._h, _p(xy, _dp)), what="ipf_cmesh_point_coords")
         return xy.reshape(-1, 2)
 
     def connectivity(self):
         inf = self.info()
         ptr = np.zeros(inf["nelem"] + 1, dtype=np.int64)
-        conn = np.zeros(inf["nelem"] * inf["nshape"], dtype=np.int64)
+        conn = np.zeros(inf["nelem"] * (inf["porder"] + 1) ** 2, dtype=np.int64)  # the largest element
         _check(lib().ipf_cmesh_connectivity(self._h, _p(ptr, _lp), _p(conn, _lp)), what="ipf_cmesh_connectivity")
-        return ptr, conn
+        return ptr, conn[:ptr[-1]]
 
     def shape_table(self, etype=-1):
         """((n_ip, n_sh), table) -- the shape table of ipf_transfer_geometry
