@@ -303,6 +303,15 @@ int ipf_cg_solve(ipf_ctx *ctx, const double *b_dev, double *x_dev, double rtol, 
  * ||r||, total CG iterations, worst CG relative residual, 0. */
 int ipf_newton(ipf_ctx *ctx, double *u_dev, double rtol, int maxit, double cg_rtol, int cg_maxit, double *stats);
 
+/* The CG iterations of ipf_cg_solve / ipf_newton run as a captured HIP graph
+ * of 10 iterations (the launches between two convergence checks), replayed
+ * on the context's stream: fewer launch gaps, the same kernels in the same
+ * order, hence the same results as launching them one by one (enable = 0).
+ * On by default; a capture the runtime refuses falls back to single
+ * launches.  No reference counterpart (the reference solves with LDL^T,
+ * main.cpp:283-285). */
+int ipf_set_cg_graph(ipf_ctx *ctx, int enable);
+
 /* ---- integration-point state (TPZConstitutiveLawProcessor.h:53-63) ---- */
 
 /* Copies the committed state to host arrays (any may be NULL):

@@ -99,6 +99,7 @@ def lib():
         "ipf_assemble_tangent": (_int, [_vp, _vp, _vp, _vp]),
         "ipf_cg_solve": (_int, [_vp, _vp, _vp, _d, _int, _ip, _dp]),
         "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
+        "ipf_set_cg_graph": (_int, [_vp, _int]),
         "ipf_get_state": (_int, [_vp, _dp, _dp, _ip, _ip]),
         "ipf_point_features": (_int, [_vp, _vp, _vp, _vp, _vp]),
         "ipf_branch_order": (_int, [_vp, _vp, _vp]),
@@ -467,6 +468,10 @@ class DeviceContext:
 
     def tangent_apply(self, v, y):
         self._c(lib().ipf_tangent_apply(self._h, _ptr(v), _ptr(y)), "ipf_tangent_apply")
+
+    def set_cg_graph(self, enable):
+        """CG iterations from a captured HIP graph (default) or launched one by one."""
+        self._c(lib().ipf_set_cg_graph(self._h, 1 if enable else 0), "ipf_set_cg_graph")
 
     def cg_solve(self, b, x, rtol=1e-10, maxit=10000):
         it = ctypes.c_int(0)

@@ -227,6 +227,16 @@ struct ipf_ctx {
     bool compact_ok = false;  // every block has the 2D elasticity pattern
     bool have_tangent = false;
 
+    // CG iterations as a captured HIP graph (one block of kCgBlock
+    // iterations between convergence checks), replayed on `stream`; rebuilt
+    // when the iterate pointer or the tangent format changes, dropped with
+    // the data
+    bool cg_graph = true;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t cg_exec = nullptr;
+    const double* cg_x = nullptr;
+    int cg_tfmt = -1;
+
     // tangent CSR pattern (ipf_set_tangent_pattern): per group element, the
     // CSR positions of its packed upper triangle
     int* dLidx = nullptr;
@@ -269,7 +279,15 @@ void dev_free(T*& p) {
     p = nullptr;
 }
 
+void drop_cg_graph(ipf_ctx* ctx) {
+    if (ctx->cg_exec) (void)hipGraphExecDestroy(ctx->cg_exec);
+    ctx->cg_exec = nullptr;
+    ctx->cg_x = nullptr;
+    ctx->cg_tfmt = -1;
+}
+
 void release_data(ipf_ctx* ctx) {
+    drop_cg_graph(ctx);
     dev_free(ctx->dB);
     dev_free(ctx->dDof);
     dev_free(ctx->dCol);
@@ -402,12 +420,64 @@ void launch_group_residual(ipf_ctx* ctx, const Group& g, const double* u, double
 }
 
 // One tangent (K v) launch for a group in the selected B format.
-void launch_group_tangent(ipf_ctx* ctx, const Group& g, const double* v, double* y) {
+void launch_group_tangent(ipf_ctx* ctx, const Group& g, const double* v, double* y, hipStream_t s) {
     if (ctx->tfmt == FMT_D64)
-        launch_tangent(g.shape, group_dev(ctx, g), v, y, ctx->dDep, ctx->npad, ctx->stream);
+        launch_tangent(g.shape, group_dev(ctx, g), v, y, ctx->dDep, ctx->npad, s);
     else
         launch_tangent_fmt(ctx->tfmt, g.shape, group_dev(ctx, g), group_fmt_B(ctx, g, ctx->tfmt), v, y, ctx->dDep,
-                           ctx->npad, ctx->stream);
+                           ctx->npad, s);
+}
+
+constexpr int kCgBlock = 10;  // CG iterations between convergence checks
+
+// `count` preconditioned CG iterations on stream s: q = K p (masked) and
+// p.q, alpha, x / r / z update and r.z, beta, new p and q = 0.  The scalars
+// stay on the device, so the sequence has no host dependency.
+void cg_iterations(ipf_ctx* ctx, double* x, int count, hipStream_t s) {
+    const int64_t n = ctx->neq;
+    const int nb = dot_blocks(n);
+    for (int k = 0; k < count; k++) {
+        for (const auto& g : ctx->groups) launch_group_tangent(ctx, g, ctx->dCp, ctx->dCq, s);
+        launch_cg_mask_dot(n, ctx->dMask, ctx->dCp, ctx->dCq, ctx->dPartial, nb, ctx->dScal + 1, s);
+        launch_cg_scalars(ctx->dScal, 0, s);
+        launch_cg_update_dot(n, ctx->dScal, ctx->dCp, ctx->dCq, x, ctx->dCr, ctx->dDiag, ctx->dCz, ctx->dPartial, nb,
+                             ctx->dScal + 2, s);
+        launch_cg_scalars(ctx->dScal, 1, s);
+        launch_cg_direction_clear(n, ctx->dScal, ctx->dCz, ctx->dCp, ctx->dCq, s);
+    }
+}
+
+// The graph of kCgBlock iterations for iterate x (captured on the context's
+// capture stream; nullptr when capture is off or fails, and the caller then
+// launches the iterations one by one).
+hipGraphExec_t cg_graph(ipf_ctx* ctx, double* x) {
+    if (!ctx->cg_graph) return nullptr;
+    if (ctx->cg_exec && ctx->cg_x == x && ctx->cg_tfmt == ctx->tfmt) return ctx->cg_exec;
+    drop_cg_graph(ctx);
+    if (!ctx->cap_stream && hipStreamCreate(&ctx->cap_stream) != hipSuccess) {
+        ctx->cap_stream = nullptr;
+        ctx->cg_graph = false;
+        return nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        ctx->cg_graph = false;
+        return nullptr;
+    }
+    cg_iterations(ctx, x, kCgBlock, ctx->cap_stream);
+    const hipError_t e1 = hipStreamEndCapture(ctx->cap_stream, &graph);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 = (e1 == hipSuccess) ? hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) : e1;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->cg_graph = false;
+        return nullptr;
+    }
+    ctx->cg_exec = exec;
+    ctx->cg_x = x;
+    ctx->cg_tfmt = ctx->tfmt;
+    return exec;
 }
 
 int check_ready(ipf_ctx* ctx, bool need_material) {
@@ -489,6 +559,7 @@ int ipf_ctx_destroy(ipf_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     release_data(ctx);
     dev_free(ctx->dMats);
+    if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return IPF_OK;
@@ -927,6 +998,7 @@ int ipf_set_material_table(ipf_ctx* ctx, int nmat, const double* params) {
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->mats.size() != mats.size()) {
+        drop_cg_graph(ctx);  // its launches carry the table's address
         dev_free(ctx->dMats);
         int rc = dev_alloc(ctx, &ctx->dMats, (int64_t)nmat);
         if (rc) {
@@ -1140,6 +1212,16 @@ int ipf_set_b_format(ipf_ctx* ctx, int residual_format, int tangent_format) {
     if ((rc = ensure_format(ctx, residual_format)) || (rc = ensure_format(ctx, tangent_format))) return rc;
     ctx->rfmt = residual_format;
     ctx->tfmt = tangent_format;
+    drop_cg_graph(ctx);
+    return IPF_OK;
+}
+
+int ipf_set_cg_graph(ipf_ctx* ctx, int enable) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drop_cg_graph(ctx);
+    ctx->cg_graph = enable != 0;
     return IPF_OK;
 }
 
@@ -1256,7 +1338,7 @@ int ipf_tangent_apply(ipf_ctx* ctx, const double* v_dev, double* y_dev) {
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     launch_fill(ctx->neq, y_dev, 0.0, ctx->stream);
     for (const auto& g : ctx->groups)
-        launch_group_tangent(ctx, g, v_dev, y_dev);
+        launch_group_tangent(ctx, g, v_dev, y_dev, ctx->stream);
     return launch_status(ctx, "tangent_apply");
 }
 
@@ -1285,27 +1367,24 @@ int ipf_cg_solve(ipf_ctx* ctx, const double* b_dev, double* x_dev, double rtol, 
     int it = 0;
     double rel = 0.0;
     if (bnorm > 0.0) {
-        const int check_every = 10;
+        // q = K p on the free DoFs; q starts at zero and every direction
+        // update clears it.  Blocks of kCgBlock iterations (one graph launch)
+        // between convergence checks; a shorter last block up to maxit.
         launch_fill(n, ctx->dCq, 0.0, s);
-        for (it = 1; it <= maxit; it++) {
-            // q = K p on the free DoFs (q was cleared by the previous direction update)
-            for (const auto& g : ctx->groups)
-                launch_group_tangent(ctx, g, ctx->dCp, ctx->dCq);
-            launch_cg_mask_dot(n, ctx->dMask, ctx->dCp, ctx->dCq, ctx->dPartial, nb, ctx->dScal + 1, s);
-            launch_cg_scalars(ctx->dScal, 0, s);
-            launch_cg_update_dot(n, ctx->dScal, ctx->dCp, ctx->dCq, x_dev, ctx->dCr, ctx->dDiag, ctx->dCz,
-                                 ctx->dPartial, nb, ctx->dScal + 2, s);
-            launch_cg_scalars(ctx->dScal, 1, s);
-            launch_cg_direction_clear(n, ctx->dScal, ctx->dCz, ctx->dCp, ctx->dCq, s);
-            if (it % check_every == 0 || it == maxit) {
-                launch_dot(n, ctx->dCr, ctx->dCr, ctx->dPartial, nb, ctx->dScal + 3, s);
-                double rr;
-                if ((rc = launch_status(ctx, "cg")) || (rc = read_scalar(ctx, 3, &rr))) return rc;
-                rel = std::sqrt(rr) / bnorm;
-                if (!(rel > rtol)) break;  // also stops on NaN
-            }
+        while (it < maxit) {
+            const int blk = std::min(kCgBlock, maxit - it);
+            hipGraphExec_t exec = blk == kCgBlock ? cg_graph(ctx, x_dev) : nullptr;
+            if (exec)
+                IPF_HIP(ctx, hipGraphLaunch(exec, s));
+            else
+                cg_iterations(ctx, x_dev, blk, s);
+            it += blk;
+            launch_dot(n, ctx->dCr, ctx->dCr, ctx->dPartial, nb, ctx->dScal + 3, s);
+            double rr;
+            if ((rc = launch_status(ctx, "cg")) || (rc = read_scalar(ctx, 3, &rr))) return rc;
+            rel = std::sqrt(rr) / bnorm;
+            if (!(rel > rtol)) break;  // also stops on NaN
         }
-        if (it > maxit) it = maxit;
     }
     if (iters) *iters = it;
     if (relres) *relres = rel;
