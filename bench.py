@@ -252,6 +252,12 @@ def run_batch(args):
     ctx.set_state(np.zeros(6 * npts))
     branch_counts = {ipf.BRANCH_NAMES[i]: int((br == i).sum()) for i in range(5)}
     log(f"[bench] branches: {branch_counts}")
+    tune = None
+    if not args.no_autotune:
+        tune = ctx.autotune(u, reps=5)
+        log(f"[bench] autotune: {', '.join(f'{k} {v:.4f} ms' for k, v in tune['residual'].items())} -> "
+            f"{tune['selected'][0]}")
+        tune = {"residual_ms": {k: round(v, 5) for k, v in tune["residual"].items()}, "selected": tune["selected"][0]}
     fmt0 = ctx.b_format()[0]
     total_ms = time_residuals(ctx, u, r, stream, args.steps, args.warmup)
     ms_per_step = total_ms / args.steps
@@ -297,7 +303,7 @@ def run_batch(args):
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "plain FP64 batched residual, 64 uneven wellbore instances (seed-0 table)",
-                      "b_format": fmt0, "instances": B.n, "elements": int(len(B.row_sizes)), "int_points": npts,
+                      "b_format": fmt0, "autotune": tune, "instances": B.n, "elements": int(len(B.row_sizes)), "int_points": npts,
                       "neq": neq, "colors": B.ncolor, "branches": branch_counts,
                       "state": "u = each instance's linear-elastic thick-cylinder displacement, zero plastic strain"},
            "roofline": roofline, "cpu_baseline": cpu}
@@ -314,6 +320,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-formats", action="store_true", help="skip the compact / FP32 B format variants")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned branch-predictor measurement")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the library's default B format instead of the autotuner's choice")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's timed work")
     ap.add_argument("--state", choices=("newton", "lame"), default="newton",
                     help="newton (default): the converged 10-step Newton solution (||r||/||r0|| <= --newton-rtol) "
@@ -422,8 +430,15 @@ def main():
     log(f"[bench] branches: {branch_counts}")
     ep_state, al_state, _, _ = ctx.get_state()  # the committed state the timed residuals start from
 
-    fmt0 = ctx.b_format()[0]  # the library's choice for these data (geometric FP64 after the device setup)
-    shape = info["porder"] - 1
+    tune = None
+    if not args.no_autotune:
+        # the fastest exact FP64 B format for these data (all give the same bits)
+        tune = ctx.autotune(u, reps=10)
+        log(f"[bench] autotune: {', '.join(f'{k} {v:.4f} ms' for k, v in tune['residual'].items())} -> "
+            f"{tune['selected'][0]}")
+        tune = {"residual_ms": {k: round(v, 5) for k, v in tune["residual"].items()}, "selected": tune["selected"][0]}
+    fmt0 = ctx.b_format()[0]  # the autotuner's choice (geometric FP64 after the device setup without it)
+    shape = {1: 0, 2: 1, 3: 2}[info["porder"]]
     total_ms = time_residuals(ctx, u, r, stream, args.steps, args.warmup, roi="timed residuals")
     ms_per_step = total_ms / args.steps
     value = npts * args.steps / (total_ms / 1e3) / 1e6
@@ -532,7 +547,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"plain FP64 residual, {args.mesh} quarter-annulus wellbore",
-                   "b_format": fmt0,
+                   "b_format": fmt0, "autotune": tune,
                    "elements": sizes["nblocks"], "int_points": npts, "neq": neq, "colors": sizes["ncolor"],
                    "state": state_desc, "branches": branch_counts, "newton": newton_info},
         "roofline": roofline,

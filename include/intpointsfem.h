@@ -205,6 +205,16 @@ enum {
  * and gives the compact FP64 results bit for bit.  No reference counterpart:
  * the reference stores dense FP64 blocks only (TPZIrregularBlocksMatrix.h:20). */
 int ipf_set_b_format(ipf_ctx *ctx, int residual_format, int tangent_format);
+/* Autotuner: times the fused residual (reps launches at u_dev, committed
+ * state untouched) in every exact FP64 form the data allows (dense, compact,
+ * geometric) and, after ipf_residual_tangent, the tangent product likewise
+ * (with allow_fp32, also the FP32 forms: the CG matvecs then steer an FP64
+ * Newton, see above), and selects the fastest of each.  The FP64 forms give
+ * bit-identical results, so the choice changes only the time.  times_ms
+ * (optional) [2 * 5]: average ms per residual by format, then per tangent
+ * product by format, -1 where not timed (format codes as above).  Frees
+ * stored forms neither selection uses.  No reference counterpart. */
+int ipf_autotune(ipf_ctx *ctx, const double *u_dev, int reps, int allow_fp32, double *times_ms);
 /* The formats in use.  ipf_transfer_data selects IPF_B_COMPACT_FP64 for both
  * when every block has the 2D elasticity pattern, IPF_B_DENSE_FP64 otherwise;
  * ipf_transfer_geometry selects IPF_B_GEOMETRY_FP64. */

@@ -100,6 +100,7 @@ def lib():
         "ipf_cg_solve": (_int, [_vp, _vp, _vp, _d, _int, _ip, _dp]),
         "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
         "ipf_set_cg_graph": (_int, [_vp, _int]),
+        "ipf_autotune": (_int, [_vp, _vp, _int, _int, _dp]),
         "ipf_get_state": (_int, [_vp, _dp, _dp, _ip, _ip]),
         "ipf_point_features": (_int, [_vp, _vp, _vp, _vp, _vp]),
         "ipf_branch_order": (_int, [_vp, _vp, _vp]),
@@ -468,6 +469,17 @@ class DeviceContext:
 
     def tangent_apply(self, v, y):
         self._c(lib().ipf_tangent_apply(self._h, _ptr(v), _ptr(y)), "ipf_tangent_apply")
+
+    def autotune(self, u, reps=20, allow_fp32=False):
+        """Selects the fastest B format for the residual and (after
+        residual_tangent) the tangent product (ipf_autotune); returns
+        {"residual": {format: ms}, "tangent": {format: ms}, "selected": (r, t)}."""
+        t = np.zeros(2 * len(B_FORMATS))
+        self._c(lib().ipf_autotune(self._h, _ptr(u), int(reps), 1 if allow_fp32 else 0, _p(t, _dp)), "ipf_autotune")
+        n = len(B_FORMATS)
+        return {"residual": {B_FORMATS[k]: t[k] for k in range(n) if t[k] >= 0},
+                "tangent": {B_FORMATS[k]: t[n + k] for k in range(n) if t[n + k] >= 0},
+                "selected": self.b_format()}
 
     def set_cg_graph(self, enable):
         """CG iterations from a captured HIP graph (default) or launched one by one."""

@@ -1216,6 +1216,92 @@ int ipf_set_b_format(ipf_ctx* ctx, int residual_format, int tangent_format) {
     return IPF_OK;
 }
 
+int ipf_autotune(ipf_ctx* ctx, const double* u_dev, int reps, int allow_fp32, double* times_ms) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!u_dev) return fail(ctx, IPF_EINVAL, "ipf_autotune: u is NULL");
+    if (reps < 1 || reps > 10000) return fail(ctx, IPF_EINVAL, "ipf_autotune: reps %d outside [1, 10000]", reps);
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if (times_ms)
+        for (int k = 0; k < 2 * FMT_COUNT; k++) times_ms[k] = -1.0;
+    // candidates: the exact FP64 forms the data allows; FP32 forms only for
+    // the tangent product and only on request
+    auto allowed = [&](int f) {
+        if (f == FMT_G64) return ctx->have_geometry;
+        if (format_compact(f)) return ctx->compact_ok;
+        return true;
+    };
+    hipEvent_t e0, e1;
+    IPF_HIP(ctx, hipEventCreate(&e0));
+    IPF_HIP(ctx, hipEventCreate(&e1));
+    const bool commit = ctx->update_mem;
+    ctx->update_mem = false;  // the committed state must not change
+    const int r0 = ctx->rfmt, t0 = ctx->tfmt;
+    auto time_it = [&](auto&& launch, double* ms) -> int {
+        for (int k = 0; k < 2; k++) launch();
+        IPF_HIP(ctx, hipEventRecord(e0, ctx->stream));
+        for (int k = 0; k < reps; k++) launch();
+        IPF_HIP(ctx, hipEventRecord(e1, ctx->stream));
+        IPF_HIP(ctx, hipEventSynchronize(e1));
+        float t = 0.f;
+        IPF_HIP(ctx, hipEventElapsedTime(&t, e0, e1));
+        *ms = (double)t / reps;
+        return launch_status(ctx, "autotune");
+    };
+    int best_r = r0, best_t = t0;
+    double tr = 1e300, tt = 1e300;
+    for (int f = 0; f < FMT_COUNT && rc == IPF_OK; f++) {
+        const bool fp32 = format_bytes(f) == 4;
+        if (!allowed(f)) continue;
+        if ((rc = ensure_format(ctx, f))) break;
+        double ms;
+        if (!fp32) {
+            ctx->rfmt = f;
+            rc = time_it(
+                [&] {
+                    launch_copy(ctx->neq, ctx->dFlin, ctx->dR, ctx->stream);
+                    for (const auto& g : ctx->groups) launch_group_residual(ctx, g, u_dev, ctx->dR, point_state(ctx));
+                },
+                &ms);
+            if (rc) break;
+            if (times_ms) times_ms[f] = ms;
+            if (ms < tr) tr = ms, best_r = f;
+        }
+        if (ctx->have_tangent && (!fp32 || allow_fp32)) {
+            ctx->tfmt = f;
+            rc = time_it(
+                [&] {
+                    launch_fill(ctx->neq, ctx->dR, 0.0, ctx->stream);
+                    for (const auto& g : ctx->groups) launch_group_tangent(ctx, g, u_dev, ctx->dR, ctx->stream);
+                },
+                &ms);
+            if (rc) break;
+            if (times_ms) times_ms[FMT_COUNT + f] = ms;
+            if (ms < tt) tt = ms, best_t = f;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->update_mem = commit;
+    ctx->rfmt = rc ? r0 : best_r;
+    ctx->tfmt = rc ? t0 : best_t;
+    drop_cg_graph(ctx);
+    if (rc) return rc;
+    // stored forms nobody uses any more go (dense FP64 stays: the separate steps read it)
+    for (int f = 1; f < FMT_COUNT; f++)
+        if (f != ctx->rfmt && f != ctx->tfmt && ctx->dBf[f]) {
+            IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            int64_t n = 0;
+            for (const auto& g : ctx->groups)
+                n = std::max(n, g.f_off[f] + g.ntiles * (int64_t)format_values(g.shape, f) * shape_tp(g.shape));
+            ctx->bytes -= n * format_bytes(f);
+            char* p = static_cast<char*>(ctx->dBf[f]);
+            dev_free(p);
+            ctx->dBf[f] = nullptr;
+        }
+    return IPF_OK;
+}
+
 int ipf_set_cg_graph(ipf_ctx* ctx, int enable) {
     if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
     IPF_HIP(ctx, hipSetDevice(ctx->device));

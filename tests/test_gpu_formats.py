@@ -187,3 +187,37 @@ def test_default_formats(gpu):
     ch.transfer_data(b["row_sizes"], b["col_sizes"], b["matrix_position"], b["row_first"], b["col_first"],
                      b["storage"], ix["dof_indexes"], ix["color_indexes"], s["ncolor"], s["neq"], ix["weight"])
     assert ch.b_format() == ("compact-fp64", "compact-fp64")
+
+
+def test_autotune_selects_the_fastest_format(gpu):
+    """ipf_autotune times every exact FP64 form (dense, compact, geometric)
+    for the residual and the tangent product, selects the fastest of each,
+    and leaves the committed state alone; the residual afterwards is the same
+    bits as before.  With allow_fp32 the FP32 forms compete for the tangent
+    product only."""
+    import torch
+    m, sm, ctx, B, ix = _setup((20, 20, 2, 1.1))
+    s = sm.sizes()
+    neq = s["neq"]
+    u = _t(wb.lame_displacement(m.nodes(), wb.normal_stress(10), scale=4.0), gpu)
+    r0 = torch.zeros(neq, dtype=torch.float64, device=gpu)
+    ctx.residual(u, r0)
+    st0 = ctx.get_state()
+    rt = torch.zeros(neq, dtype=torch.float64, device=gpu)
+    ctx.residual_tangent(u, rt)
+    t = ctx.autotune(u, reps=5)
+    assert set(t["residual"]) == {"dense-fp64", "compact-fp64", "geometry-fp64"}
+    assert set(t["tangent"]) == {"dense-fp64", "compact-fp64", "geometry-fp64"}
+    r_sel, t_sel = t["selected"]
+    assert t["residual"][r_sel] == min(t["residual"].values())
+    assert t["tangent"][t_sel] == min(t["tangent"].values())
+    r1 = torch.zeros(neq, dtype=torch.float64, device=gpu)
+    ctx.residual(u, r1)
+    ctx.synchronize()
+    assert (r1.cpu().numpy() == r0.cpu().numpy()).all()
+    assert all(np.array_equal(a, b) for a, b in zip(ctx.get_state(), st0))
+    t2 = ctx.autotune(u, reps=5, allow_fp32=True)
+    assert "dense-fp32" not in t2["residual"] and "compact-fp32" not in t2["residual"]
+    assert {"dense-fp32", "compact-fp32"} <= set(t2["tangent"])
+    assert t2["tangent"][t2["selected"][1]] == min(t2["tangent"].values())
+    print(f"autotune: residual {t['residual']} -> {r_sel}; tangent {t2['tangent']} -> {t2['selected'][1]}")
