@@ -313,6 +313,21 @@ int ipf_cg_solve(ipf_ctx *ctx, const double *b_dev, double *x_dev, double rtol, 
  * ||r||, total CG iterations, worst CG relative residual, 0. */
 int ipf_newton(ipf_ctx *ctx, double *u_dev, double rtol, int maxit, double cg_rtol, int cg_maxit, double *stats);
 
+/* ipf_newton for independent blocks of equations (batched instances): segment
+ * k holds the equations [seg_first[k], seg_first[k+1]) (seg_first [nseg+1],
+ * from 0 to neq) and converges on its own criterion ||r_k|| <= rtol ||r0_k||
+ * (norms over the DoFs the Dirichlet mask leaves free); once it has, its
+ * DoFs are held -- out of the CG and of the update -- while the others
+ * iterate, and the mask is restored afterwards.  The commit follows as in
+ * ipf_newton.  stats as ipf_newton's, with [1] = all segments converged, [2]
+ * the largest ||r_k|| / ||r0_k||, [3] / [4] the 2-norms over all segments;
+ * seg_rel [nseg] (optional) the final ratio per segment, seg_iters [nseg]
+ * (optional) the Newton iteration at which each converged (-1: not
+ * converged).  The per-instance convergence of a batch; no reference
+ * counterpart (the reference solves one mesh). */
+int ipf_newton_segmented(ipf_ctx *ctx, double *u_dev, int nseg, const int64_t *seg_first, double rtol, int maxit,
+                         double cg_rtol, int cg_maxit, double *stats, double *seg_rel, int *seg_iters);
+
 /* The CG iterations of ipf_cg_solve / ipf_newton run as a captured HIP graph
  * of 10 iterations (the launches between two convergence checks), replayed
  * on the context's stream: fewer launch gaps, the same kernels in the same

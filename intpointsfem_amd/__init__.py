@@ -100,6 +100,7 @@ def lib():
         "ipf_cg_solve": (_int, [_vp, _vp, _vp, _d, _int, _ip, _dp]),
         "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
         "ipf_set_cg_graph": (_int, [_vp, _int]),
+        "ipf_newton_segmented": (_int, [_vp, _vp, _int, _lp, _d, _int, _d, _int, _dp, _dp, _ip]),
         "ipf_autotune": (_int, [_vp, _vp, _int, _int, _dp]),
         "ipf_get_state": (_int, [_vp, _dp, _dp, _ip, _ip]),
         "ipf_point_features": (_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -480,6 +481,23 @@ class DeviceContext:
         return {"residual": {B_FORMATS[k]: t[k] for k in range(n) if t[k] >= 0},
                 "tangent": {B_FORMATS[k]: t[n + k] for k in range(n) if t[n + k] >= 0},
                 "selected": self.b_format()}
+
+    def newton_segmented(self, u, seg_first, rtol=1e-10, maxit=50, cg_rtol=1e-10, cg_maxit=20000):
+        """ipf_newton_segmented: every segment [seg_first[k], seg_first[k+1])
+        of the equations converges on its own criterion.  Returns the stats
+        dict of newton() plus seg_rel and seg_iters arrays."""
+        sf = np.ascontiguousarray(seg_first, dtype=np.int64)
+        n = sf.size - 1
+        st = np.zeros(8)
+        rel = np.zeros(n)
+        its = np.zeros(n, dtype=np.int32)
+        self._c(lib().ipf_newton_segmented(self._h, _ptr(u), n, _p(sf, _lp), rtol, maxit, cg_rtol, cg_maxit,
+                                           _p(st, _dp), _p(rel, _dp), _p(its, _ip)), "ipf_newton_segmented")
+        keys = ("iterations", "converged", "rel_residual", "r0", "rn", "cg_iterations", "worst_cg_rel", "reserved")
+        out = dict(zip(keys, (float(v) for v in st)))
+        out["seg_rel"] = rel
+        out["seg_iters"] = its
+        return out
 
     def set_cg_graph(self, enable):
         """CG iterations from a captured HIP graph (default) or launched one by one."""

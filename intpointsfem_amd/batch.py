@@ -194,3 +194,9 @@ class Batch:
         self.row_sizes = h["row_sizes"]
         self.col_sizes = h["col_sizes"]
         self.meshes = list(meshes)
+
+    def newton(self, u, rtol=1e-10, maxit=80, cg_rtol=1e-12, cg_maxit=40000):
+        """Newton over the batch with per-instance convergence
+        (ipf_newton_segmented over the instances' equation ranges)."""
+        return self.ctx.newton_segmented(u, self.eq_offsets, rtol=rtol, maxit=maxit, cg_rtol=cg_rtol,
+                                         cg_maxit=cg_maxit)

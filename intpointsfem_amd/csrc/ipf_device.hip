@@ -1523,6 +1523,136 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
     return IPF_OK;
 }
 
+int ipf_newton_segmented(ipf_ctx* ctx, double* u_dev, int nseg, const int64_t* seg_first, double rtol, int maxit,
+                         double cg_rtol, int cg_maxit, double* stats, double* seg_rel, int* seg_iters) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!u_dev || !seg_first) return fail(ctx, IPF_EINVAL, "ipf_newton_segmented: NULL pointer");
+    if (nseg < 1) return fail(ctx, IPF_EINVAL, "ipf_newton_segmented: nseg %d < 1", nseg);
+    if (seg_first[0] != 0 || seg_first[nseg] != ctx->neq)
+        return fail(ctx, IPF_EINVAL, "ipf_newton_segmented: segments must cover [0, neq)");
+    for (int k = 0; k < nseg; k++)
+        if (seg_first[k + 1] <= seg_first[k])
+            return fail(ctx, IPF_EINVAL, "ipf_newton_segmented: segment %d is empty or reversed", k);
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_solver_buffers(ctx))) return rc;
+    // chunks of at most 2^16 equations inside each segment
+    constexpr int64_t kChunk = 1 << 16;
+    std::vector<int64_t> cb, ce;
+    std::vector<int> sfc(nseg + 1, 0);
+    for (int k = 0; k < nseg; k++) {
+        sfc[k] = (int)cb.size();
+        for (int64_t b = seg_first[k]; b < seg_first[k + 1]; b += kChunk) {
+            cb.push_back(b);
+            ce.push_back(std::min(b + kChunk, seg_first[k + 1]));
+        }
+    }
+    sfc[nseg] = (int)cb.size();
+    const int nchunk = (int)cb.size();
+    int64_t* dcb = nullptr;
+    int64_t* dce = nullptr;
+    int* dsfc = nullptr;
+    double* dpart = nullptr;
+    double* dseg = nullptr;
+    signed char* dmask0 = nullptr;
+    auto cleanup = [&] {
+        dev_free(dcb);
+        dev_free(dce);
+        dev_free(dsfc);
+        dev_free(dpart);
+        dev_free(dseg);
+        dev_free(dmask0);
+    };
+    auto bail = [&](int code) {
+        cleanup();
+        return code;
+    };
+    const int64_t b0 = ctx->bytes;
+    if ((rc = dev_alloc(ctx, &dcb, nchunk)) || (rc = dev_alloc(ctx, &dce, nchunk)) ||
+        (rc = dev_alloc(ctx, &dsfc, nseg + 1)) || (rc = dev_alloc(ctx, &dpart, nchunk)) ||
+        (rc = dev_alloc(ctx, &dseg, nseg)) || (rc = dev_alloc(ctx, &dmask0, ctx->neq)))
+        return bail(rc);
+    ctx->bytes = b0;  // workspace of this call only
+    if (hipMemcpy(dcb, cb.data(), sizeof(int64_t) * nchunk, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dce, ce.data(), sizeof(int64_t) * nchunk, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dsfc, sfc.data(), sizeof(int) * (nseg + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpyAsync(dmask0, ctx->dMask, ctx->neq, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+        return bail(fail(ctx, IPF_EHIP, "ipf_newton_segmented: workspace upload failed"));
+    const bool commit_flag = ctx->update_mem;
+    ctx->update_mem = false;
+    std::vector<double> r0(nseg, 0.0), rn(nseg, 0.0);
+    std::vector<char> done(nseg, 0);
+    std::vector<int> its(nseg, -1);
+    double worst_cg = 0.0;
+    int it = 0, cg_total = 0;
+    bool all = false;
+    for (it = 0; it <= maxit; it++) {
+        if ((rc = ipf_residual_tangent(ctx, u_dev, ctx->dNr))) break;
+        // per-segment norms over the constrained DoFs of the caller's mask
+        launch_segment_sumsq(ctx->dNr, dmask0, dcb, dce, nchunk, dpart, dsfc, nseg, dseg, ctx->stream);
+        if (hipMemcpyAsync(rn.data(), dseg, sizeof(double) * nseg, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = fail(ctx, IPF_EHIP, "ipf_newton_segmented: norm read failed");
+            break;
+        }
+        all = true;
+        for (int k = 0; k < nseg; k++) {
+            rn[k] = std::sqrt(rn[k]);
+            if (it == 0) r0[k] = rn[k];
+            if (!done[k] && (rn[k] <= rtol * r0[k] || rn[k] == 0.0)) {
+                done[k] = 1;
+                its[k] = it;
+                // hold the segment: its DoFs leave the CG and the update
+                if (hipMemsetAsync(ctx->dMask + seg_first[k], 1, seg_first[k + 1] - seg_first[k], ctx->stream) !=
+                    hipSuccess) {
+                    rc = fail(ctx, IPF_EHIP, "ipf_newton_segmented: mask update failed");
+                    break;
+                }
+            }
+            all = all && done[k];
+        }
+        if (rc || all || it == maxit) break;
+        int cgi = 0;
+        double cgr = 0.0;
+        if ((rc = ipf_cg_solve(ctx, ctx->dNr, ctx->dNd, cg_rtol, cg_maxit, &cgi, &cgr))) break;
+        cg_total += cgi;
+        worst_cg = std::max(worst_cg, cgr);
+        launch_axpy(ctx->neq, 1.0, ctx->dNd, u_dev, ctx->stream);
+    }
+    // the caller's constraints back, then AcceptPseudoTimeStepSolution
+    const hipError_t e = hipMemcpyAsync(ctx->dMask, dmask0, ctx->neq, hipMemcpyDeviceToDevice, ctx->stream);
+    if (!rc && e != hipSuccess) rc = fail(ctx, IPF_EHIP, "ipf_newton_segmented: mask restore failed");
+    if (!rc) {
+        ctx->update_mem = true;
+        rc = ipf_residual(ctx, u_dev, ctx->dNr);
+    }
+    ctx->update_mem = commit_flag;
+    const int rs = ipf_synchronize(ctx);
+    if (!rc) rc = rs;
+    cleanup();
+    if (rc) return rc;
+    double worst = 0.0, s0 = 0.0, sn = 0.0;
+    for (int k = 0; k < nseg; k++) {
+        const double rel = r0[k] > 0 ? rn[k] / r0[k] : 0.0;
+        worst = std::max(worst, rel);
+        s0 += r0[k] * r0[k];
+        sn += rn[k] * rn[k];
+        if (seg_rel) seg_rel[k] = rel;
+        if (seg_iters) seg_iters[k] = done[k] ? its[k] : -1;
+    }
+    if (stats) {
+        stats[0] = it;
+        stats[1] = all ? 1.0 : 0.0;
+        stats[2] = worst;
+        stats[3] = std::sqrt(s0);
+        stats[4] = std::sqrt(sn);
+        stats[5] = cg_total;
+        stats[6] = worst_cg;
+        stats[7] = 0.0;
+    }
+    return IPF_OK;
+}
+
 int ipf_get_state(ipf_ctx* ctx, double* plastic_strain_host, double* alpha_host, int* mtype_host, int* branch_host) {
     int rc = check_ready(ctx, false);
     if (rc) return rc;

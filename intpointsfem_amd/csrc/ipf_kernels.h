@@ -167,5 +167,11 @@ void launch_cg_direction_clear(int64_t n, const double* scal, const double* z, d
 void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t stream);
 void launch_mask_copy(int64_t n, const signed char* mask, const double* src, double* dst, hipStream_t stream);
 int dot_blocks(int64_t n);
+// out[k] = sum of x[i]^2 over the free (mask 0) i of segment k, the segment
+// being chunks [seg_first_chunk[k], seg_first_chunk[k+1]) of [chunk_begin,
+// chunk_end) ranges; fixed order (deterministic).
+void launch_segment_sumsq(const double* x, const signed char* mask, const int64_t* chunk_begin,
+                          const int64_t* chunk_end, int nchunk, double* partial, const int* seg_first_chunk, int nseg,
+                          double* out, hipStream_t stream);
 
 }  // namespace ipf

@@ -153,7 +153,42 @@ __global__ void k_mask_copy(int64_t n, const signed char* __restrict__ mask, con
 
 inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// Sum of squares of the free entries of x in chunk c = [begin[c], end[c]),
+// fixed reduction order.
+__global__ void __launch_bounds__(kDotThreads)
+k_chunk_sumsq(const double* __restrict__ x, const signed char* __restrict__ mask, const int64_t* __restrict__ begin,
+              const int64_t* __restrict__ end, double* __restrict__ partial) {
+    __shared__ double sh[kDotThreads];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int64_t i = begin[c] + threadIdx.x; i < end[c]; i += kDotThreads)
+        if (!mask[i]) s = s + x[i] * x[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[c] = t;
+}
+
+// Segment k's chunks [first[k], first[k+1]) summed in order.
+__global__ void k_segment_sum(const double* __restrict__ partial, const int* __restrict__ first, int nseg,
+                              double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nseg) return;
+    double s = 0.0;
+    for (int c = first[k]; c < first[k + 1]; c++) s = s + partial[c];
+    out[k] = s;
+}
+
 }  // namespace
+
+void launch_segment_sumsq(const double* x, const signed char* mask, const int64_t* chunk_begin,
+                          const int64_t* chunk_end, int nchunk, double* partial, const int* seg_first_chunk, int nseg,
+                          double* out, hipStream_t s) {
+    if (nchunk > 0)
+        hipLaunchKernelGGL(k_chunk_sumsq, dim3(nchunk), dim3(kDotThreads), 0, s, x, mask, chunk_begin, chunk_end,
+                           partial);
+    if (nseg > 0)
+        hipLaunchKernelGGL(k_segment_sum, dim3((nseg + 255) / 256), dim3(256), 0, s, partial, seg_first_chunk, nseg,
+                           out);
+}
 
 void launch_cg_mask_dot(int64_t n, const signed char* mask, const double* p, double* q, double* partial, int nb,
                         double* out, hipStream_t s) {

@@ -72,7 +72,11 @@ NEWTON_TABLE = [dict(N=9, p=1, phi_deg=30.0, cohesion=10.0, E=5000.0, nu=0.2, pw
 
 def test_batch_newton_reaches_each_instance_solution(gpu):
     """10 load steps (the reference driver's ramp) of the batched Newton --
-    one global system, block diagonal -- against each instance's own Newton."""
+    one global system, block diagonal -- against each instance's own Newton:
+    with the global criterion (ipf_newton) within 1e-7 of max |u_i|, and with
+    per-instance convergence (ipf_newton_segmented: every instance to its own
+    ||r_i|| <= 1e-10 ||r0_i|| at every step) within 1e-12 (measured 4e-16 to
+    9e-16 on MI355X, against 1e-13 to 2e-12 with the global criterion)."""
     import torch
     table = NEWTON_TABLE
     meshes = [batch.instance_mesh(i) for i in table]
@@ -86,6 +90,17 @@ def test_batch_newton_reaches_each_instance_solution(gpu):
     _, _, mt, _ = B.ctx.get_state()
     assert (mt != 0).any()
     ub = u.cpu().numpy()
+    # per-instance convergence from the same start in a fresh batch
+    Bs = batch.Batch(meshes, device=0)
+    Bs.ctx.set_dirichlet_mask(np.concatenate([m.dirichlet_mask() for m in meshes]))
+    us = torch.zeros(Bs.neq, dtype=torch.float64, device=gpu)
+    for k in range(1, wb.NSTEPS + 1):
+        Bs.ctx.set_rhs_linear(Bs.rhs_linear * (k / wb.NSTEPS))
+        st = Bs.newton(us, rtol=1e-10, maxit=80, cg_rtol=1e-12, cg_maxit=40000)
+        assert st["converged"] == 1.0, (k, st)
+        assert (st["seg_rel"] <= 1e-10).all() and (st["seg_iters"] >= 0).all(), (k, st)
+        print(f"segmented step {k}: newton iterations per instance {st['seg_iters'].tolist()}", flush=True)
+    useg = us.cpu().numpy()
     for i, m in enumerate(meshes):
         sm = ipf.StructMatrix(m, device=0)
         ctx = sm.context()
@@ -98,8 +113,10 @@ def test_batch_newton_reaches_each_instance_solution(gpu):
             assert si["converged"] == 1.0, (i, k, si)
         ref = ui.cpu().numpy()
         err = np.abs(ub[B.eq_offsets[i]:B.eq_offsets[i + 1]] - ref).max() / np.abs(ref).max()
-        print(f"instance {i}: |u_batch - u_alone| / |u| = {err:.2e}")
+        err_s = np.abs(useg[Bs.eq_offsets[i]:Bs.eq_offsets[i + 1]] - ref).max() / np.abs(ref).max()
+        print(f"instance {i}: |u_batch - u_alone| / |u| = {err:.2e} (global criterion), {err_s:.2e} (per instance)")
         assert err <= 1e-7
+        assert err_s <= 1e-12
 
 
 def test_material_errors(gpu):
