@@ -109,8 +109,6 @@ def time_residuals(ctx, u, r, stream, steps, warmup, roi=None):
 
 def kernel_name(fmt, shape):
     """Name rocprofv3 shows for the residual kernel of a B format and element shape."""
-    if fmt == "dense-fp64":
-        return f"void k_residual<{shape}, false>"
     import intpointsfem_amd as ipf
     return f"void k_residual_fmt<{shape}, {ipf.B_FORMATS.index(fmt)}, false>"
 

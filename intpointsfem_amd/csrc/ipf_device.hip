@@ -407,25 +407,20 @@ PointState point_state(ipf_ctx* ctx) {
 
 const void* group_fmt_B(const ipf_ctx* ctx, const Group& g, int fmt) {
     if (fmt == FMT_G64) return ctx->dGeo + 8 * g.slot0;
+    if (fmt == FMT_D64) return ctx->dB + g.b_off;
     return static_cast<const char*>(ctx->dBf[fmt]) + format_bytes(fmt) * (size_t)g.f_off[fmt];
 }
 
 // One residual launch for a group in the selected B format.
 void launch_group_residual(ipf_ctx* ctx, const Group& g, const double* u, double* r, const PointState& st) {
-    if (ctx->rfmt == FMT_D64)
-        launch_residual(g.shape, ctx->update_mem, group_dev(ctx, g), u, r, st, ctx->stream);
-    else
-        launch_residual_fmt(ctx->rfmt, g.shape, ctx->update_mem, group_dev(ctx, g), group_fmt_B(ctx, g, ctx->rfmt),
-                            u, r, st, ctx->stream);
+    launch_residual_fmt(ctx->rfmt, g.shape, ctx->update_mem, group_dev(ctx, g), group_fmt_B(ctx, g, ctx->rfmt), u, r,
+                        st, ctx->stream);
 }
 
 // One tangent (K v) launch for a group in the selected B format.
 void launch_group_tangent(ipf_ctx* ctx, const Group& g, const double* v, double* y, hipStream_t s) {
-    if (ctx->tfmt == FMT_D64)
-        launch_tangent(g.shape, group_dev(ctx, g), v, y, ctx->dDep, ctx->npad, s);
-    else
-        launch_tangent_fmt(ctx->tfmt, g.shape, group_dev(ctx, g), group_fmt_B(ctx, g, ctx->tfmt), v, y, ctx->dDep,
-                           ctx->npad, s);
+    launch_tangent_fmt(ctx->tfmt, g.shape, group_dev(ctx, g), group_fmt_B(ctx, g, ctx->tfmt), v, y, ctx->dDep,
+                       ctx->npad, s);
 }
 
 constexpr int kCgBlock = 10;  // CG iterations between convergence checks

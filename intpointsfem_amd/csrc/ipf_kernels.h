@@ -103,8 +103,6 @@ constexpr int format_values(int s, int f) {
 }
 
 // Kernel launchers (ipf_kernels.hip).  All enqueue on `stream`.
-void launch_residual(int shape, bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
-                     hipStream_t stream);
 void launch_multiply(int shape, const GroupDev& g, const double* u, double* strain, const int64_t* orig_of_new,
                      hipStream_t stream);
 void launch_transpose(int shape, const GroupDev& g, const double* wsigma, double* r, const int64_t* orig_of_new,
@@ -117,10 +115,9 @@ void launch_sigma(bool dep, bool commit, int shape, const GroupDev& g, int64_t n
 void launch_copy(int64_t n, const double* src, double* dst, hipStream_t stream);
 void launch_fill(int64_t n, double* dst, double value, hipStream_t stream);
 
-// Tangent operator (matrix-free): y += sum_e B_e^T (w Dep) B_e v_e for one
-// element group, w Dep stored SoA as dep_soa[k * npad + P], k = 3*row + col.
-void launch_tangent(int shape, const GroupDev& g, const double* v, double* y, const double* dep_soa, int64_t npad,
-                    hipStream_t stream);
+// Tangent operator (matrix-free) y += sum_e B_e^T (w Dep) B_e v_e: see
+// launch_tangent_fmt (every B format); w Dep is stored SoA as
+// dep_soa[k * npad + P], k = 3*row + col.
 // diag += diagonal of the same operator.
 void launch_tangent_diag(int shape, const GroupDev& g, double* diag, const double* dep_soa, int64_t npad,
                          hipStream_t stream);
@@ -131,9 +128,10 @@ void launch_tangent_diag(int shape, const GroupDev& g, double* diag, const doubl
 void launch_assemble_K(int shape, const GroupDev& g, const double* dep_soa, int64_t npad, const int* lidx,
                        double* kg, hipStream_t stream);
 
-// Format variants (ipf_kernels_fmt.hip): Bv points at the group's first tile
-// in format fmt (FMT_C64, FMT_D32 or FMT_C32), or for FMT_G64 at the corners
-// of the group's first element slot (with g.tab set).
+// The fused residual and the tangent product (ipf_kernels_fmt.hip): Bv points
+// at the group's first tile in format fmt (FMT_D64: the dense tiles g.B;
+// FMT_C64, FMT_D32, FMT_C32: the converted copies), or for FMT_G64 at the
+// corners of the group's first element slot (with g.tab set).
 void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
                          double* r, const PointState& st, hipStream_t stream);
 void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, const double* v, double* y,

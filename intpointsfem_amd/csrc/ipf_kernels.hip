@@ -1,17 +1,14 @@
-// ipf_kernels.hip -- gfx950 kernels of the integration-point residual path.
-//
-// k_residual is the hot path: one launch per element color, one 64-lane
-// wavefront per tile of elements (ipf_kernels.h).  Per tile it
-//   1. gathers u at the elements' DoFs (TPZNumericalIntegrator.cpp:32,47),
-//   2. streams the tile's B values with coalesced 16-byte loads and forms the
-//      strain of each point, summing columns j = 0..C_e-1 in order
-//      (TPZIrregularBlocksMatrix.cpp:97-140),
-//   3. runs the Mohr-Coulomb return map per point (ipf_pointmath.h),
-//   4. forms each point's contribution to -B^T (w sigma) from the B values
-//      still in registers, reduces the element's points through LDS, and
-//   5. adds the element forces into r.  Elements of one color share no DoF
-//      (TPZNumericalIntegrator.cpp:452-536), so the read-modify-write of r
-//      needs no atomics; colors run as consecutive launches on one stream.
+// ipf_kernels.hip -- gfx950 kernels of the reference's separate steps on the
+// dense FP64 blocks: Multiply (gather + B.u, TPZNumericalIntegrator.cpp:25-51),
+// MultiplyTranspose (-B^T x with colored accumulation, :54-105), ComputeSigma
+// / ComputeSigmaDep per point (ipf_pointmath.h), the Jacobi diagonal of the
+// tangent, copies and fills.  The fused residual and the tangent product, in
+// every B format, are in ipf_kernels_fmt.hip.  One 64-lane wavefront per tile
+// of elements (ipf_kernels.h); B values are streamed with coalesced 16-byte
+// loads and summed column by column in order (TPZIrregularBlocksMatrix.cpp:
+// 97-140); elements of one color share no DoF
+// (TPZNumericalIntegrator.cpp:452-536), so the read-modify-write of r needs no
+// atomics and colors run as consecutive launches on one stream.
 #include "ipf_kernels.h"
 
 namespace ipf {
@@ -102,67 +99,6 @@ __device__ __forceinline__ void reduce_scatter(const double* L, const GroupDev& 
             r[d] = r[d] + (-f);
         }
     }
-}
-
-template <int SHAPE, bool COMMIT>
-__global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave, Cfg<SHAPE>::MIN_WAVES)
-k_residual(GroupDev g, const double* __restrict__ u, double* __restrict__ r, PointState st) {
-    using C = Cfg<SHAPE>;
-    static_assert(C::ET * C::CE <= 2 * kWave, "two reduction tasks per lane");
-    __shared__ double lds[C::WPB][C::TP * C::CE];
-    const int wave = threadIdx.x / kWave;
-    const int lane = threadIdx.x % kWave;
-    const int64_t tile = (int64_t)blockIdx.x * C::WPB + wave;
-    const bool tile_ok = tile < g.ntiles;
-    const int el = lane / C::NIP;
-    const int64_t elem = tile * C::ET + el;
-    const bool slot = tile_ok && lane < C::TP;
-    const bool valid = slot && el < tile_nvalid(g, tile);
-
-    // B stream first: the largest and independent loads
-    double b[C::NB];
-    if (slot) {
-        load_point_B<SHAPE>(g.B, tile, lane, b);
-    } else {
-#pragma unroll
-        for (int k = 0; k < C::NB; k++) b[k] = 0.0;
-    }
-    double ue[C::CE];
-    if (valid) {
-        const int* d = g.dof + elem * C::CE;
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) ue[j] = u[ld_stream(&d[j])];
-    } else {
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) ue[j] = 0.0;
-    }
-    double e3[3];
-    strain_of<SHAPE>(b, ue, e3);
-
-    double ws[3] = {0.0, 0.0, 0.0};
-    if (valid) {
-        const int64_t P = g.pt0 + tile * C::TP + lane;
-        double ep[4] = {ld_stream(&st.ep0[P]), ld_stream(&st.ep1[P]), ld_stream(&st.ep2[P]), ld_stream(&st.ep3[P])};
-        // alpha only accumulates the plastic increment (H = 0: it does not
-        // enter the stress); without a commit it is not needed
-        double alpha = COMMIT ? ld_stream(&st.alpha[P]) : 0.0;
-        int mt, br;
-        point_sigma<false>(e3, ep, alpha, mt, br, tile_material(g, tile), ld_stream(&st.w[P]), ws, nullptr);
-        if (COMMIT) {
-            st.ep0[P] = ep[0];
-            st.ep1[P] = ep[1];
-            st.ep2[P] = ep[2];
-            st.ep3[P] = ep[3];
-            st.alpha[P] = alpha;
-            st.mtype[P] = (signed char)mt;
-            st.branch[P] = (signed char)br;
-        }
-    }
-    if (!C::KEEP_B && slot) load_point_B<SHAPE>(g.B, tile, lane, b);
-    double* L = lds[wave];
-    if (slot) bt_partial<SHAPE>(b, ws, L + lane * C::CE);
-    __syncthreads();
-    if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, r);
 }
 
 template <int SHAPE>
@@ -266,62 +202,6 @@ k_sigma(GroupDev g, int tp, int64_t npad, const double* __restrict__ strain, dou
 // matrix-free tangent of the Newton / CG solve.  Same tiling as k_residual,
 // with the return map replaced by the stored 3x3 w*Dep of each point.
 template <int SHAPE>
-__global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave, Cfg<SHAPE>::MIN_WAVES)
-k_tangent(GroupDev g, const double* __restrict__ v, double* __restrict__ y, const double* __restrict__ dep,
-          int64_t npad) {
-    using C = Cfg<SHAPE>;
-    __shared__ double lds[C::WPB][C::TP * C::CE];
-    const int wave = threadIdx.x / kWave;
-    const int lane = threadIdx.x % kWave;
-    const int64_t tile = (int64_t)blockIdx.x * C::WPB + wave;
-    const bool tile_ok = tile < g.ntiles;
-    const int el = lane / C::NIP;
-    const int64_t elem = tile * C::ET + el;
-    const bool slot = tile_ok && lane < C::TP;
-    const bool valid = slot && el < tile_nvalid(g, tile);
-    double b[C::NB];
-    if (slot) {
-        load_point_B<SHAPE>(g.B, tile, lane, b);
-    } else {
-#pragma unroll
-        for (int k = 0; k < C::NB; k++) b[k] = 0.0;
-    }
-    double ve[C::CE];
-    if (valid) {
-        const int* d = g.dof + elem * C::CE;
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) ve[j] = v[ld_stream(&d[j])];
-    } else {
-#pragma unroll
-        for (int j = 0; j < C::CE; j++) ve[j] = 0.0;
-    }
-    double e3[3];
-    strain_of<SHAPE>(b, ve, e3);
-    double x[3] = {0.0, 0.0, 0.0};
-    if (valid) {
-        const int64_t P = g.pt0 + tile * C::TP + lane;
-        double D[9];
-#pragma unroll
-        for (int i = 0; i < 9; i++) D[i] = ld_stream(&dep[i * npad + P]);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            double a = D[3 * k] * e3[0];
-            a = a + D[3 * k + 1] * e3[1];
-            a = a + D[3 * k + 2] * e3[2];
-            // reduce_scatter adds -(sum of partials); y accumulates +K v
-            x[k] = -a;
-        }
-    }
-    if (!C::KEEP_B && slot) load_point_B<SHAPE>(g.B, tile, lane, b);
-    double* L = lds[wave];
-    if (slot) bt_partial<SHAPE>(b, x, L + lane * C::CE);
-    __syncthreads();
-    if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, y);
-}
-
-// diag += diagonal of sum_e B_e^T (w Dep) B_e: per point and column j,
-// sum_{k,l} B[k][j] D[k][l] B[l][j].
-template <int SHAPE>
 __global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave)
 k_tangent_diag(GroupDev g, double* __restrict__ diag, const double* __restrict__ dep, int64_t npad) {
     using C = Cfg<SHAPE>;
@@ -376,23 +256,7 @@ dim3 tile_grid(const GroupDev& g) {
     return dim3((unsigned)((g.ntiles + Cfg<SHAPE>::WPB - 1) / Cfg<SHAPE>::WPB));
 }
 
-template <int SHAPE>
-void residual_shape(bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
-                    hipStream_t s) {
-    if (g.ntiles == 0) return;
-    const dim3 block(Cfg<SHAPE>::WPB * kWave);
-    if (commit)
-        hipLaunchKernelGGL((k_residual<SHAPE, true>), tile_grid<SHAPE>(g), block, 0, s, g, u, r, st);
-    else
-        hipLaunchKernelGGL((k_residual<SHAPE, false>), tile_grid<SHAPE>(g), block, 0, s, g, u, r, st);
-}
-
 }  // namespace
-
-void launch_residual(int shape, bool commit, const GroupDev& g, const double* u, double* r, const PointState& st,
-                     hipStream_t s) {
-    dispatch_shape(shape, [&](auto sc) { residual_shape<decltype(sc)::value>(commit, g, u, r, st, s); });
-}
 
 void launch_multiply(int shape, const GroupDev& g, const double* u, double* strain, const int64_t* orig,
                      hipStream_t s) {
@@ -423,15 +287,6 @@ void launch_sigma(bool dep, bool commit, int shape, const GroupDev& g, int64_t n
     else
         hipLaunchKernelGGL(k_sigma<false>, grid, dim3(kWave), 0, s, g, tp, npad, strain, wsigma, wdep, orig, st,
                            commit ? 1 : 0, dep_soa);
-}
-
-void launch_tangent(int shape, const GroupDev& g, const double* v, double* y, const double* dep, int64_t npad,
-                    hipStream_t s) {
-    if (g.ntiles == 0) return;
-    dispatch_shape(shape, [&](auto sc) {
-        constexpr int S = decltype(sc)::value;
-        hipLaunchKernelGGL(k_tangent<S>, tile_grid<S>(g), dim3(Cfg<S>::WPB * kWave), 0, s, g, v, y, dep, npad);
-    });
 }
 
 void launch_tangent_diag(int shape, const GroupDev& g, double* diag, const double* dep, int64_t npad,

@@ -1,6 +1,7 @@
-// ipf_kernels_fmt.hip -- the residual and tangent kernels for the alternative
-// storage formats of B (see ipf_kernels.h, BFormat):
+// ipf_kernels_fmt.hip -- the fused residual (the hot path) and the tangent
+// product, for every storage format of B (see ipf_kernels.h, BFormat):
 //
+//   FMT_D64  dense FP64: every stored value of the reference's block.
 //   FMT_C64  compact FP64: per point only (dN_a/dx, dN_a/dy), the values the
 //            reference writes into its B block (TPZNumericalIntegrator.cpp:
 //            371-381; the other entries of the 3 x 2n block are zeros or
@@ -438,6 +439,7 @@ void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, con
                          double* r, const PointState& st, hipStream_t s) {
     if (g.ntiles == 0) return;
     switch (fmt) {
+        case FMT_D64: residual_fmt<FMT_D64>(shape, commit, g, Bv, u, r, st, s); break;
         case FMT_C64: residual_fmt<FMT_C64>(shape, commit, g, Bv, u, r, st, s); break;
         case FMT_G64: residual_fmt<FMT_G64>(shape, commit, g, Bv, u, r, st, s); break;
         case FMT_D32: residual_fmt<FMT_D32>(shape, commit, g, Bv, u, r, st, s); break;
@@ -450,6 +452,7 @@ void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, c
                         const double* dep, int64_t npad, hipStream_t s) {
     if (g.ntiles == 0) return;
     switch (fmt) {
+        case FMT_D64: tangent_fmt<FMT_D64>(shape, g, Bv, v, y, dep, npad, s); break;
         case FMT_C64: tangent_fmt<FMT_C64>(shape, g, Bv, v, y, dep, npad, s); break;
         case FMT_G64: tangent_fmt<FMT_G64>(shape, g, Bv, v, y, dep, npad, s); break;
         case FMT_D32: tangent_fmt<FMT_D32>(shape, g, Bv, v, y, dep, npad, s); break;
