@@ -170,3 +170,44 @@ def test_geometric_format_needs_geometry(gpu):
     with pytest.raises(ipf.IPFError, match="geometry"):
         ch.set_b_format("geometry-fp64")
     assert ch.b_format() == ("compact-fp64", "compact-fp64")
+
+
+def test_transfer_geometry_errors_and_empty_mesh(gpu):
+    """ipf_transfer_geometry rejects an unknown shape index, a non-finite
+    corner, a shape without a kernel and two shapes of equal sizes, each with
+    IPF_EINVAL / IPF_EUNSUPPORTED and a message; a mesh without elements
+    transfers and its residual is f_lin."""
+    import torch
+    m = _mesh(MESHES[1])
+    (nip, nsh), tab = m.shape_table()
+    sm = ipf.StructMatrix(m, device=-1)
+    s = sm.sizes()
+    ix = sm.export_indexes()
+    corners = m.corners()
+    nb = corners.shape[0]
+    args = dict(shape_sizes=[(nip, nsh)], shape_tables=tab, elem_shape=np.zeros(nb, np.int32), corners=corners,
+                dof_indexes=ix["dof_indexes"], color_indexes=ix["color_indexes"], ncolor=s["ncolor"], neq=s["neq"])
+    ctx = ipf.DeviceContext(device=0)
+    ctx.transfer_geometry(**args)  # the good input
+    bad = dict(args, elem_shape=np.full(nb, 1, np.int32))
+    with pytest.raises(ipf.IPFError, match="shape 1 not in"):
+        ctx.transfer_geometry(**bad)
+    c2 = corners.copy()
+    c2[3, 5] = np.nan
+    with pytest.raises(ipf.IPFError, match="not finite"):
+        ctx.transfer_geometry(**dict(args, corners=c2))
+    with pytest.raises(ipf.IPFError, match="no kernel"):
+        ctx.transfer_geometry(**dict(args, shape_sizes=[(5, 5)], shape_tables=np.zeros(5 * 13)))
+    with pytest.raises(ipf.IPFError, match="same sizes"):
+        ctx.transfer_geometry(**dict(args, shape_sizes=[(nip, nsh), (nip, nsh)], shape_tables=np.concatenate([tab, tab])))
+    # no elements: the residual is f_lin
+    e = ipf.DeviceContext(device=0)
+    e.transfer_geometry([(nip, nsh)], tab, np.zeros(0, np.int32), np.zeros((0, 8)), np.zeros(0, np.int32),
+                        np.zeros(0, np.int32), 1, 4)
+    e.set_material(1.0, 1.0, 0.5, 0.5, 1.0)
+    f = np.array([1.0, -2.0, 3.0, 0.5])
+    e.set_rhs_linear(f)
+    r = torch.zeros(4, dtype=torch.float64, device=gpu)
+    e.residual(torch.zeros(4, dtype=torch.float64, device=gpu), r)
+    e.synchronize()
+    assert np.array_equal(r.cpu().numpy(), f)
