@@ -200,21 +200,58 @@ __device__ __forceinline__ void reduce_cols(const typename FC<SHAPE, FMT>::T* L,
     }
 }
 
+// reduce_cols with one column sum per lane (ET * W <= 64 for every shape):
+// element e, column J0 + jj, DoF d, rv = r[d] read by the caller (before the
+// return map or here).
+template <int SHAPE, int FMT, int W>
+__device__ __forceinline__ void reduce_lane(const typename FC<SHAPE, FMT>::T* L, int e, int jj, int d, double rv,
+                                            double* __restrict__ r) {
+    using C = FC<SHAPE, FMT>;
+    using T = typename C::T;
+    T f = L[(e * C::NIP) * W + jj];
+#pragma unroll
+    for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e * C::NIP + ip) * W + jj];
+    r[d] = rv + (-double(f));
+}
+
+// Memory requests leave each wave early so that their latency overlaps the
+// geometry and the return map: the point's committed state before the
+// gather, and (R_EARLY) the r values of the lane's column sums right after it
+// -- the color's elements share no DoF, so no other wave of the launch writes
+// them.  R_EARLY costs 7 VGPRs; it is on where that keeps the waves per SIMD
+// (Q3: LDS already limits them to 4), off where it would cost one wave
+// (measured slower at Q2 on MI355X).
 template <int SHAPE, int FMT, bool COMMIT>
 __global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
 k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ u, double* __restrict__ r,
                PointState st) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
+    constexpr int H = C::CE / 2;
+    static_assert(C::ET * H <= kWave, "one column sum per lane");
+    constexpr bool R_EARLY = SHAPE == SHAPE_Q3 && !COMMIT;
     __shared__ T lds[C::TP * (C::CE - C::CE / 2)];
     __shared__ double us[C::ET * C::CE];
     __shared__ int dsx[C::ET * C::CE];
     const int lane = threadIdx.x;
     const int64_t tile = blockIdx.x;
     const int el = lane / C::NIP;
-    const int64_t elem = tile * C::ET + el;
     const bool slot = lane < C::TP;
-    const bool valid = slot && el < tile_nvalid(g, tile);
+    const int nv = tile_nvalid(g, tile);
+    const bool valid = slot && el < nv;
+    const int64_t P = g.pt0 + tile * C::TP + lane;
+    double ep[4] = {0.0, 0.0, 0.0, 0.0};
+    double alpha = 0.0, wst = 0.0;
+    if (valid) {
+        ep[0] = ld_stream(&st.ep0[P]);
+        ep[1] = ld_stream(&st.ep1[P]);
+        ep[2] = ld_stream(&st.ep2[P]);
+        ep[3] = ld_stream(&st.ep3[P]);
+        // alpha only accumulates the plastic increment (H = 0: it does not
+        // enter the stress); without a commit it is not needed
+        if (COMMIT) alpha = ld_stream(&st.alpha[P]);
+        if (!FmtTraits<FMT>::GEOM) wst = ld_stream(&st.w[P]);
+    }
     T b[C::NV];
     double wgeo = 0.0;
     if constexpr (FmtTraits<FMT>::GEOM) {
@@ -233,7 +270,6 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     // the tile's DoF indexes and u values, gathered once per element into LDS
     // (the lanes of an element read them from there: no per-lane copy of the
     // element's u in registers, no repeated gathers)
-    const int nv = tile_nvalid(g, tile);
     for (int t = lane; t < C::ET * C::CE; t += kWave) {
         if (t / C::CE < nv) {
             const int d = ld_stream(&g.dof[tile * (int64_t)(C::ET * C::CE) + t]);
@@ -242,17 +278,20 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
         }
     }
     __syncthreads();
+    // the lane's column sums: columns rj and H + rj of element re
+    const int re = lane / H, rj = lane - re * H;
+    const bool task = lane < C::ET * H && re < nv;
+    double r0 = 0.0, r1 = 0.0;
+    if (R_EARLY && task) {
+        r0 = r[dsx[re * C::CE + rj]];
+        r1 = r[dsx[re * C::CE + H + rj]];
+    }
     double e3[3] = {0.0, 0.0, 0.0};
     if (valid) strain_fmt<SHAPE, FMT>(b, us + el * C::CE, e3);
     double ws[3] = {0.0, 0.0, 0.0};
     if (valid) {
-        const int64_t P = g.pt0 + tile * C::TP + lane;
-        double ep[4] = {ld_stream(&st.ep0[P]), ld_stream(&st.ep1[P]), ld_stream(&st.ep2[P]), ld_stream(&st.ep3[P])};
-        // alpha only accumulates the plastic increment (H = 0: it does not
-        // enter the stress); without a commit it is not needed
-        double alpha = COMMIT ? ld_stream(&st.alpha[P]) : 0.0;
         int mt, br;
-        const double w = FmtTraits<FMT>::GEOM ? wgeo : ld_stream(&st.w[P]);
+        const double w = FmtTraits<FMT>::GEOM ? wgeo : wst;
         point_sigma<false>(e3, ep, alpha, mt, br, tile_material(g, tile), w, ws, nullptr);
         if (COMMIT) {
             st.ep0[P] = ep[0];
@@ -264,14 +303,19 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
             st.branch[P] = (signed char)br;
         }
     }
-    constexpr int H = C::CE / 2;
     if (slot) bt_cols<SHAPE, FMT, 0, H>(b, ws, lds + lane * H);
     __syncthreads();
-    reduce_cols<SHAPE, FMT, 0, H>(lds, dsx, nv, lane, r);
+    if (task) {
+        const int d = dsx[re * C::CE + rj];
+        reduce_lane<SHAPE, FMT, H>(lds, re, rj, d, R_EARLY ? r0 : r[d], r);
+    }
     __syncthreads();
     if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, ws, lds + lane * (C::CE - H));
     __syncthreads();
-    reduce_cols<SHAPE, FMT, H, C::CE>(lds, dsx, nv, lane, r);
+    if (task) {
+        const int d = dsx[re * C::CE + H + rj];
+        reduce_lane<SHAPE, FMT, C::CE - H>(lds, re, rj, d, R_EARLY ? r1 : r[d], r);
+    }
 }
 
 // y += sum_e B_e^T (w Dep) B_e v_e with B in format FMT (FP32 formats: FP32
