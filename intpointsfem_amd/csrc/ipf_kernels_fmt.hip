@@ -177,32 +177,12 @@ __device__ __forceinline__ void bt_cols(const typename FC<SHAPE, FMT>::T* b, con
     }
 }
 
-// Per element and column: the sum of its points' partials in point order,
-// added negated to r (TPZNumericalIntegrator::MultiplyTranspose's
-// y = -B^T x; the color's elements share no DoF, so no atomics).  dsx holds
-// the tile's DoF indexes (staged in LDS by the kernel), nv its valid elements.
-template <int SHAPE, int FMT, int J0, int J1>
-__device__ __forceinline__ void reduce_cols(const typename FC<SHAPE, FMT>::T* L, const int* dsx, int nv, int lane,
-                                              double* __restrict__ r) {
-    using C = FC<SHAPE, FMT>;
-    using T = typename C::T;
-    constexpr int W = J1 - J0;
-    for (int task = lane; task < C::ET * W; task += kWave) {
-        const int e2 = task / W;
-        const int jj = task - e2 * W;
-        if (e2 < nv) {
-            T f = L[(e2 * C::NIP) * W + jj];
-#pragma unroll
-            for (int ip = 1; ip < C::NIP; ip++) f = f + L[(e2 * C::NIP + ip) * W + jj];
-            const int d = dsx[e2 * C::CE + J0 + jj];
-            r[d] = r[d] + (-double(f));
-        }
-    }
-}
-
-// reduce_cols with one column sum per lane (ET * W <= 64 for every shape):
-// element e, column J0 + jj, DoF d, rv = r[d] read by the caller (before the
-// return map or here).
+// One lane's column sum: the partials L of element e's points in column jj of
+// the staged half (W columns), summed in point order and added negated to
+// r[d] (TPZNumericalIntegrator::MultiplyTranspose's y = -B^T x); rv = r[d],
+// read by the caller ahead of time.  Each lane owns one column sum per half
+// (ET * W <= 64 for every shape), and the color's elements share no DoF, so
+// no atomics.
 template <int SHAPE, int FMT, int W>
 __device__ __forceinline__ void reduce_lane(const typename FC<SHAPE, FMT>::T* L, int e, int jj, int d, double rv,
                                             double* __restrict__ r) {
@@ -215,12 +195,14 @@ __device__ __forceinline__ void reduce_lane(const typename FC<SHAPE, FMT>::T* L,
 }
 
 // Memory requests leave each wave early so that their latency overlaps the
-// geometry and the return map: the point's committed state before the
-// gather, and (R_EARLY) the r values of the lane's column sums right after it
-// -- the color's elements share no DoF, so no other wave of the launch writes
-// them.  R_EARLY costs 7 VGPRs; it is on where that keeps the waves per SIMD
-// (Q3: LDS already limits them to 4), off where it would cost one wave
-// (measured slower at Q2 on MI355X).
+// geometry and the return map: the tile's DoF indexes and the point's
+// committed state first, the u values as soon as the indexes arrive (one
+// round trip before the gather completes, not two per gather round), and the
+// r values of the lane's column sums right after the gather (R_EARLY) or
+// both right after the return map -- the color's elements share no DoF, so
+// no other wave of the launch writes them.  R_EARLY costs 7 VGPRs; it is on
+// where that keeps the waves per SIMD (Q3: LDS already limits them to 4), off
+// where it would cost one wave (measured slower at Q2 on MI355X).
 template <int SHAPE, int FMT, bool COMMIT>
 __global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
 k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ u, double* __restrict__ r,
@@ -239,19 +221,29 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     const bool slot = lane < C::TP;
     const int nv = tile_nvalid(g, tile);
     const bool valid = slot && el < nv;
-    const int64_t P = g.pt0 + tile * C::TP + lane;
-    double ep[4] = {0.0, 0.0, 0.0, 0.0};
-    double alpha = 0.0, wst = 0.0;
-    if (valid) {
-        ep[0] = ld_stream(&st.ep0[P]);
-        ep[1] = ld_stream(&st.ep1[P]);
-        ep[2] = ld_stream(&st.ep2[P]);
-        ep[3] = ld_stream(&st.ep3[P]);
-        // alpha only accumulates the plastic increment (H = 0: it does not
-        // enter the stress); without a commit it is not needed
-        if (COMMIT) alpha = ld_stream(&st.alpha[P]);
-        if (!FmtTraits<FMT>::GEOM) wst = ld_stream(&st.w[P]);
+    // The tile's DoF indexes (NG per lane), the point's state, then the u
+    // values the indexes address -- loads outside branches, so that each use
+    // waits only for its own requests.  Every address is in bounds: padding
+    // slots of a tile hold DoF 0 and points of their own, lanes past the
+    // tile's points read its first point, and what invalid lanes load is
+    // never used.
+    constexpr int NG = (C::ET * C::CE + kWave - 1) / kWave;
+    static_assert(C::ET * C::CE >= kWave, "the first gather round fills the wave");
+    int dl[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) {
+        const int t = min(lane + k * kWave, C::ET * C::CE - 1);
+        dl[k] = ld_stream(&g.dof[tile * (int64_t)(C::ET * C::CE) + t]);
     }
+    const int64_t P = g.pt0 + tile * C::TP + (slot ? lane : 0);
+    double ep[4] = {ld_stream(&st.ep0[P]), ld_stream(&st.ep1[P]), ld_stream(&st.ep2[P]), ld_stream(&st.ep3[P])};
+    // alpha only accumulates the plastic increment (H = 0: it does not enter
+    // the stress); without a commit it is not needed
+    double alpha = COMMIT ? ld_stream(&st.alpha[P]) : 0.0;
+    const double wst = FmtTraits<FMT>::GEOM ? 0.0 : ld_stream(&st.w[P]);
+    double ul[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) ul[k] = u[dl[k]];
     T b[C::NV];
     double wgeo = 0.0;
     if constexpr (FmtTraits<FMT>::GEOM) {
@@ -270,11 +262,11 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     // the tile's DoF indexes and u values, gathered once per element into LDS
     // (the lanes of an element read them from there: no per-lane copy of the
     // element's u in registers, no repeated gathers)
-    for (int t = lane; t < C::ET * C::CE; t += kWave) {
-        if (t / C::CE < nv) {
-            const int d = ld_stream(&g.dof[tile * (int64_t)(C::ET * C::CE) + t]);
-            dsx[t] = d;
-            us[t] = u[d];
+#pragma unroll
+    for (int k = 0; k < NG; k++) {
+        if (k == 0 || lane + k * kWave < C::ET * C::CE) {
+            dsx[lane + k * kWave] = dl[k];
+            us[lane + k * kWave] = ul[k];
         }
     }
     __syncthreads();
@@ -303,19 +295,18 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
             st.branch[P] = (signed char)br;
         }
     }
+    // otherwise both r values now, in flight during the first half's products
+    if (!R_EARLY && task) {
+        r0 = r[dsx[re * C::CE + rj]];
+        r1 = r[dsx[re * C::CE + H + rj]];
+    }
     if (slot) bt_cols<SHAPE, FMT, 0, H>(b, ws, lds + lane * H);
     __syncthreads();
-    if (task) {
-        const int d = dsx[re * C::CE + rj];
-        reduce_lane<SHAPE, FMT, H>(lds, re, rj, d, R_EARLY ? r0 : r[d], r);
-    }
+    if (task) reduce_lane<SHAPE, FMT, H>(lds, re, rj, dsx[re * C::CE + rj], r0, r);
     __syncthreads();
     if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, ws, lds + lane * (C::CE - H));
     __syncthreads();
-    if (task) {
-        const int d = dsx[re * C::CE + H + rj];
-        reduce_lane<SHAPE, FMT, C::CE - H>(lds, re, rj, d, R_EARLY ? r1 : r[d], r);
-    }
+    if (task) reduce_lane<SHAPE, FMT, C::CE - H>(lds, re, rj, dsx[re * C::CE + H + rj], r1, r);
 }
 
 // y += sum_e B_e^T (w Dep) B_e v_e with B in format FMT (FP32 formats: FP32
@@ -326,15 +317,30 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
               const double* __restrict__ dep, int64_t npad) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
+    constexpr int H = C::CE / 2;
+    static_assert(C::ET * H <= kWave, "one column sum per lane");
     __shared__ T lds[C::TP * (C::CE - C::CE / 2)];
     __shared__ double vs[C::ET * C::CE];
     __shared__ int dsx[C::ET * C::CE];
     const int lane = threadIdx.x;
     const int64_t tile = blockIdx.x;
     const int el = lane / C::NIP;
-    const int64_t elem = tile * C::ET + el;
     const bool slot = lane < C::TP;
-    const bool valid = slot && el < tile_nvalid(g, tile);
+    const int nv = tile_nvalid(g, tile);
+    const bool valid = slot && el < nv;
+    // DoF indexes, then the v values they address, outside branches (the
+    // addresses are in bounds as in k_residual_fmt)
+    constexpr int NG = (C::ET * C::CE + kWave - 1) / kWave;
+    static_assert(C::ET * C::CE >= kWave, "the first gather round fills the wave");
+    int dl[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) {
+        const int t = min(lane + k * kWave, C::ET * C::CE - 1);
+        dl[k] = ld_stream(&g.dof[tile * (int64_t)(C::ET * C::CE) + t]);
+    }
+    double vl[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) vl[k] = v[dl[k]];
     T b[C::NV];
     if constexpr (FmtTraits<FMT>::GEOM) {
         if (valid) {
@@ -349,36 +355,52 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
 #pragma unroll
         for (int k = 0; k < C::NV; k++) b[k] = T(0);
     }
-    const int nv = tile_nvalid(g, tile);
-    for (int t = lane; t < C::ET * C::CE; t += kWave) {
-        if (t / C::CE < nv) {
-            const int d = ld_stream(&g.dof[tile * (int64_t)(C::ET * C::CE) + t]);
-            dsx[t] = d;
-            vs[t] = v[d];
+#pragma unroll
+    for (int k = 0; k < NG; k++) {
+        if (k == 0 || lane + k * kWave < C::ET * C::CE) {
+            dsx[lane + k * kWave] = dl[k];
+            vs[lane + k * kWave] = vl[k];
         }
     }
     __syncthreads();
+    // the point's 9 tangent values and the lane's two y values (the color's
+    // elements share no DoF), requested before the strain -- except next to
+    // dense FP64 B of up to 18 columns, whose 3 CE values already fill the
+    // registers (measured: spills, slower); there they follow the strain
+    constexpr bool EARLY = !(FMT == FMT_D64 && C::CE <= 18);
+    const int64_t P = g.pt0 + tile * C::TP + (slot ? lane : 0);
+    const int re = lane / H, rj = lane - re * H;
+    const bool task = lane < C::ET * H && re < nv;
+    double dp[9], y0 = 0.0, y1 = 0.0;
+    auto request = [&]() {
+#pragma unroll
+        for (int k = 0; k < 9; k++) dp[k] = ld_stream(&dep[k * npad + P]);
+        if (task) {
+            y0 = y[dsx[re * C::CE + rj]];
+            y1 = y[dsx[re * C::CE + H + rj]];
+        }
+    };
+    if constexpr (EARLY) request();
     double e3[3] = {0.0, 0.0, 0.0};
     if (valid) strain_fmt<SHAPE, FMT>(b, vs + el * C::CE, e3);
+    if constexpr (!EARLY) request();
     double x[3] = {0.0, 0.0, 0.0};
     if (valid) {
-        const int64_t P = g.pt0 + tile * C::TP + lane;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            double a = ld_stream(&dep[(3 * k) * npad + P]) * e3[0];
-            a = a + ld_stream(&dep[(3 * k + 1) * npad + P]) * e3[1];
-            a = a + ld_stream(&dep[(3 * k + 2) * npad + P]) * e3[2];
-            x[k] = -a;  // reduce_cols adds -(sum); y accumulates +K v
+            double a = dp[3 * k] * e3[0];
+            a = a + dp[3 * k + 1] * e3[1];
+            a = a + dp[3 * k + 2] * e3[2];
+            x[k] = -a;  // reduce_lane adds -(sum); y accumulates +K v
         }
     }
-    constexpr int H = C::CE / 2;
     if (slot) bt_cols<SHAPE, FMT, 0, H>(b, x, lds + lane * H);
     __syncthreads();
-    reduce_cols<SHAPE, FMT, 0, H>(lds, dsx, nv, lane, y);
+    if (task) reduce_lane<SHAPE, FMT, H>(lds, re, rj, dsx[re * C::CE + rj], y0, y);
     __syncthreads();
     if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, x, lds + lane * (C::CE - H));
     __syncthreads();
-    reduce_cols<SHAPE, FMT, H, C::CE>(lds, dsx, nv, lane, y);
+    if (task) reduce_lane<SHAPE, FMT, C::CE - H>(lds, re, rj, dsx[re * C::CE + H + rj], y1, y);
 }
 
 // Builds format FMT from the dense FP64 tile layout of one group (device side).
