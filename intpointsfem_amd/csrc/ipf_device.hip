@@ -194,6 +194,11 @@ struct ipf_ctx {
     signed char* dBranch = nullptr;
     int64_t* dOrig = nullptr;   // orig point of each new slot (-1 for padding)
     double* dFlin = nullptr;
+    // dDof with kFirstWrite on each DoF's first write in launch order, for the
+    // residual launches; fold_flin: every DoF has one, so those launches start
+    // r from f_lin themselves and need no copy
+    int* dDofR = nullptr;
+    bool fold_flin = false;
     double* dU = nullptr;       // staging for host-pointer calls
     double* dR = nullptr;
     int64_t bytes = 0;
@@ -298,6 +303,8 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dBranch);
     dev_free(ctx->dOrig);
     dev_free(ctx->dFlin);
+    dev_free(ctx->dDofR);
+    ctx->fold_flin = false;
     dev_free(ctx->dU);
     dev_free(ctx->dR);
     dev_free(ctx->dDep);
@@ -411,10 +418,23 @@ const void* group_fmt_B(const ipf_ctx* ctx, const Group& g, int fmt) {
     return static_cast<const char*>(ctx->dBf[fmt]) + format_bytes(fmt) * (size_t)g.f_off[fmt];
 }
 
-// One residual launch for a group in the selected B format.
-void launch_group_residual(ipf_ctx* ctx, const Group& g, const double* u, double* r, const PointState& st) {
-    launch_residual_fmt(ctx->rfmt, g.shape, ctx->update_mem, group_dev(ctx, g), group_fmt_B(ctx, g, ctx->rfmt), u, r,
-                        st, ctx->stream);
+// One residual launch for a group in the selected B format (flin: see
+// residual_start).
+void launch_group_residual(ipf_ctx* ctx, const Group& g, const double* u, double* r, const double* flin,
+                           const PointState& st) {
+    GroupDev d = group_dev(ctx, g);
+    if (flin) d.dof = ctx->dDofR + g.dof_off;
+    launch_residual_fmt(ctx->rfmt, g.shape, ctx->update_mem, d, group_fmt_B(ctx, g, ctx->rfmt), u, r, flin, st,
+                        ctx->stream);
+}
+
+// r = f_lin (TPZElastoPlasticIntPointsStructMatrix.cpp:147) before the
+// residual launches: folded into them when every DoF has a first write
+// (returns f_lin for launch_group_residual), else a copy (returns null).
+const double* residual_start(ipf_ctx* ctx, double* r) {
+    if (ctx->fold_flin) return ctx->dFlin;
+    launch_copy(ctx->neq, ctx->dFlin, r, ctx->stream);
+    return nullptr;
 }
 
 // One tangent (K v) launch for a group in the selected B format.
@@ -799,6 +819,31 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
         }
     }
 
+    // --- first writes: in launch order (groups in order; inside a color each
+    // DoF belongs to one element), the entry of each DoF that is written
+    // first starts from f_lin; when some DoF belongs to no element the
+    // residual keeps the copy r = f_lin
+    std::vector<int> hDofR(hDof);
+    bool fold = false;
+    {
+        std::vector<unsigned char> seen(neq, 0);
+        int64_t touched = 0;
+        for (const auto& g : groups) {
+            const int CE = shape_ce(g.shape), ET = shape_et(g.shape);
+            for (int64_t t = 0; t < g.ntiles; t++) {
+                const int nvt = g.tmeta[t] & 255;
+                for (int64_t i = g.dof_off + t * ET * CE; i < g.dof_off + (t * ET + nvt) * CE; i++) {
+                    const int d = hDof[i];
+                    if (seen[d]) continue;
+                    seen[d] = 1;
+                    touched++;
+                    hDofR[i] = d | kFirstWrite;
+                }
+            }
+        }
+        fold = touched == neq;
+    }
+
     // --- device buffers
     int rc;
     if ((rc = dev_alloc(ctx, &ctx->dB, nb)) || (rc = dev_alloc(ctx, &ctx->dDof, nd)) ||
@@ -810,7 +855,8 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
         (rc = dev_alloc(ctx, &ctx->dOrig, npad)) || (rc = dev_alloc(ctx, &ctx->dFlin, neq)) ||
         (rc = dev_alloc(ctx, &ctx->dU, neq)) || (rc = dev_alloc(ctx, &ctx->dR, neq)) ||
         (rc = dev_alloc(ctx, &ctx->dTmeta, ntile_all)) || (rc = dev_alloc(ctx, &ctx->dNewOf, npts)) ||
-        (rc = dev_alloc(ctx, &ctx->dPmat, npad))) {
+        (rc = dev_alloc(ctx, &ctx->dPmat, npad)) ||
+        (fold && (rc = dev_alloc(ctx, &ctx->dDofR, nd)))) {
         release_data(ctx);
         return rc;
     }
@@ -843,7 +889,8 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
         (rc = zero(ctx->dEp[0], sizeof(double) * npad)) || (rc = zero(ctx->dEp[1], sizeof(double) * npad)) ||
         (rc = zero(ctx->dEp[2], sizeof(double) * npad)) || (rc = zero(ctx->dEp[3], sizeof(double) * npad)) ||
         (rc = zero(ctx->dAlpha, sizeof(double) * npad)) || (rc = zero(ctx->dMtype, npad)) ||
-        (rc = zero(ctx->dBranch, npad)) || (rc = zero(ctx->dFlin, sizeof(double) * neq))) {
+        (rc = zero(ctx->dBranch, npad)) || (rc = zero(ctx->dFlin, sizeof(double) * neq)) ||
+        (fold && (rc = up(ctx->dDofR, hDofR.data(), sizeof(int) * nd)))) {
         release_data(ctx);
         return rc;
     }
@@ -877,6 +924,7 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
     ctx->nstorage = nblocks ? matrix_position[nblocks] : 0;
     ctx->npad = npad;
     ctx->max_mat = max_mat;
+    ctx->fold_flin = fold;
     ctx->groups = std::move(groups);
     ctx->compact_ok = compact_ok;
     ctx->rfmt = ctx->tfmt = FMT_D64;
@@ -1044,11 +1092,11 @@ int ipf_residual(ipf_ctx* ctx, const double* u_dev, double* r_dev) {
     if (rc) return rc;
     if (!u_dev || !r_dev) return fail(ctx, IPF_EINVAL, "ipf_residual: NULL pointer");
     IPF_HIP(ctx, hipSetDevice(ctx->device));
-    // r = f_lin (TPZElastoPlasticIntPointsStructMatrix.cpp:147), then each color adds -B^T(w sigma)
-    launch_copy(ctx->neq, ctx->dFlin, r_dev, ctx->stream);
+    // r = f_lin, then each color adds -B^T(w sigma)
+    const double* flin = residual_start(ctx, r_dev);
     const PointState st = point_state(ctx);
     for (const auto& g : ctx->groups)
-        launch_group_residual(ctx, g, u_dev, r_dev, st);
+        launch_group_residual(ctx, g, u_dev, r_dev, flin, st);
     return launch_status(ctx, "residual");
 }
 
@@ -1064,11 +1112,11 @@ int ipf_residual_timed(ipf_ctx* ctx, const double* u_dev, double* r_dev, double*
     for (auto& e : ev) IPF_HIP(ctx, hipEventCreate(&e));
     const PointState st = point_state(ctx);
     IPF_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
-    launch_copy(ctx->neq, ctx->dFlin, r_dev, ctx->stream);
+    const double* flin = residual_start(ctx, r_dev);  // launch 0: the copy, or nothing when folded
     IPF_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
     for (size_t i = 0; i < ctx->groups.size(); i++) {
         const auto& g = ctx->groups[i];
-        launch_group_residual(ctx, g, u_dev, r_dev, st);
+        launch_group_residual(ctx, g, u_dev, r_dev, flin, st);
         IPF_HIP(ctx, hipEventRecord(ev[i + 2], ctx->stream));
     }
     rc = launch_status(ctx, "residual");
@@ -1254,8 +1302,9 @@ int ipf_autotune(ipf_ctx* ctx, const double* u_dev, int reps, int allow_fp32, do
             ctx->rfmt = f;
             rc = time_it(
                 [&] {
-                    launch_copy(ctx->neq, ctx->dFlin, ctx->dR, ctx->stream);
-                    for (const auto& g : ctx->groups) launch_group_residual(ctx, g, u_dev, ctx->dR, point_state(ctx));
+                    const double* flin = residual_start(ctx, ctx->dR);
+                    for (const auto& g : ctx->groups)
+                        launch_group_residual(ctx, g, u_dev, ctx->dR, flin, point_state(ctx));
                 },
                 &ms);
             if (rc) break;

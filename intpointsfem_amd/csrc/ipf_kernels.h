@@ -37,6 +37,11 @@ struct GroupDev {
     const double* tab;     // shape table of the group's shape (ipf_geometry.h layout), or null
 };
 
+// DoF index entries of the residual launches may carry kFirstWrite: the entry
+// is its DoF's first write in launch order (see launch_residual_fmt).
+constexpr int kFirstWrite = int(0x80000000u);
+constexpr int kDofMask = 0x7fffffff;
+
 __device__ __forceinline__ int tile_nvalid(const GroupDev& g, int64_t tile) { return g.tmeta[tile] & 255; }
 __device__ __forceinline__ const MatConst& tile_material(const GroupDev& g, int64_t tile) {
     return g.mats[g.tmeta[tile] >> 8];
@@ -132,8 +137,11 @@ void launch_assemble_K(int shape, const GroupDev& g, const double* dep_soa, int6
 // at the group's first tile in format fmt (FMT_D64: the dense tiles g.B;
 // FMT_C64, FMT_D32, FMT_C32: the converted copies), or for FMT_G64 at the
 // corners of the group's first element slot (with g.tab set).
+// With flin set, g.dof's entries flagged kFirstWrite start from flin[d]
+// instead of r[d] (each DoF's first write in launch order): r = flin then
+// needs no copy before the launches.
 void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
-                         double* r, const PointState& st, hipStream_t stream);
+                         double* r, const double* flin, const PointState& st, hipStream_t stream);
 void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, const double* v, double* y,
                         const double* dep_soa, int64_t npad, hipStream_t stream);
 // dense FP64 tiles of one group -> format fmt

@@ -206,7 +206,7 @@ __device__ __forceinline__ void reduce_lane(const typename FC<SHAPE, FMT>::T* L,
 template <int SHAPE, int FMT, bool COMMIT>
 __global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
 k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ u, double* __restrict__ r,
-               PointState st) {
+               const double* __restrict__ flin, PointState st) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
     constexpr int H = C::CE / 2;
@@ -243,7 +243,7 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     const double wst = FmtTraits<FMT>::GEOM ? 0.0 : ld_stream(&st.w[P]);
     double ul[NG];
 #pragma unroll
-    for (int k = 0; k < NG; k++) ul[k] = u[dl[k]];
+    for (int k = 0; k < NG; k++) ul[k] = u[dl[k] & kDofMask];
     T b[C::NV];
     double wgeo = 0.0;
     if constexpr (FmtTraits<FMT>::GEOM) {
@@ -270,13 +270,15 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
         }
     }
     __syncthreads();
-    // the lane's column sums: columns rj and H + rj of element re
+    // the lane's column sums: columns rj and H + rj of element re; an entry
+    // flagged kFirstWrite (flin set) starts from f_lin instead of r
     const int re = lane / H, rj = lane - re * H;
     const bool task = lane < C::ET * H && re < nv;
+    auto start = [&](int de) -> double { return (de < 0 ? flin : r)[de & kDofMask]; };
     double r0 = 0.0, r1 = 0.0;
     if (R_EARLY && task) {
-        r0 = r[dsx[re * C::CE + rj]];
-        r1 = r[dsx[re * C::CE + H + rj]];
+        r0 = start(dsx[re * C::CE + rj]);
+        r1 = start(dsx[re * C::CE + H + rj]);
     }
     double e3[3] = {0.0, 0.0, 0.0};
     if (valid) strain_fmt<SHAPE, FMT>(b, us + el * C::CE, e3);
@@ -297,16 +299,16 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     }
     // otherwise both r values now, in flight during the first half's products
     if (!R_EARLY && task) {
-        r0 = r[dsx[re * C::CE + rj]];
-        r1 = r[dsx[re * C::CE + H + rj]];
+        r0 = start(dsx[re * C::CE + rj]);
+        r1 = start(dsx[re * C::CE + H + rj]);
     }
     if (slot) bt_cols<SHAPE, FMT, 0, H>(b, ws, lds + lane * H);
     __syncthreads();
-    if (task) reduce_lane<SHAPE, FMT, H>(lds, re, rj, dsx[re * C::CE + rj], r0, r);
+    if (task) reduce_lane<SHAPE, FMT, H>(lds, re, rj, dsx[re * C::CE + rj] & kDofMask, r0, r);
     __syncthreads();
     if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, ws, lds + lane * (C::CE - H));
     __syncthreads();
-    if (task) reduce_lane<SHAPE, FMT, C::CE - H>(lds, re, rj, dsx[re * C::CE + H + rj], r1, r);
+    if (task) reduce_lane<SHAPE, FMT, C::CE - H>(lds, re, rj, dsx[re * C::CE + H + rj] & kDofMask, r1, r);
 }
 
 // y += sum_e B_e^T (w Dep) B_e v_e with B in format FMT (FP32 formats: FP32
@@ -466,18 +468,19 @@ __global__ void k_build_B(GroupDev g, const double* __restrict__ corners, double
 
 template <int SHAPE, int FMT>
 void residual_fmt_shape(bool commit, const GroupDev& g, const void* Bv, const double* u, double* r,
-                        const PointState& st, hipStream_t s) {
+                        const double* flin, const PointState& st, hipStream_t s) {
     const dim3 grid((unsigned)g.ntiles), block(kWave);
     if (commit)
-        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, true>), grid, block, 0, s, g, Bv, u, r, st);
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, true>), grid, block, 0, s, g, Bv, u, r, flin, st);
     else
-        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false>), grid, block, 0, s, g, Bv, u, r, st);
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false>), grid, block, 0, s, g, Bv, u, r, flin, st);
 }
 
 template <int FMT>
 void residual_fmt(int shape, bool commit, const GroupDev& g, const void* Bv, const double* u, double* r,
-                  const PointState& st, hipStream_t s) {
-    dispatch_shape(shape, [&](auto sc) { residual_fmt_shape<decltype(sc)::value, FMT>(commit, g, Bv, u, r, st, s); });
+                  const double* flin, const PointState& st, hipStream_t s) {
+    dispatch_shape(shape,
+                   [&](auto sc) { residual_fmt_shape<decltype(sc)::value, FMT>(commit, g, Bv, u, r, flin, st, s); });
 }
 
 template <int FMT>
@@ -502,14 +505,14 @@ void convert_fmt(int shape, const double* dense, void* out, int64_t ntiles, hipS
 }  // namespace
 
 void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
-                         double* r, const PointState& st, hipStream_t s) {
+                         double* r, const double* flin, const PointState& st, hipStream_t s) {
     if (g.ntiles == 0) return;
     switch (fmt) {
-        case FMT_D64: residual_fmt<FMT_D64>(shape, commit, g, Bv, u, r, st, s); break;
-        case FMT_C64: residual_fmt<FMT_C64>(shape, commit, g, Bv, u, r, st, s); break;
-        case FMT_G64: residual_fmt<FMT_G64>(shape, commit, g, Bv, u, r, st, s); break;
-        case FMT_D32: residual_fmt<FMT_D32>(shape, commit, g, Bv, u, r, st, s); break;
-        case FMT_C32: residual_fmt<FMT_C32>(shape, commit, g, Bv, u, r, st, s); break;
+        case FMT_D64: residual_fmt<FMT_D64>(shape, commit, g, Bv, u, r, flin, st, s); break;
+        case FMT_C64: residual_fmt<FMT_C64>(shape, commit, g, Bv, u, r, flin, st, s); break;
+        case FMT_G64: residual_fmt<FMT_G64>(shape, commit, g, Bv, u, r, flin, st, s); break;
+        case FMT_D32: residual_fmt<FMT_D32>(shape, commit, g, Bv, u, r, flin, st, s); break;
+        case FMT_C32: residual_fmt<FMT_C32>(shape, commit, g, Bv, u, r, flin, st, s); break;
         default: break;
     }
 }
