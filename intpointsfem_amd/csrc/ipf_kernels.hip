@@ -135,7 +135,6 @@ k_transpose(GroupDev g, const double* __restrict__ x, double* __restrict__ r, co
     const int64_t tile = (int64_t)blockIdx.x * C::WPB + wave;
     const bool tile_ok = tile < g.ntiles;
     const int el = lane / C::NIP;
-    const int64_t elem = tile * C::ET + el;
     const bool slot = tile_ok && lane < C::TP;
     const bool valid = slot && el < tile_nvalid(g, tile);
     double ws[3] = {0.0, 0.0, 0.0};
@@ -211,7 +210,6 @@ k_tangent_diag(GroupDev g, double* __restrict__ diag, const double* __restrict__
     const int64_t tile = (int64_t)blockIdx.x * C::WPB + wave;
     const bool tile_ok = tile < g.ntiles;
     const int el = lane / C::NIP;
-    const int64_t elem = tile * C::ET + el;
     const bool slot = tile_ok && lane < C::TP;
     const bool valid = slot && el < tile_nvalid(g, tile);
     double* L = lds[wave];
