@@ -4,6 +4,8 @@
   * compact FP64 B, dense FP64 B and the geometric format (B recomputed from
     the element corners) give bit-identical residuals, branch flags and
     plastic strains (the compact products only skip exact zeros);
+  * repeated residuals from the committed state are bit-identical and do not
+    depend on what r held before the call;
   * the fused residual equals the reference's separate steps
     (Multiply -> ComputeSigma -> MultiplyTranspose + f_lin) to 1e-12 of
     max |f_int| (same per-element sums, same color order);
@@ -54,6 +56,17 @@ def test_production_mesh_properties(gpu, mesh):
     st_g = ctx.get_state()
     assert (r_c == r_g).all()
     assert all(np.array_equal(a, b) for a, b in zip(st_c, st_g))
+    # repeated residuals from the committed state give the same bits, whatever
+    # r held before (NaN here): no dependence on timing or on r's contents
+    ctx.set_state(st_c[0], st_c[1])
+    reps = []
+    for _ in range(4):
+        r = torch.full((neq,), float("nan"), dtype=torch.float64, device=gpu)
+        ctx.residual(u, r)
+        ctx.synchronize()
+        reps.append(r.cpu().numpy())
+    assert all(np.array_equal(reps[0], x) for x in reps[1:])
+    assert np.isfinite(reps[0]).all()
     ctx.set_b_format("compact-fp64")
     plastic = int((st_c[2] != 0).sum())
     assert plastic > 0
