@@ -147,7 +147,9 @@ int ipf_info(const ipf_ctx *ctx, int64_t out[8]);
  * (TPZElastoPlasticIntPointsStructMatrix.cpp:137-154) ->
  * TPZNumericalIntegrator::ResidualIntegration (TPZNumericalIntegrator.cpp:107-128).
  * One fused kernel per element color: gather, B.u, return map, B^T (w sigma)
- * and the color's atomic-free accumulation into r. */
+ * and the color's atomic-free accumulation into r.  r is overwritten
+ * entirely (its previous contents are never read); u and r must not
+ * overlap.  Asynchronous on the context's stream. */
 int ipf_residual(ipf_ctx *ctx, const double *u_dev, double *r_dev);
 int ipf_residual_host(ipf_ctx *ctx, const double *u_host, double *r_host);
 
