@@ -199,6 +199,9 @@ struct ipf_ctx {
     // r from f_lin themselves and need no copy
     int* dDofR = nullptr;
     bool fold_flin = false;
+    // per element slot and node, the node's x DoF (y = x + 1), flagged like
+    // dDofR, when every element's DoFs come in such pairs (GroupDev::node)
+    int* dNodeR = nullptr;
     double* dU = nullptr;       // staging for host-pointer calls
     double* dR = nullptr;
     int64_t bytes = 0;
@@ -305,6 +308,7 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dFlin);
     dev_free(ctx->dDofR);
     ctx->fold_flin = false;
+    dev_free(ctx->dNodeR);
     dev_free(ctx->dU);
     dev_free(ctx->dR);
     dev_free(ctx->dDep);
@@ -396,6 +400,7 @@ GroupDev group_dev(const ipf_ctx* ctx, const Group& g, bool columns = false) {
     d.ntiles = g.ntiles;
     d.pt0 = g.pt0;
     d.tab = (ctx->dTab && ctx->tab_off[g.shape] >= 0) ? ctx->dTab + ctx->tab_off[g.shape] : nullptr;
+    d.node = nullptr;
     return d;
 }
 
@@ -424,6 +429,7 @@ void launch_group_residual(ipf_ctx* ctx, const Group& g, const double* u, double
                            const PointState& st) {
     GroupDev d = group_dev(ctx, g);
     if (flin) d.dof = ctx->dDofR + g.dof_off;
+    if (flin && ctx->dNodeR) d.node = ctx->dNodeR + g.dof_off / 2;
     launch_residual_fmt(ctx->rfmt, g.shape, ctx->update_mem, d, group_fmt_B(ctx, g, ctx->rfmt), u, r, flin, st,
                         ctx->stream);
 }
@@ -843,6 +849,23 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
         }
         fold = touched == neq;
     }
+    // node pairs: every valid element slot numbers node a's DoFs (2a, 2a + 1)
+    // as (d, d + 1), whose first writes fall on the same entry pair; then the
+    // residual gathers one index per node
+    std::vector<int> hNodeR;
+    bool pairs = fold;
+    for (const auto& g : groups) {
+        const int CE = shape_ce(g.shape), ET = shape_et(g.shape);
+        for (int64_t t = 0; pairs && t < g.ntiles; t++) {
+            const int nvt = g.tmeta[t] & 255;
+            for (int64_t i = g.dof_off + t * ET * CE; pairs && i < g.dof_off + (t * ET + nvt) * CE; i += 2)
+                if (hDof[i + 1] != hDof[i] + 1 || (hDofR[i] < 0) != (hDofR[i + 1] < 0)) pairs = false;
+        }
+    }
+    if (pairs) {
+        hNodeR.resize(nd / 2);
+        for (int64_t i = 0; i < nd / 2; i++) hNodeR[i] = hDofR[2 * i];
+    }
 
     // --- device buffers
     int rc;
@@ -856,7 +879,7 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
         (rc = dev_alloc(ctx, &ctx->dU, neq)) || (rc = dev_alloc(ctx, &ctx->dR, neq)) ||
         (rc = dev_alloc(ctx, &ctx->dTmeta, ntile_all)) || (rc = dev_alloc(ctx, &ctx->dNewOf, npts)) ||
         (rc = dev_alloc(ctx, &ctx->dPmat, npad)) ||
-        (fold && (rc = dev_alloc(ctx, &ctx->dDofR, nd)))) {
+        (fold && (rc = dev_alloc(ctx, &ctx->dDofR, nd))) || (pairs && (rc = dev_alloc(ctx, &ctx->dNodeR, nd / 2)))) {
         release_data(ctx);
         return rc;
     }
@@ -890,7 +913,8 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
         (rc = zero(ctx->dEp[2], sizeof(double) * npad)) || (rc = zero(ctx->dEp[3], sizeof(double) * npad)) ||
         (rc = zero(ctx->dAlpha, sizeof(double) * npad)) || (rc = zero(ctx->dMtype, npad)) ||
         (rc = zero(ctx->dBranch, npad)) || (rc = zero(ctx->dFlin, sizeof(double) * neq)) ||
-        (fold && (rc = up(ctx->dDofR, hDofR.data(), sizeof(int) * nd)))) {
+        (fold && (rc = up(ctx->dDofR, hDofR.data(), sizeof(int) * nd))) ||
+        (pairs && (rc = up(ctx->dNodeR, hNodeR.data(), sizeof(int) * (nd / 2))))) {
         release_data(ctx);
         return rc;
     }

@@ -35,6 +35,11 @@ struct GroupDev {
     int64_t ntiles;
     int64_t pt0;           // new-point index of the group's first slot
     const double* tab;     // shape table of the group's shape (ipf_geometry.h layout), or null
+    // [nelem * CE / 2] per element slot and node, the node's x DoF (its y DoF
+    // is the next one), kFirstWrite-flagged like the residual's DoF copy; set
+    // for the non-committing residual when every element numbers its nodes'
+    // DoFs as consecutive (x, y) pairs, else null
+    const int* node;
 };
 
 // DoF index entries of the residual launches may carry kFirstWrite: the entry

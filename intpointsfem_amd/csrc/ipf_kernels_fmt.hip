@@ -203,37 +203,42 @@ __device__ __forceinline__ void reduce_lane(const typename FC<SHAPE, FMT>::T* L,
 // no other wave of the launch writes them.  R_EARLY costs 7 VGPRs; it is on
 // where that keeps the waves per SIMD (Q3: LDS already limits them to 4), off
 // where it would cost one wave (measured slower at Q2 on MI355X).
-template <int SHAPE, int FMT, bool COMMIT>
+template <int SHAPE, int FMT, bool COMMIT, bool PAIR>
 __global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
 k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ u, double* __restrict__ r,
                const double* __restrict__ flin, PointState st) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
-    constexpr int H = C::CE / 2;
-    static_assert(C::ET * H <= kWave, "one column sum per lane");
+    constexpr int NSH = C::CE / 2;
+    // staged halves of the columns: whole nodes with PAIR (the first half
+    // holds the first ceil(NSH / 2) nodes), else CE / 2 columns each
+    constexpr int H = PAIR ? 2 * ((NSH + 1) / 2) : C::CE / 2;
+    constexpr int H2 = C::CE - H;
+    static_assert(C::ET * NSH <= kWave, "one column sum (PAIR: node) per lane and half");
     constexpr bool R_EARLY = SHAPE == SHAPE_Q3 && !COMMIT;
-    __shared__ T lds[C::TP * (C::CE - C::CE / 2)];
+    constexpr int NE = PAIR ? C::ET * NSH : C::ET * C::CE;  // index entries per tile
+    __shared__ T lds[C::TP * H];
     __shared__ double us[C::ET * C::CE];
-    __shared__ int dsx[C::ET * C::CE];
+    __shared__ int dsx[NE];
     const int lane = threadIdx.x;
     const int64_t tile = blockIdx.x;
     const int el = lane / C::NIP;
     const bool slot = lane < C::TP;
     const int nv = tile_nvalid(g, tile);
     const bool valid = slot && el < nv;
-    // The tile's DoF indexes (NG per lane), the point's state, then the u
+    // The tile's DoF (PAIR: node) indexes, the point's state, then the u
     // values the indexes address -- loads outside branches, so that each use
     // waits only for its own requests.  Every address is in bounds: padding
-    // slots of a tile hold DoF 0 and points of their own, lanes past the
-    // tile's points read its first point, and what invalid lanes load is
-    // never used.
-    constexpr int NG = (C::ET * C::CE + kWave - 1) / kWave;
-    static_assert(C::ET * C::CE >= kWave, "the first gather round fills the wave");
+    // slots of a tile hold DoF 0 (PAIR: node DoFs 0 and 1) and points of
+    // their own, lanes past the tile's points read its first point, and what
+    // invalid lanes load is never used.
+    constexpr int NG = (NE + kWave - 1) / kWave;
+    static_assert(NE >= kWave || PAIR, "the first gather round fills the wave");
     int dl[NG];
 #pragma unroll
     for (int k = 0; k < NG; k++) {
-        const int t = min(lane + k * kWave, C::ET * C::CE - 1);
-        dl[k] = ld_stream(&g.dof[tile * (int64_t)(C::ET * C::CE) + t]);
+        const int t = min(lane + k * kWave, NE - 1);
+        dl[k] = ld_stream(&(PAIR ? g.node : g.dof)[tile * (int64_t)NE + t]);
     }
     const int64_t P = g.pt0 + tile * C::TP + (slot ? lane : 0);
     double ep[4] = {ld_stream(&st.ep0[P]), ld_stream(&st.ep1[P]), ld_stream(&st.ep2[P]), ld_stream(&st.ep3[P])};
@@ -241,9 +246,12 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     // the stress); without a commit it is not needed
     double alpha = COMMIT ? ld_stream(&st.alpha[P]) : 0.0;
     const double wst = FmtTraits<FMT>::GEOM ? 0.0 : ld_stream(&st.w[P]);
-    double ul[NG];
+    double ul[NG], vl[NG];
 #pragma unroll
-    for (int k = 0; k < NG; k++) ul[k] = u[dl[k] & kDofMask];
+    for (int k = 0; k < NG; k++) {
+        ul[k] = u[dl[k] & kDofMask];
+        vl[k] = PAIR ? u[(dl[k] & kDofMask) + 1] : 0.0;
+    }
     T b[C::NV];
     double wgeo = 0.0;
     if constexpr (FmtTraits<FMT>::GEOM) {
@@ -259,27 +267,43 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
 #pragma unroll
         for (int k = 0; k < C::NV; k++) b[k] = T(0);
     }
-    // the tile's DoF indexes and u values, gathered once per element into LDS
+    // the tile's indexes and u values, gathered once per element into LDS
     // (the lanes of an element read them from there: no per-lane copy of the
     // element's u in registers, no repeated gathers)
 #pragma unroll
     for (int k = 0; k < NG; k++) {
-        if (k == 0 || lane + k * kWave < C::ET * C::CE) {
-            dsx[lane + k * kWave] = dl[k];
-            us[lane + k * kWave] = ul[k];
+        const int t = lane + k * kWave;
+        if ((k == 0 && !PAIR) || t < NE) {
+            dsx[t] = dl[k];
+            if (PAIR) {
+                us[2 * t] = ul[k];
+                us[2 * t + 1] = vl[k];
+            } else {
+                us[t] = ul[k];
+            }
         }
     }
     __syncthreads();
-    // the lane's column sums: columns rj and H + rj of element re; an entry
-    // flagged kFirstWrite (flin set) starts from f_lin instead of r
-    const int re = lane / H, rj = lane - re * H;
-    const bool task = lane < C::ET * H && re < nv;
-    auto start = [&](int de) -> double { return (de < 0 ? flin : r)[de & kDofMask]; };
+    // The lane's column sums and where their r values start (an entry flagged
+    // kFirstWrite, flin set, starts from f_lin instead of r).  PAIR: node ra
+    // of element re, its x and y columns, in the half that holds the node;
+    // else columns rj and H + rj of element re.
+    const int re = lane / (PAIR ? NSH : H), rj = lane - re * (PAIR ? NSH : H);
+    const bool task = lane < C::ET * (PAIR ? NSH : H) && re < nv;
+    const bool first_half = !PAIR || rj < H / 2;
+    auto start = [&](int de, int off) -> double { return (de < 0 ? flin : r)[(de & kDofMask) + off]; };
+    auto request_r = [&](double& r0, double& r1) {
+        if (PAIR) {
+            const int de = dsx[re * NSH + rj];
+            r0 = start(de, 0);
+            r1 = start(de, 1);
+        } else {
+            r0 = start(dsx[re * C::CE + rj], 0);
+            r1 = start(dsx[re * C::CE + H + rj], 0);
+        }
+    };
     double r0 = 0.0, r1 = 0.0;
-    if (R_EARLY && task) {
-        r0 = start(dsx[re * C::CE + rj]);
-        r1 = start(dsx[re * C::CE + H + rj]);
-    }
+    if (R_EARLY && task) request_r(r0, r1);
     double e3[3] = {0.0, 0.0, 0.0};
     if (valid) strain_fmt<SHAPE, FMT>(b, us + el * C::CE, e3);
     double ws[3] = {0.0, 0.0, 0.0};
@@ -298,17 +322,30 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
         }
     }
     // otherwise both r values now, in flight during the first half's products
-    if (!R_EARLY && task) {
-        r0 = start(dsx[re * C::CE + rj]);
-        r1 = start(dsx[re * C::CE + H + rj]);
-    }
+    if (!R_EARLY && task) request_r(r0, r1);
     if (slot) bt_cols<SHAPE, FMT, 0, H>(b, ws, lds + lane * H);
     __syncthreads();
-    if (task) reduce_lane<SHAPE, FMT, H>(lds, re, rj, dsx[re * C::CE + rj] & kDofMask, r0, r);
+    if (PAIR) {
+        if (task && first_half) {
+            const int d = dsx[re * NSH + rj] & kDofMask;
+            reduce_lane<SHAPE, FMT, H>(lds, re, 2 * rj, d, r0, r);
+            reduce_lane<SHAPE, FMT, H>(lds, re, 2 * rj + 1, d + 1, r1, r);
+        }
+    } else if (task) {
+        reduce_lane<SHAPE, FMT, H>(lds, re, rj, dsx[re * C::CE + rj] & kDofMask, r0, r);
+    }
     __syncthreads();
-    if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, ws, lds + lane * (C::CE - H));
+    if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, ws, lds + lane * H2);
     __syncthreads();
-    if (task) reduce_lane<SHAPE, FMT, C::CE - H>(lds, re, rj, dsx[re * C::CE + H + rj] & kDofMask, r1, r);
+    if (PAIR) {
+        if (task && !first_half) {
+            const int d = dsx[re * NSH + rj] & kDofMask;
+            reduce_lane<SHAPE, FMT, H2>(lds, re, 2 * rj - H, d, r0, r);
+            reduce_lane<SHAPE, FMT, H2>(lds, re, 2 * rj - H + 1, d + 1, r1, r);
+        }
+    } else if (task) {
+        reduce_lane<SHAPE, FMT, H2>(lds, re, rj, dsx[re * C::CE + H + rj] & kDofMask, r1, r);
+    }
 }
 
 // y += sum_e B_e^T (w Dep) B_e v_e with B in format FMT (FP32 formats: FP32
@@ -471,9 +508,11 @@ void residual_fmt_shape(bool commit, const GroupDev& g, const void* Bv, const do
                         const double* flin, const PointState& st, hipStream_t s) {
     const dim3 grid((unsigned)g.ntiles), block(kWave);
     if (commit)
-        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, true>), grid, block, 0, s, g, Bv, u, r, flin, st);
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, true, false>), grid, block, 0, s, g, Bv, u, r, flin, st);
+    else if (g.node)
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false, true>), grid, block, 0, s, g, Bv, u, r, flin, st);
     else
-        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false>), grid, block, 0, s, g, Bv, u, r, flin, st);
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false, false>), grid, block, 0, s, g, Bv, u, r, flin, st);
 }
 
 template <int FMT>

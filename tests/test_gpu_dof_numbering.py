@@ -1,0 +1,69 @@
+"""The residual does not depend on how the DoFs are numbered.
+
+When every element numbers its nodes' DoFs as consecutive (x, y) pairs (the
+meshes here and NeoPZ's connect blocks), the residual gathers one index per
+node; any other numbering takes the per-DoF path.  Bar: the problem
+renumbered by a random permutation p of the equations (DoF d becomes p[d] in
+the DoF and color indexes, u and f_lin) gives r'[p[d]] == r[d] bit for bit,
+for the geometric and the dense B formats, with and without a commit (the
+committed state equal bit for bit too).
+"""
+import numpy as np
+import pytest
+
+import intpointsfem_amd as ipf
+from intpointsfem_amd import wellbore as wb
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mesh", [(12, 9, 2, 1.2), (6, 5, 3, 1.3), (8, 8, 1, 1.1)], ids=["q2", "q3", "q1"])
+@pytest.mark.parametrize("fmt", ["geometry-fp64", "dense-fp64"])
+def test_renumbered_dofs_give_the_same_residual(gpu, mesh, fmt):
+    import torch
+    m = wb.make_mesh(mesh)
+    sm = ipf.StructMatrix(m, device=-1)
+    b = sm.export_blocks()
+    ix = sm.export_indexes()
+    s = sm.sizes()
+    neq, npts = s["neq"], s["npts"]
+    d = m.material()
+    mat = (d["lambda"], d["mu"], d["phi"], d["psi"], d["cohesion"])
+    (nip, nsh), tab = m.shape_table()
+    corners = m.corners()
+    perm = np.random.default_rng(3).permutation(neq)
+    dof = np.asarray(ix["dof_indexes"], dtype=np.int64)
+    ci = np.asarray(ix["color_indexes"], dtype=np.int64)
+    u = wb.lame_displacement(m.nodes(), wb.normal_stress(10), scale=1.5)
+    flin = ix["rhs_linear"]
+    u_p = np.empty(neq)
+    u_p[perm] = u
+    flin_p = np.empty(neq)
+    flin_p[perm] = flin
+
+    def run(dofs, colors, uu, ff, commit):
+        ctx = ipf.DeviceContext(device=0)
+        if fmt == "geometry-fp64":
+            ctx.transfer_geometry([(nip, nsh)], tab, np.zeros(corners.shape[0], np.int32), corners, dofs, colors,
+                                  s["ncolor"], neq)
+        else:
+            ctx.transfer_data(b["row_sizes"], b["col_sizes"], b["matrix_position"], b["row_first"], b["col_first"],
+                              b["storage"], dofs, colors, s["ncolor"], neq, ix["weight"])
+        ctx.set_material(*mat)
+        ctx.set_b_format(fmt)
+        ctx.set_rhs_linear(ff)
+        r = torch.full((neq,), float("nan"), dtype=torch.float64, device=gpu)
+        ctx.set_update_mem(commit)
+        ctx.residual(torch.as_tensor(uu, device=gpu), r)
+        ctx.set_update_mem(False)
+        ctx.synchronize()
+        return r.cpu().numpy(), ctx.get_state()
+
+    dof_p = perm[dof].astype(np.int32)
+    ci_p = (perm[ci % neq] + (ci // neq) * neq).astype(np.int32)
+    for commit in (False, True):
+        r, st = run(dof.astype(np.int32), ci.astype(np.int32), u, flin, commit)
+        r_p, st_p = run(dof_p, ci_p, u_p, flin_p, commit)
+        assert np.array_equal(r_p[perm], r), commit
+        assert all(np.array_equal(x, y) for x, y in zip(st, st_p)), commit
+    assert (st[2] != 0).any(), "the load should make some points plastic"
