@@ -127,10 +127,13 @@ def time_residuals(ctx, u, r, stream, steps, warmup, roi=None):
     return start.elapsed_time(end)
 
 
-def kernel_name(fmt, shape):
-    """Name rocprofv3 shows for the residual kernel of a B format and element shape."""
+def kernel_name(fmt, shape, pairs=True):
+    """Name rocprofv3 shows for the non-committing residual kernel of a B
+    format and element shape (pairs: one gathered index per node, taken when
+    every element numbers its nodes' DoFs as (x, y) pairs, as the wellbore
+    meshes do)."""
     import intpointsfem_amd as ipf
-    return f"void k_residual_fmt<{shape}, {ipf.B_FORMATS.index(fmt)}, false>"
+    return f"void k_residual_fmt<{shape}, {ipf.B_FORMATS.index(fmt)}, false, {'true' if pairs else 'false'}>"
 
 
 def launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt, step_ms=None):
