@@ -429,7 +429,7 @@ void launch_group_residual(ipf_ctx* ctx, const Group& g, const double* u, double
                            const PointState& st) {
     GroupDev d = group_dev(ctx, g);
     if (flin) d.dof = ctx->dDofR + g.dof_off;
-    if (flin && ctx->dNodeR) d.node = ctx->dNodeR + g.dof_off / 2;
+    if (ctx->dNodeR) d.node = ctx->dNodeR + g.dof_off / 2;  // flagged only when f_lin is folded in
     launch_residual_fmt(ctx->rfmt, g.shape, ctx->update_mem, d, group_fmt_B(ctx, g, ctx->rfmt), u, r, flin, st,
                         ctx->stream);
 }
@@ -445,8 +445,9 @@ const double* residual_start(ipf_ctx* ctx, double* r) {
 
 // One tangent (K v) launch for a group in the selected B format.
 void launch_group_tangent(ipf_ctx* ctx, const Group& g, const double* v, double* y, hipStream_t s) {
-    launch_tangent_fmt(ctx->tfmt, g.shape, group_dev(ctx, g), group_fmt_B(ctx, g, ctx->tfmt), v, y, ctx->dDep,
-                       ctx->npad, s);
+    GroupDev d = group_dev(ctx, g);
+    if (ctx->dNodeR) d.node = ctx->dNodeR + g.dof_off / 2;  // flags masked off by the kernel
+    launch_tangent_fmt(ctx->tfmt, g.shape, d, group_fmt_B(ctx, g, ctx->tfmt), v, y, ctx->dDep, ctx->npad, s);
 }
 
 constexpr int kCgBlock = 10;  // CG iterations between convergence checks
@@ -853,18 +854,18 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
     // as (d, d + 1), whose first writes fall on the same entry pair; then the
     // residual gathers one index per node
     std::vector<int> hNodeR;
-    bool pairs = fold;
+    bool pairs = true;
     for (const auto& g : groups) {
         const int CE = shape_ce(g.shape), ET = shape_et(g.shape);
         for (int64_t t = 0; pairs && t < g.ntiles; t++) {
             const int nvt = g.tmeta[t] & 255;
             for (int64_t i = g.dof_off + t * ET * CE; pairs && i < g.dof_off + (t * ET + nvt) * CE; i += 2)
-                if (hDof[i + 1] != hDof[i] + 1 || (hDofR[i] < 0) != (hDofR[i + 1] < 0)) pairs = false;
+                if (hDof[i + 1] != hDof[i] + 1 || (fold && (hDofR[i] < 0) != (hDofR[i + 1] < 0))) pairs = false;
         }
     }
     if (pairs) {
         hNodeR.resize(nd / 2);
-        for (int64_t i = 0; i < nd / 2; i++) hNodeR[i] = hDofR[2 * i];
+        for (int64_t i = 0; i < nd / 2; i++) hNodeR[i] = fold ? hDofR[2 * i] : hDof[2 * i];
     }
 
     // --- device buffers

@@ -350,36 +350,44 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
 
 // y += sum_e B_e^T (w Dep) B_e v_e with B in format FMT (FP32 formats: FP32
 // products and accumulation, the matvec of the inner CG solves).
-template <int SHAPE, int FMT>
+template <int SHAPE, int FMT, bool PAIR>
 __global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
 k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ v, double* __restrict__ y,
               const double* __restrict__ dep, int64_t npad) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
-    constexpr int H = C::CE / 2;
-    static_assert(C::ET * H <= kWave, "one column sum per lane");
-    __shared__ T lds[C::TP * (C::CE - C::CE / 2)];
+    constexpr int NSH = C::CE / 2;
+    // staged halves of the columns as in k_residual_fmt
+    constexpr int H = PAIR ? 2 * ((NSH + 1) / 2) : C::CE / 2;
+    constexpr int H2 = C::CE - H;
+    static_assert(C::ET * NSH <= kWave, "one column sum (PAIR: node) per lane and half");
+    constexpr int NE = PAIR ? C::ET * NSH : C::ET * C::CE;
+    __shared__ T lds[C::TP * H];
     __shared__ double vs[C::ET * C::CE];
-    __shared__ int dsx[C::ET * C::CE];
+    __shared__ int dsx[NE];
     const int lane = threadIdx.x;
     const int64_t tile = blockIdx.x;
     const int el = lane / C::NIP;
     const bool slot = lane < C::TP;
     const int nv = tile_nvalid(g, tile);
     const bool valid = slot && el < nv;
-    // DoF indexes, then the v values they address, outside branches (the
-    // addresses are in bounds as in k_residual_fmt)
-    constexpr int NG = (C::ET * C::CE + kWave - 1) / kWave;
-    static_assert(C::ET * C::CE >= kWave, "the first gather round fills the wave");
+    // DoF (PAIR: node; flags masked off) indexes, then the v values they
+    // address, outside branches (the addresses are in bounds as in
+    // k_residual_fmt)
+    constexpr int NG = (NE + kWave - 1) / kWave;
+    static_assert(NE >= kWave || PAIR, "the first gather round fills the wave");
     int dl[NG];
 #pragma unroll
     for (int k = 0; k < NG; k++) {
-        const int t = min(lane + k * kWave, C::ET * C::CE - 1);
-        dl[k] = ld_stream(&g.dof[tile * (int64_t)(C::ET * C::CE) + t]);
+        const int t = min(lane + k * kWave, NE - 1);
+        dl[k] = ld_stream(&(PAIR ? g.node : g.dof)[tile * (int64_t)NE + t]) & kDofMask;
     }
-    double vl[NG];
+    double vl[NG], vl2[NG];
 #pragma unroll
-    for (int k = 0; k < NG; k++) vl[k] = v[dl[k]];
+    for (int k = 0; k < NG; k++) {
+        vl[k] = v[dl[k]];
+        vl2[k] = PAIR ? v[dl[k] + 1] : 0.0;
+    }
     T b[C::NV];
     if constexpr (FmtTraits<FMT>::GEOM) {
         if (valid) {
@@ -396,27 +404,42 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     }
 #pragma unroll
     for (int k = 0; k < NG; k++) {
-        if (k == 0 || lane + k * kWave < C::ET * C::CE) {
-            dsx[lane + k * kWave] = dl[k];
-            vs[lane + k * kWave] = vl[k];
+        const int t = lane + k * kWave;
+        if ((k == 0 && !PAIR) || t < NE) {
+            dsx[t] = dl[k];
+            if (PAIR) {
+                vs[2 * t] = vl[k];
+                vs[2 * t + 1] = vl2[k];
+            } else {
+                vs[t] = vl[k];
+            }
         }
     }
     __syncthreads();
     // the point's 9 tangent values and the lane's two y values (the color's
     // elements share no DoF), requested before the strain -- except next to
     // dense FP64 B of up to 18 columns, whose 3 CE values already fill the
-    // registers (measured: spills, slower); there they follow the strain
+    // registers (measured: spills, slower); there they follow the strain.
+    // PAIR: node rj of element re, its x and y columns; else columns rj and
+    // H + rj of element re.
     constexpr bool EARLY = !(FMT == FMT_D64 && C::CE <= 18);
     const int64_t P = g.pt0 + tile * C::TP + (slot ? lane : 0);
-    const int re = lane / H, rj = lane - re * H;
-    const bool task = lane < C::ET * H && re < nv;
+    const int re = lane / (PAIR ? NSH : H), rj = lane - re * (PAIR ? NSH : H);
+    const bool task = lane < C::ET * (PAIR ? NSH : H) && re < nv;
+    const bool first_half = !PAIR || rj < H / 2;
     double dp[9], y0 = 0.0, y1 = 0.0;
     auto request = [&]() {
 #pragma unroll
         for (int k = 0; k < 9; k++) dp[k] = ld_stream(&dep[k * npad + P]);
         if (task) {
-            y0 = y[dsx[re * C::CE + rj]];
-            y1 = y[dsx[re * C::CE + H + rj]];
+            if (PAIR) {
+                const int d = dsx[re * NSH + rj];
+                y0 = y[d];
+                y1 = y[d + 1];
+            } else {
+                y0 = y[dsx[re * C::CE + rj]];
+                y1 = y[dsx[re * C::CE + H + rj]];
+            }
         }
     };
     if constexpr (EARLY) request();
@@ -435,11 +458,27 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     }
     if (slot) bt_cols<SHAPE, FMT, 0, H>(b, x, lds + lane * H);
     __syncthreads();
-    if (task) reduce_lane<SHAPE, FMT, H>(lds, re, rj, dsx[re * C::CE + rj], y0, y);
+    if (PAIR) {
+        if (task && first_half) {
+            const int d = dsx[re * NSH + rj];
+            reduce_lane<SHAPE, FMT, H>(lds, re, 2 * rj, d, y0, y);
+            reduce_lane<SHAPE, FMT, H>(lds, re, 2 * rj + 1, d + 1, y1, y);
+        }
+    } else if (task) {
+        reduce_lane<SHAPE, FMT, H>(lds, re, rj, dsx[re * C::CE + rj], y0, y);
+    }
     __syncthreads();
-    if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, x, lds + lane * (C::CE - H));
+    if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, x, lds + lane * H2);
     __syncthreads();
-    if (task) reduce_lane<SHAPE, FMT, C::CE - H>(lds, re, rj, dsx[re * C::CE + H + rj], y1, y);
+    if (PAIR) {
+        if (task && !first_half) {
+            const int d = dsx[re * NSH + rj];
+            reduce_lane<SHAPE, FMT, H2>(lds, re, 2 * rj - H, d, y0, y);
+            reduce_lane<SHAPE, FMT, H2>(lds, re, 2 * rj - H + 1, d + 1, y1, y);
+        }
+    } else if (task) {
+        reduce_lane<SHAPE, FMT, H2>(lds, re, rj, dsx[re * C::CE + H + rj], y1, y);
+    }
 }
 
 // Builds format FMT from the dense FP64 tile layout of one group (device side).
@@ -528,7 +567,10 @@ void tangent_fmt(int shape, const GroupDev& g, const void* Bv, const double* v, 
     const dim3 grid((unsigned)g.ntiles), block(kWave);
     dispatch_shape(shape, [&](auto sc) {
         constexpr int S = decltype(sc)::value;
-        hipLaunchKernelGGL((k_tangent_fmt<S, FMT>), grid, block, 0, s, g, Bv, v, y, dep, npad);
+        if (g.node)
+            hipLaunchKernelGGL((k_tangent_fmt<S, FMT, true>), grid, block, 0, s, g, Bv, v, y, dep, npad);
+        else
+            hipLaunchKernelGGL((k_tangent_fmt<S, FMT, false>), grid, block, 0, s, g, Bv, v, y, dep, npad);
     });
 }
 

@@ -1,12 +1,14 @@
-"""The residual does not depend on how the DoFs are numbered.
+"""The residual and the tangent product do not depend on how the DoFs are
+numbered.
 
 When every element numbers its nodes' DoFs as consecutive (x, y) pairs (the
-meshes here and NeoPZ's connect blocks), the residual gathers one index per
-node; any other numbering takes the per-DoF path.  Bar: the problem
-renumbered by a random permutation p of the equations (DoF d becomes p[d] in
-the DoF and color indexes, u and f_lin) gives r'[p[d]] == r[d] bit for bit,
-for the geometric and the dense B formats, with and without a commit (the
-committed state equal bit for bit too).
+meshes here and NeoPZ's connect blocks), the residual and the CG matvec
+gather one index per node; any other numbering takes the per-DoF path.  Bar:
+the problem renumbered by a random permutation p of the equations (DoF d
+becomes p[d] in the DoF and color indexes, u, v and f_lin) gives
+r'[p[d]] == r[d] and (K'v')[p[d]] == (K v)[d] bit for bit, for the geometric
+and the dense B formats, with and without a commit (the committed state
+equal bit for bit too).
 """
 import numpy as np
 import pytest
@@ -56,14 +58,21 @@ def test_renumbered_dofs_give_the_same_residual(gpu, mesh, fmt):
         ctx.set_update_mem(commit)
         ctx.residual(torch.as_tensor(uu, device=gpu), r)
         ctx.set_update_mem(False)
+        # K v at the same state (tangents from ipf_residual_tangent)
+        ctx.set_b_format(fmt, tangent=fmt)
+        ctx.residual_tangent(torch.as_tensor(uu, device=gpu), torch.empty_like(r))
+        kv = torch.zeros(neq, dtype=torch.float64, device=gpu)
+        ctx.tangent_apply(torch.as_tensor(uu * 3.0 - 1e-4, device=gpu), kv)
         ctx.synchronize()
-        return r.cpu().numpy(), ctx.get_state()
+        return r.cpu().numpy(), ctx.get_state(), kv.cpu().numpy()
 
     dof_p = perm[dof].astype(np.int32)
     ci_p = (perm[ci % neq] + (ci // neq) * neq).astype(np.int32)
     for commit in (False, True):
-        r, st = run(dof.astype(np.int32), ci.astype(np.int32), u, flin, commit)
-        r_p, st_p = run(dof_p, ci_p, u_p, flin_p, commit)
+        r, st, kv = run(dof.astype(np.int32), ci.astype(np.int32), u, flin, commit)
+        r_p, st_p, kv_p = run(dof_p, ci_p, u_p, flin_p, commit)
         assert np.array_equal(r_p[perm], r), commit
         assert all(np.array_equal(x, y) for x, y in zip(st, st_p)), commit
+        assert np.array_equal(kv_p[perm], kv), commit
+        assert np.abs(kv).max() > 0
     assert (st[2] != 0).any(), "the load should make some points plastic"
