@@ -293,11 +293,28 @@ def run_batch(args):
     ctx.synchronize()
     log(f"[bench] launches {', '.join(f'{x:.4f}' for x in roof['color_ms'])} ms; {roof['achieved']:.0f} GB/s on "
         f"plain FP64 bytes, {roof['achieved_stored']:.0f} GB/s on stored bytes")
+    # HBM bytes per launch of each element order, from the committed rocprofv3
+    # profile of this workload (PMC FETCH_SIZE + WRITE_SIZE, calibrated on k_copy)
+    traffic, by_shape = None, None
+    tfile = os.path.join(ROOT, "profiles", "k_residual_batch64_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tk = json.load(f)["kernels"]
+        by_shape = {}
+        for sh, nm in enumerate(("Q1", "Q2", "Q3")):
+            kn = kernel_name(fmt0, sh)
+            if kn in tk and "total_bytes" in tk[kn]:
+                by_shape[nm] = {"traffic": int(tk[kn]["total_bytes"]), "valu_busy": tk[kn].get("valu_busy")}
+        if len(by_shape) == 3 and len(roof["color_ms"]) == 3 * B.ncolor:
+            # every color holds all three orders: the mean over the launches
+            traffic = int(sum(v["traffic"] for v in by_shape.values()) / 3)
     roofline = {"bound": "hbm", "achieved": round(roof["achieved_stored"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_residual_fmt, one launch per (color, element order) over all instances",
                 "algorithmic_bytes_per_point": round(roof["stored_bytes"] / npts, 1),
                 "avg_launch_ms": round(roof["avg_launch_ms"], 5),
+                "measured_traffic_rate": None if traffic is None else round(traffic / (roof["avg_launch_ms"] / 1e3) / 1e9, 1),
+                "traffic_by_order": by_shape,
                 "dense_fp64_bytes_rate": round(roof["achieved"], 1)}
     cpu = None
     if not args.no_cpu_baseline:
