@@ -5,7 +5,7 @@ A step is one full residual evaluation r = f_lin - sum_e B_e^T (w sigma(B_e u_e)
 quarter-annulus wellbore (640,000 elements, 5.76 M integration points,
 5,126,402 DoF), plain FP64, with u and r resident in HBM, at the converged
 Newton state of the 10 load steps (||r||/||r0|| <= 1e-8; the solve -- Newton
-with Jacobi-PCG inner solves, FP64 throughout -- runs first, untimed).  The
+with block-Jacobi PCG inner solves, FP64 throughout -- runs first, untimed).  The
 data is set up on the GPU from the element corners, and the library's B
 format is the geometric one: B and the weights recomputed inside the kernel
 from 8 corner coordinates per element, bit-identical to the stored dense and
@@ -374,7 +374,7 @@ def main():
                          "thick-cylinder displacement of the last load step with zero plastic strain (no solve)")
     ap.add_argument("--newton-rtol", type=float, default=1e-8)
     ap.add_argument("--cg-rtol", type=float, default=1e-1,
-                    help="inner Jacobi-PCG tolerance of each Newton step (the Newton loop converges linearly "
+                    help="inner block-Jacobi PCG tolerance of each Newton step (the Newton loop converges linearly "
                          "with the reference's tangent, so a loose inner solve costs few extra Newton steps)")
     ap.add_argument("--save-state", default=None, metavar="FILE",
                     help="--state newton: write the converged state (u, plastic strain, alpha) to FILE (.npz)")
@@ -440,7 +440,7 @@ def main():
         state_desc = (f"converged Newton solution of load step 10 (||r||/||r0|| <= {args.newton_rtol:g}), "
                       "committed plastic strain of the 10 load steps (loaded from --save-state output)")
     else:
-        # 10 load steps of Newton + Jacobi-PCG on the GPU; the state is committed after each
+        # 10 load steps of Newton + block-Jacobi PCG on the GPU; the state is committed after each
         ctx.set_dirichlet_mask(mesh.dirichlet_mask())
         fmt_default = ctx.b_format()
         # FP64 residual; tangent products in the chosen format (FP32: mixed Newton)

@@ -301,7 +301,9 @@ int ipf_assemble_tangent(ipf_ctx *ctx, const double *u_dev, double *r_dev, doubl
  * (all DoFs, no mask), device pointers [neq]. */
 int ipf_tangent_apply(ipf_ctx *ctx, const double *v_dev, double *y_dev);
 
-/* Jacobi-preconditioned CG on the free DoFs: K x = b, x0 = 0, until
+/* Block-Jacobi-preconditioned CG on the free DoFs (each node's 2x2 diagonal
+ * block of K inverted where its DoFs are an aligned (2k, 2k + 1) pair, the
+ * diagonal elsewhere): K x = b, x0 = 0, until
  * ||r|| <= rtol * ||b|| or maxit iterations.  Device pointers [neq]; fixed DoFs
  * of x are zero.  iters / relres may be NULL. */
 int ipf_cg_solve(ipf_ctx *ctx, const double *b_dev, double *x_dev, double rtol, int maxit, int *iters,

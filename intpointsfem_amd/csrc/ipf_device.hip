@@ -212,6 +212,13 @@ struct ipf_ctx {
     double* dSigw = nullptr;    // 3 * npts
     signed char* dMask = nullptr;  // neq
     double* dDiag = nullptr;
+    // preconditioner: the pairs' coupling and z = pa r + pb r_partner
+    // (launch_block_jacobi); pairs_aligned: every element numbers node a's
+    // DoFs as (2k, 2k + 1), so the DoF pairs of the vectors are the nodes
+    double* dOffd = nullptr;
+    double* dPa = nullptr;
+    double* dPb = nullptr;
+    bool pairs_aligned = false;
     double* dCr = nullptr;
     double* dCz = nullptr;
     double* dCp = nullptr;
@@ -316,6 +323,9 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dSigw);
     dev_free(ctx->dMask);
     dev_free(ctx->dDiag);
+    dev_free(ctx->dOffd);
+    dev_free(ctx->dPa);
+    dev_free(ctx->dPb);
     dev_free(ctx->dCr);
     dev_free(ctx->dCz);
     dev_free(ctx->dCp);
@@ -352,6 +362,8 @@ int ensure_solver_buffers(ipf_ctx* ctx) {
     int rc;
     if ((rc = dev_alloc(ctx, &ctx->dDep, 9 * ctx->npad)) || (rc = dev_alloc(ctx, &ctx->dStrain, 3 * ctx->npts)) ||
         (rc = dev_alloc(ctx, &ctx->dSigw, 3 * ctx->npts)) || (rc = dev_alloc(ctx, &ctx->dDiag, ctx->neq)) ||
+        (rc = dev_alloc(ctx, &ctx->dOffd, ctx->neq)) || (rc = dev_alloc(ctx, &ctx->dPa, ctx->neq)) ||
+        (rc = dev_alloc(ctx, &ctx->dPb, ctx->neq)) ||
         (rc = dev_alloc(ctx, &ctx->dCr, ctx->neq)) || (rc = dev_alloc(ctx, &ctx->dCz, ctx->neq)) ||
         (rc = dev_alloc(ctx, &ctx->dCp, ctx->neq)) || (rc = dev_alloc(ctx, &ctx->dCq, ctx->neq)) ||
         (rc = dev_alloc(ctx, &ctx->dNr, ctx->neq)) || (rc = dev_alloc(ctx, &ctx->dNd, ctx->neq)) ||
@@ -462,7 +474,7 @@ void cg_iterations(ipf_ctx* ctx, double* x, int count, hipStream_t s) {
         for (const auto& g : ctx->groups) launch_group_tangent(ctx, g, ctx->dCp, ctx->dCq, s);
         launch_cg_mask_dot(n, ctx->dMask, ctx->dCp, ctx->dCq, ctx->dPartial, nb, ctx->dScal + 1, s);
         launch_cg_scalars(ctx->dScal, 0, s);
-        launch_cg_update_dot(n, ctx->dScal, ctx->dCp, ctx->dCq, x, ctx->dCr, ctx->dDiag, ctx->dCz, ctx->dPartial, nb,
+        launch_cg_update_dot(n, ctx->dScal, ctx->dCp, ctx->dCq, x, ctx->dCr, ctx->dPa, ctx->dPb, ctx->dCz, ctx->dPartial, nb,
                              ctx->dScal + 2, s);
         launch_cg_scalars(ctx->dScal, 1, s);
         launch_cg_direction_clear(n, ctx->dScal, ctx->dCz, ctx->dCp, ctx->dCq, s);
@@ -854,13 +866,14 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
     // as (d, d + 1), whose first writes fall on the same entry pair; then the
     // residual gathers one index per node
     std::vector<int> hNodeR;
-    bool pairs = true;
+    bool pairs = true, aligned = true;
     for (const auto& g : groups) {
         const int CE = shape_ce(g.shape), ET = shape_et(g.shape);
         for (int64_t t = 0; pairs && t < g.ntiles; t++) {
             const int nvt = g.tmeta[t] & 255;
             for (int64_t i = g.dof_off + t * ET * CE; pairs && i < g.dof_off + (t * ET + nvt) * CE; i += 2)
                 if (hDof[i + 1] != hDof[i] + 1 || (fold && (hDofR[i] < 0) != (hDofR[i + 1] < 0))) pairs = false;
+                else if (hDof[i] & 1) aligned = false;
         }
     }
     if (pairs) {
@@ -950,6 +963,7 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
     ctx->npad = npad;
     ctx->max_mat = max_mat;
     ctx->fold_flin = fold;
+    ctx->pairs_aligned = pairs && aligned;
     ctx->groups = std::move(groups);
     ctx->compact_ok = compact_ok;
     ctx->rfmt = ctx->tfmt = FMT_D64;
@@ -1509,11 +1523,16 @@ int ipf_cg_solve(ipf_ctx* ctx, const double* b_dev, double* x_dev, double rtol, 
     const int64_t n = ctx->neq;
     const int nb = dot_blocks(n);
     hipStream_t s = ctx->stream;
-    // Jacobi diagonal of the current tangent
+    // block-Jacobi preconditioner of the current tangent: each node's 2x2
+    // diagonal block inverted where the DoFs pair up, else its diagonal
+    double* offd = ctx->pairs_aligned ? ctx->dOffd : nullptr;
     launch_fill(n, ctx->dDiag, 0.0, s);
-    for (const auto& g : ctx->groups) launch_tangent_diag(g.shape, group_dev(ctx, g), ctx->dDiag, ctx->dDep, ctx->npad, s);
+    if (offd) launch_fill(n, offd, 0.0, s);
+    for (const auto& g : ctx->groups)
+        launch_tangent_diag(g.shape, group_dev(ctx, g), ctx->dDiag, offd, ctx->dDep, ctx->npad, s);
+    launch_block_jacobi(n, ctx->dMask, ctx->dDiag, offd, ctx->dPa, ctx->dPb, s);
     launch_fill(n, x_dev, 0.0, s);
-    launch_cg_init(n, ctx->dMask, b_dev, ctx->dCr, ctx->dDiag, ctx->dCz, ctx->dCp, s);
+    launch_cg_init(n, ctx->dMask, b_dev, ctx->dCr, ctx->dPa, ctx->dPb, ctx->dCz, ctx->dCp, s);
     launch_dot(n, ctx->dCr, ctx->dCz, ctx->dPartial, nb, ctx->dScal + 0, s);
     launch_dot(n, ctx->dCr, ctx->dCr, ctx->dPartial, nb, ctx->dScal + 3, s);
     double bb;

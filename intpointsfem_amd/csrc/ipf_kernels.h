@@ -129,8 +129,10 @@ void launch_fill(int64_t n, double* dst, double value, hipStream_t stream);
 // launch_tangent_fmt (every B format); w Dep is stored SoA as
 // dep_soa[k * npad + P], k = 3*row + col.
 // diag += diagonal of the same operator.
-void launch_tangent_diag(int shape, const GroupDev& g, double* diag, const double* dep_soa, int64_t npad,
-                         hipStream_t stream);
+// offd (optional) += for each element column j, the symmetric coupling of
+// columns j and j ^ 1 (a node's x and y DoFs), at column j's DoF.
+void launch_tangent_diag(int shape, const GroupDev& g, double* diag, double* offd, const double* dep_soa,
+                         int64_t npad, hipStream_t stream);
 
 // Tangent stiffness into CSR values (ipf_assemble.hip): kg[lidx[e * NT + t]] +=
 // entry t of element e's packed upper triangle (NT = CE (CE + 1) / 2), for
@@ -161,18 +163,26 @@ void launch_build_B(int shape, const GroupDev& g, const double* corners, double*
 // device array of scalars (see ipf_solver.hip for the slots).
 void launch_dot(int64_t n, const double* x, const double* y, double* partial, int nblocks, double* out,
                 hipStream_t stream);
-void launch_cg_init(int64_t n, const signed char* mask, const double* b, double* r, const double* diag, double* z,
-                    double* p, hipStream_t stream);
+// The preconditioner (pa, pb): z_i = pa_i r_i + pb_i r_partner on DoF pairs
+// (i, i + 1), i even -- see launch_block_jacobi.
+void launch_cg_init(int64_t n, const signed char* mask, const double* b, double* r, const double* pa,
+                    const double* pb, double* z, double* p, hipStream_t stream);
+// pa, pb from the tangent's diagonal and, when the DoFs come in aligned
+// (x, y) = (2k, 2k + 1) pairs, the pairs' coupling offd (else null): the
+// inverse of each node's 2x2 diagonal block, or Jacobi where none applies.
+void launch_block_jacobi(int64_t n, const signed char* mask, const double* diag, const double* offd, double* pa,
+                         double* pb, hipStream_t stream);
 void launch_cg_mask(int64_t n, const signed char* mask, double* q, hipStream_t stream);
 void launch_cg_update(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
-                      const double* diag, double* z, hipStream_t stream);
+                      const double* pa, const double* pb, double* z, hipStream_t stream);
 void launch_cg_direction(int64_t n, const double* scal, const double* z, double* p, hipStream_t stream);
 void launch_cg_scalars(double* scal, int stage, hipStream_t stream);
 // fused forms: q masked + p.q into *out; update + r.z into *out; direction + q = 0
 void launch_cg_mask_dot(int64_t n, const signed char* mask, const double* p, double* q, double* partial, int nb,
                         double* out, hipStream_t stream);
 void launch_cg_update_dot(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
-                          const double* diag, double* z, double* partial, int nb, double* out, hipStream_t stream);
+                          const double* pa, const double* pb, double* z, double* partial, int nb, double* out,
+                          hipStream_t stream);
 void launch_cg_direction_clear(int64_t n, const double* scal, const double* z, double* p, double* q,
                                hipStream_t stream);
 void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t stream);

@@ -1,7 +1,7 @@
 // ipf_kernels.hip -- gfx950 kernels of the reference's separate steps on the
 // dense FP64 blocks: Multiply (gather + B.u, TPZNumericalIntegrator.cpp:25-51),
 // MultiplyTranspose (-B^T x with colored accumulation, :54-105), ComputeSigma
-// / ComputeSigmaDep per point (ipf_pointmath.h), the Jacobi diagonal of the
+// / ComputeSigmaDep per point (ipf_pointmath.h), the diagonal (and node blocks) of the
 // tangent, copies and fills.  The fused residual and the tangent product, in
 // every B format, are in ipf_kernels_fmt.hip.  One 64-lane wavefront per tile
 // of elements (ipf_kernels.h); B values are streamed with coalesced 16-byte
@@ -202,7 +202,8 @@ k_sigma(GroupDev g, int tp, int64_t npad, const double* __restrict__ strain, dou
 // with the return map replaced by the stored 3x3 w*Dep of each point.
 template <int SHAPE>
 __global__ void __launch_bounds__(Cfg<SHAPE>::WPB * kWave)
-k_tangent_diag(GroupDev g, double* __restrict__ diag, const double* __restrict__ dep, int64_t npad) {
+k_tangent_diag(GroupDev g, double* __restrict__ diag, double* __restrict__ offd, const double* __restrict__ dep,
+               int64_t npad) {
     using C = Cfg<SHAPE>;
     __shared__ double lds[C::WPB][C::TP * C::CE];
     const int wave = threadIdx.x / kWave;
@@ -213,30 +214,43 @@ k_tangent_diag(GroupDev g, double* __restrict__ diag, const double* __restrict__
     const bool slot = tile_ok && lane < C::TP;
     const bool valid = slot && el < tile_nvalid(g, tile);
     double* L = lds[wave];
-    if (slot) {
-        double D[9];
-        const int64_t P = g.pt0 + tile * C::TP + lane;
+    double D[9];
+    const int64_t P = g.pt0 + (tile_ok ? tile : 0) * C::TP + lane;
 #pragma unroll
-        for (int i = 0; i < 9; i++) D[i] = valid ? dep[i * npad + P] : 0.0;
-        const dvec2* Bt = reinterpret_cast<const dvec2*>(g.B + tile * (int64_t)(C::NB * C::TP));
-        for (int j = 0; j < C::CE; j++) {
-            double bj[3];
+    for (int i = 0; i < 9; i++) D[i] = valid ? dep[i * npad + P] : 0.0;
+    const dvec2* Bt = reinterpret_cast<const dvec2*>(g.B + (tile_ok ? tile : 0) * (int64_t)(C::NB * C::TP));
+    auto col = [&](int j, double* bj) {
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int mm = k * C::CE + j;
-                const dvec2 pr = Bt[(mm >> 1) * C::TP + lane];
-                bj[k] = (mm & 1) ? pr.y : pr.x;
-            }
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < 3; k++)
-#pragma unroll
-                for (int l = 0; l < 3; l++) s = s + bj[k] * D[3 * k + l] * bj[l];
-            L[lane * C::CE + j] = -s;
+        for (int k = 0; k < 3; k++) {
+            const int mm = k * C::CE + j;
+            const dvec2 pr = Bt[(mm >> 1) * C::TP + lane];
+            bj[k] = (mm & 1) ? pr.y : pr.x;
         }
+    };
+    // pass 0: the diagonal b_j^T D b_j; pass 1 (offd set): the symmetric
+    // coupling of the node's two columns, (b_j^T D b_j' + b_j'^T D b_j) / 2
+    // with j' = j ^ 1, into offd at column j's DoF
+    for (int pass = 0; pass < (offd ? 2 : 1); pass++) {
+        if (slot) {
+            for (int j = 0; j < C::CE; j++) {
+                double bj[3], bk[3];
+                col(j, bj);
+                col(pass ? (j ^ 1) : j, bk);
+                double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+#pragma unroll
+                    for (int l = 0; l < 3; l++) {
+                        s1 = s1 + bj[k] * D[3 * k + l] * bk[l];
+                        s2 = s2 + bk[k] * D[3 * k + l] * bj[l];
+                    }
+                L[lane * C::CE + j] = pass ? -0.5 * (s1 + s2) : -s1;
+            }
+        }
+        __syncthreads();
+        if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, pass ? offd : diag);
+        __syncthreads();
     }
-    __syncthreads();
-    if (tile_ok) reduce_scatter<SHAPE>(L, g, tile, lane, diag);
 }
 
 __global__ void k_copy(int64_t n, const double* __restrict__ src, double* __restrict__ dst) {
@@ -287,12 +301,13 @@ void launch_sigma(bool dep, bool commit, int shape, const GroupDev& g, int64_t n
                            commit ? 1 : 0, dep_soa);
 }
 
-void launch_tangent_diag(int shape, const GroupDev& g, double* diag, const double* dep, int64_t npad,
+void launch_tangent_diag(int shape, const GroupDev& g, double* diag, double* offd, const double* dep, int64_t npad,
                          hipStream_t s) {
     if (g.ntiles == 0) return;
     dispatch_shape(shape, [&](auto sc) {
         constexpr int S = decltype(sc)::value;
-        hipLaunchKernelGGL(k_tangent_diag<S>, tile_grid<S>(g), dim3(Cfg<S>::WPB * kWave), 0, s, g, diag, dep, npad);
+        hipLaunchKernelGGL(k_tangent_diag<S>, tile_grid<S>(g), dim3(Cfg<S>::WPB * kWave), 0, s, g, diag, offd, dep,
+                           npad);
     });
 }
 

@@ -44,18 +44,58 @@ k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out
     if (threadIdx.x == 0) *out = t;
 }
 
-__device__ __forceinline__ double precond(double r, double d) { return d != 0.0 ? r / d : 0.0; }
-
+// The preconditioner works on DoF pairs (i, i + 1), i even: z_i = pa_i r_i +
+// pb_i r_(i+1), z_(i+1) = pa_(i+1) r_(i+1) + pb_(i+1) r_i -- the inverse of a
+// node's 2x2 diagonal block of the tangent, or Jacobi (pb = 0) where no block
+// applies.  One thread owns a pair, so each z sees its partner's new r.
 __global__ void k_cg_init(int64_t n, const signed char* __restrict__ mask, const double* __restrict__ b,
-                          double* __restrict__ r, const double* __restrict__ diag, double* __restrict__ z,
-                          double* __restrict__ p) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double ri = mask[i] ? 0.0 : b[i];
-    r[i] = ri;
-    const double zi = precond(ri, diag[i]);
-    z[i] = zi;
-    p[i] = zi;
+                          double* __restrict__ r, const double* __restrict__ pa, const double* __restrict__ pb,
+                          double* __restrict__ z, double* __restrict__ p) {
+    const int64_t i0 = 2 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (i0 >= n) return;
+    const int64_t i1 = i0 + 1;
+    const bool has1 = i1 < n;
+    const double r0 = mask[i0] ? 0.0 : b[i0];
+    const double r1 = has1 && !mask[i1] ? b[i1] : 0.0;
+    const double z0 = pa[i0] * r0 + pb[i0] * r1;
+    r[i0] = r0;
+    z[i0] = z0;
+    p[i0] = z0;
+    if (has1) {
+        const double z1 = pa[i1] * r1 + pb[i1] * r0;
+        r[i1] = r1;
+        z[i1] = z1;
+        p[i1] = z1;
+    }
+}
+
+// pa, pb from the tangent's diagonal and (aligned pairs) the symmetric
+// coupling offd of a node's x and y DoFs: the 2x2 block's inverse where both
+// DoFs are free and the block is positive definite, else 1 / diagonal on a
+// free DoF and 0 on a fixed one.
+__global__ void k_block_jacobi(int64_t n, const signed char* __restrict__ mask, const double* __restrict__ diag,
+                               const double* __restrict__ offd, double* __restrict__ pa, double* __restrict__ pb) {
+    const int64_t i0 = 2 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (i0 >= n) return;
+    const int64_t i1 = i0 + 1;
+    const bool has1 = i1 < n;
+    const double d0 = diag[i0], d1 = has1 ? diag[i1] : 0.0;
+    const bool f0 = !mask[i0] && d0 != 0.0, f1 = has1 && !mask[i1] && d1 != 0.0;
+    const double c = (offd && has1) ? offd[i0] : 0.0;
+    const double det = d0 * d1 - c * c;
+    if (offd && f0 && f1 && d0 > 0.0 && d1 > 0.0 && det > 0.0) {
+        pa[i0] = d1 / det;
+        pa[i1] = d0 / det;
+        pb[i0] = -c / det;
+        pb[i1] = -c / det;
+    } else {
+        pa[i0] = f0 ? 1.0 / d0 : 0.0;
+        pb[i0] = 0.0;
+        if (has1) {
+            pa[i1] = f1 ? 1.0 / d1 : 0.0;
+            pb[i1] = 0.0;
+        }
+    }
 }
 
 __global__ void k_cg_mask(int64_t n, const signed char* __restrict__ mask, double* __restrict__ q) {
@@ -65,14 +105,23 @@ __global__ void k_cg_mask(int64_t n, const signed char* __restrict__ mask, doubl
 
 __global__ void k_cg_update(int64_t n, const double* __restrict__ scal, const double* __restrict__ p,
                             const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
-                            const double* __restrict__ diag, double* __restrict__ z) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                            const double* __restrict__ pa, const double* __restrict__ pb, double* __restrict__ z) {
+    const int64_t i0 = 2 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (i0 >= n) return;
+    const int64_t i1 = i0 + 1;
+    const bool has1 = i1 < n;
     const double alpha = scal[4];
-    x[i] = x[i] + alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    z[i] = precond(ri, diag[i]);
+    x[i0] = x[i0] + alpha * p[i0];
+    const double r0 = r[i0] - alpha * q[i0];
+    r[i0] = r0;
+    double r1 = 0.0;
+    if (has1) {
+        x[i1] = x[i1] + alpha * p[i1];
+        r1 = r[i1] - alpha * q[i1];
+        r[i1] = r1;
+        z[i1] = pa[i1] * r1 + pb[i1] * r0;
+    }
+    z[i0] = pa[i0] * r0 + pb[i0] * r1;
 }
 
 // q = 0 on fixed DoFs, and the block partials of p.q (k_dot_partial's order)
@@ -97,17 +146,33 @@ k_cg_mask_dot(int64_t n, const signed char* __restrict__ mask, const double* __r
 __global__ void __launch_bounds__(kDotThreads)
 k_cg_update_dot(int64_t n, const double* __restrict__ scal, const double* __restrict__ p,
                 const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
-                const double* __restrict__ diag, double* __restrict__ z, double* __restrict__ partial) {
+                const double* __restrict__ pa, const double* __restrict__ pb, double* __restrict__ z,
+                double* __restrict__ partial) {
     __shared__ double sh[kDotThreads];
     const double alpha = scal[4];
     double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kDotThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kDotThreads) {
-        x[i] = x[i] + alpha * p[i];
-        const double ri = r[i] - alpha * q[i];
-        r[i] = ri;
-        const double zi = precond(ri, diag[i]);
-        z[i] = zi;
-        s = s + ri * zi;
+    // one DoF pair per thread and step (see k_cg_init)
+    for (int64_t k = (int64_t)blockIdx.x * kDotThreads + threadIdx.x; 2 * k < n;
+         k += (int64_t)gridDim.x * kDotThreads) {
+        const int64_t i0 = 2 * k, i1 = i0 + 1;
+        const bool has1 = i1 < n;
+        x[i0] = x[i0] + alpha * p[i0];
+        const double r0 = r[i0] - alpha * q[i0];
+        r[i0] = r0;
+        double r1 = 0.0;
+        if (has1) {
+            x[i1] = x[i1] + alpha * p[i1];
+            r1 = r[i1] - alpha * q[i1];
+            r[i1] = r1;
+        }
+        const double z0 = pa[i0] * r0 + pb[i0] * r1;
+        z[i0] = z0;
+        s = s + r0 * z0;
+        if (has1) {
+            const double z1 = pa[i1] * r1 + pb[i1] * r0;
+            z[i1] = z1;
+            s = s + r1 * z1;
+        }
     }
     const double t = block_sum(s, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -197,8 +262,9 @@ void launch_cg_mask_dot(int64_t n, const signed char* mask, const double* p, dou
 }
 
 void launch_cg_update_dot(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
-                          const double* diag, double* z, double* partial, int nb, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_update_dot, dim3(nb), dim3(kDotThreads), 0, s, n, scal, p, q, x, r, diag, z, partial);
+                          const double* pa, const double* pb, double* z, double* partial, int nb, double* out,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_update_dot, dim3(nb), dim3(kDotThreads), 0, s, n, scal, p, q, x, r, pa, pb, z, partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, nb, partial, out);
 }
 
@@ -220,9 +286,14 @@ void launch_dot(int64_t n, const double* x, const double* y, double* partial, in
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, nblocks, partial, out);
 }
 
-void launch_cg_init(int64_t n, const signed char* mask, const double* b, double* r, const double* diag, double* z,
-                    double* p, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_cg_init, grid1(n), dim3(256), 0, s, n, mask, b, r, diag, z, p);
+void launch_cg_init(int64_t n, const signed char* mask, const double* b, double* r, const double* pa,
+                    const double* pb, double* z, double* p, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_cg_init, grid1((n + 1) / 2), dim3(256), 0, s, n, mask, b, r, pa, pb, z, p);
+}
+
+void launch_block_jacobi(int64_t n, const signed char* mask, const double* diag, const double* offd, double* pa,
+                         double* pb, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_block_jacobi, grid1((n + 1) / 2), dim3(256), 0, s, n, mask, diag, offd, pa, pb);
 }
 
 void launch_cg_mask(int64_t n, const signed char* mask, double* q, hipStream_t s) {
@@ -230,8 +301,8 @@ void launch_cg_mask(int64_t n, const signed char* mask, double* q, hipStream_t s
 }
 
 void launch_cg_update(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
-                      const double* diag, double* z, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_cg_update, grid1(n), dim3(256), 0, s, n, scal, p, q, x, r, diag, z);
+                      const double* pa, const double* pb, double* z, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_cg_update, grid1((n + 1) / 2), dim3(256), 0, s, n, scal, p, q, x, r, pa, pb, z);
 }
 
 void launch_cg_direction(int64_t n, const double* scal, const double* z, double* p, hipStream_t s) {

@@ -11,7 +11,7 @@ Bars:
     reference's elastoplastic verification case (WellboreConfigRK,
     main.cpp:374-411: E 2000, nu 0.2, c 5, phi = psi = 20 deg, wellbore
     traction -10, far field -0.00625, fixed_x / fixed_y points), p = 2, 10
-    load steps of Newton + Jacobi-PCG: the solution is axisymmetric, so every
+    load steps of Newton + block-Jacobi PCG: the solution is axisymmetric, so every
     integration point's sigma_rr and sigma_tt (rotated by its own angle) is
     compared with the Runge-Kutta profile (tests/golden/rk_wellbore_profile.json)
     for r >= 0.2 -- bars 0.02 MPa (0.2 % of the load; measured 0.0122 and

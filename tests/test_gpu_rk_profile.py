@@ -1,5 +1,5 @@
 """The reference's elastoplastic verification case solved on the GPU (40 x 40
-Q2 wellbore, 10 load steps of Newton + Jacobi-PCG), against the reference-held
+Q2 wellbore, 10 load steps of Newton + block-Jacobi PCG), against the reference-held
 Runge-Kutta profile and the reference IntPointsFEM CPU result along y = 0
 (tests/golden/rk_wellbore_profile.json).  Bars: for r >= 0.2, sigma_rr and
 sigma_tt within 0.012 MPa of RK (0.12 % of the 10 MPa load; the CPU oracle on

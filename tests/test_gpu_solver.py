@@ -95,7 +95,7 @@ def test_cg_solves_tangent_system(gpu):
 
 def test_newton_load_steps_match_oracle(gpu):
     """Config 1 (40 x 40 Q2, ratio 1.1): 10 load steps on the GPU (Newton +
-    Jacobi-PCG) and on the CPU oracle (Newton + direct solve)."""
+    block-Jacobi PCG) and on the CPU oracle (Newton + direct solve)."""
     import torch
     m, sm, ctx, B, ix, mask = _setup("40x40-q2")
     s = sm.sizes()
