@@ -455,29 +455,36 @@ const double* residual_start(ipf_ctx* ctx, double* r) {
     return nullptr;
 }
 
-// One tangent (K v) launch for a group in the selected B format.
-void launch_group_tangent(ipf_ctx* ctx, const Group& g, const double* v, double* y, hipStream_t s) {
-    GroupDev d = group_dev(ctx, g);
-    if (ctx->dNodeR) d.node = ctx->dNodeR + g.dof_off / 2;  // flags masked off by the kernel
-    launch_tangent_fmt(ctx->tfmt, g.shape, d, group_fmt_B(ctx, g, ctx->tfmt), v, y, ctx->dDep, ctx->npad, s);
+// y = K v on stream s: the tangent launches of every group in the selected
+// B format.  When every DoF has a first write (fold_flin), the flagged entries
+// start from 0 and y needs no clear; else y is cleared first.
+void tangent_product(ipf_ctx* ctx, const double* v, double* y, hipStream_t s) {
+    const bool zf = ctx->fold_flin;
+    if (!zf) launch_fill(ctx->neq, y, 0.0, s);
+    for (const auto& g : ctx->groups) {
+        GroupDev d = group_dev(ctx, g);
+        if (zf) d.dof = ctx->dDofR + g.dof_off;
+        if (ctx->dNodeR) d.node = ctx->dNodeR + g.dof_off / 2;
+        launch_tangent_fmt(ctx->tfmt, g.shape, d, group_fmt_B(ctx, g, ctx->tfmt), v, y, ctx->dDep, ctx->npad, zf, s);
+    }
 }
 
 constexpr int kCgBlock = 10;  // CG iterations between convergence checks
 
 // `count` preconditioned CG iterations on stream s: q = K p (masked) and
-// p.q, alpha, x / r / z update and r.z, beta, new p and q = 0.  The scalars
-// stay on the device, so the sequence has no host dependency.
+// p.q, alpha, x / r / z update and r.z, beta, new p.  The scalars stay on
+// the device, so the sequence has no host dependency.
 void cg_iterations(ipf_ctx* ctx, double* x, int count, hipStream_t s) {
     const int64_t n = ctx->neq;
     const int nb = dot_blocks(n);
     for (int k = 0; k < count; k++) {
-        for (const auto& g : ctx->groups) launch_group_tangent(ctx, g, ctx->dCp, ctx->dCq, s);
+        tangent_product(ctx, ctx->dCp, ctx->dCq, s);
         launch_cg_mask_dot(n, ctx->dMask, ctx->dCp, ctx->dCq, ctx->dPartial, nb, ctx->dScal + 1, s);
         launch_cg_scalars(ctx->dScal, 0, s);
         launch_cg_update_dot(n, ctx->dScal, ctx->dCp, ctx->dCq, x, ctx->dCr, ctx->dPa, ctx->dPb, ctx->dCz, ctx->dPartial, nb,
                              ctx->dScal + 2, s);
         launch_cg_scalars(ctx->dScal, 1, s);
-        launch_cg_direction_clear(n, ctx->dScal, ctx->dCz, ctx->dCp, ctx->dCq, s);
+        launch_cg_direction(n, ctx->dScal, ctx->dCz, ctx->dCp, s);
     }
 }
 
@@ -1354,8 +1361,7 @@ int ipf_autotune(ipf_ctx* ctx, const double* u_dev, int reps, int allow_fp32, do
             ctx->tfmt = f;
             rc = time_it(
                 [&] {
-                    launch_fill(ctx->neq, ctx->dR, 0.0, ctx->stream);
-                    for (const auto& g : ctx->groups) launch_group_tangent(ctx, g, u_dev, ctx->dR, ctx->stream);
+                    tangent_product(ctx, u_dev, ctx->dR, ctx->stream);
                 },
                 &ms);
             if (rc) break;
@@ -1505,9 +1511,7 @@ int ipf_tangent_apply(ipf_ctx* ctx, const double* v_dev, double* y_dev) {
     if (!v_dev || !y_dev) return fail(ctx, IPF_EINVAL, "ipf_tangent_apply: NULL pointer");
     if (!ctx->have_tangent) return fail(ctx, IPF_ESTATE, "no tangent: call ipf_residual_tangent first");
     IPF_HIP(ctx, hipSetDevice(ctx->device));
-    launch_fill(ctx->neq, y_dev, 0.0, ctx->stream);
-    for (const auto& g : ctx->groups)
-        launch_group_tangent(ctx, g, v_dev, y_dev, ctx->stream);
+    tangent_product(ctx, v_dev, y_dev, ctx->stream);
     return launch_status(ctx, "tangent_apply");
 }
 
@@ -1541,10 +1545,9 @@ int ipf_cg_solve(ipf_ctx* ctx, const double* b_dev, double* x_dev, double rtol, 
     int it = 0;
     double rel = 0.0;
     if (bnorm > 0.0) {
-        // q = K p on the free DoFs; q starts at zero and every direction
-        // update clears it.  Blocks of kCgBlock iterations (one graph launch)
-        // between convergence checks; a shorter last block up to maxit.
-        launch_fill(n, ctx->dCq, 0.0, s);
+        // q = K p on the free DoFs.  Blocks of kCgBlock iterations (one
+        // graph launch) between convergence checks; a shorter last block up
+        // to maxit.
         while (it < maxit) {
             const int blk = std::min(kCgBlock, maxit - it);
             hipGraphExec_t exec = blk == kCgBlock ? cg_graph(ctx, x_dev) : nullptr;

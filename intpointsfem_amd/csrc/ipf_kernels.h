@@ -37,8 +37,8 @@ struct GroupDev {
     const double* tab;     // shape table of the group's shape (ipf_geometry.h layout), or null
     // [nelem * CE / 2] per element slot and node, the node's x DoF (its y DoF
     // is the next one), kFirstWrite-flagged like the residual's DoF copy; set
-    // for the non-committing residual when every element numbers its nodes'
-    // DoFs as consecutive (x, y) pairs, else null
+    // for the non-committing residual and the tangent product when every
+    // element numbers its nodes' DoFs as consecutive (x, y) pairs, else null
     const int* node;
 };
 
@@ -149,8 +149,11 @@ void launch_assemble_K(int shape, const GroupDev& g, const double* dep_soa, int6
 // needs no copy before the launches.
 void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
                          double* r, const double* flin, const PointState& st, hipStream_t stream);
+// The tangent launches accumulate y += B^T (w Dep) B v; with zero_first,
+// g.dof's (g.node's) entries flagged kFirstWrite start from 0 instead of y[d],
+// so the launches of all groups overwrite y with K v and need no clear.
 void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, const double* v, double* y,
-                        const double* dep_soa, int64_t npad, hipStream_t stream);
+                        const double* dep_soa, int64_t npad, bool zero_first, hipStream_t stream);
 // dense FP64 tiles of one group -> format fmt
 void launch_convert_B(int fmt, int shape, const double* dense, void* out, int64_t ntiles, hipStream_t stream);
 // Device setup of one group from geometry (corners: 8 doubles per element
@@ -177,14 +180,12 @@ void launch_cg_update(int64_t n, const double* scal, const double* p, const doub
                       const double* pa, const double* pb, double* z, hipStream_t stream);
 void launch_cg_direction(int64_t n, const double* scal, const double* z, double* p, hipStream_t stream);
 void launch_cg_scalars(double* scal, int stage, hipStream_t stream);
-// fused forms: q masked + p.q into *out; update + r.z into *out; direction + q = 0
+// fused forms: q masked + p.q into *out; update + r.z into *out
 void launch_cg_mask_dot(int64_t n, const signed char* mask, const double* p, double* q, double* partial, int nb,
                         double* out, hipStream_t stream);
 void launch_cg_update_dot(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
                           const double* pa, const double* pb, double* z, double* partial, int nb, double* out,
                           hipStream_t stream);
-void launch_cg_direction_clear(int64_t n, const double* scal, const double* z, double* p, double* q,
-                               hipStream_t stream);
 void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t stream);
 void launch_mask_copy(int64_t n, const signed char* mask, const double* src, double* dst, hipStream_t stream);
 int dot_blocks(int64_t n);

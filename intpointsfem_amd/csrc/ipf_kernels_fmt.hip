@@ -349,11 +349,13 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
 }
 
 // y += sum_e B_e^T (w Dep) B_e v_e with B in format FMT (FP32 formats: FP32
-// products and accumulation, the matvec of the inner CG solves).
+// products and accumulation, the matvec of the inner CG solves).  With
+// zero_first, index entries flagged kFirstWrite start from 0 instead of y[d]
+// (y = K v then needs no clear before the launches).
 template <int SHAPE, int FMT, bool PAIR>
 __global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
 k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ v, double* __restrict__ y,
-              const double* __restrict__ dep, int64_t npad) {
+              const double* __restrict__ dep, int64_t npad, bool zero_first) {
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
     constexpr int NSH = C::CE / 2;
@@ -371,7 +373,7 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     const bool slot = lane < C::TP;
     const int nv = tile_nvalid(g, tile);
     const bool valid = slot && el < nv;
-    // DoF (PAIR: node; flags masked off) indexes, then the v values they
+    // DoF (PAIR: node) indexes with their flags, then the v values they
     // address, outside branches (the addresses are in bounds as in
     // k_residual_fmt)
     constexpr int NG = (NE + kWave - 1) / kWave;
@@ -380,13 +382,13 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
 #pragma unroll
     for (int k = 0; k < NG; k++) {
         const int t = min(lane + k * kWave, NE - 1);
-        dl[k] = ld_stream(&(PAIR ? g.node : g.dof)[tile * (int64_t)NE + t]) & kDofMask;
+        dl[k] = ld_stream(&(PAIR ? g.node : g.dof)[tile * (int64_t)NE + t]);
     }
     double vl[NG], vl2[NG];
 #pragma unroll
     for (int k = 0; k < NG; k++) {
-        vl[k] = v[dl[k]];
-        vl2[k] = PAIR ? v[dl[k] + 1] : 0.0;
+        vl[k] = v[dl[k] & kDofMask];
+        vl2[k] = PAIR ? v[(dl[k] & kDofMask) + 1] : 0.0;
     }
     T b[C::NV];
     if constexpr (FmtTraits<FMT>::GEOM) {
@@ -428,17 +430,20 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     const bool task = lane < C::ET * (PAIR ? NSH : H) && re < nv;
     const bool first_half = !PAIR || rj < H / 2;
     double dp[9], y0 = 0.0, y1 = 0.0;
+    auto start = [&](int de, int off) -> double {
+        return (zero_first && de < 0) ? 0.0 : y[(de & kDofMask) + off];
+    };
     auto request = [&]() {
 #pragma unroll
         for (int k = 0; k < 9; k++) dp[k] = ld_stream(&dep[k * npad + P]);
         if (task) {
             if (PAIR) {
-                const int d = dsx[re * NSH + rj];
-                y0 = y[d];
-                y1 = y[d + 1];
+                const int de = dsx[re * NSH + rj];
+                y0 = start(de, 0);
+                y1 = start(de, 1);
             } else {
-                y0 = y[dsx[re * C::CE + rj]];
-                y1 = y[dsx[re * C::CE + H + rj]];
+                y0 = start(dsx[re * C::CE + rj], 0);
+                y1 = start(dsx[re * C::CE + H + rj], 0);
             }
         }
     };
@@ -460,24 +465,24 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     __syncthreads();
     if (PAIR) {
         if (task && first_half) {
-            const int d = dsx[re * NSH + rj];
+            const int d = dsx[re * NSH + rj] & kDofMask;
             reduce_lane<SHAPE, FMT, H>(lds, re, 2 * rj, d, y0, y);
             reduce_lane<SHAPE, FMT, H>(lds, re, 2 * rj + 1, d + 1, y1, y);
         }
     } else if (task) {
-        reduce_lane<SHAPE, FMT, H>(lds, re, rj, dsx[re * C::CE + rj], y0, y);
+        reduce_lane<SHAPE, FMT, H>(lds, re, rj, dsx[re * C::CE + rj] & kDofMask, y0, y);
     }
     __syncthreads();
     if (slot) bt_cols<SHAPE, FMT, H, C::CE>(b, x, lds + lane * H2);
     __syncthreads();
     if (PAIR) {
         if (task && !first_half) {
-            const int d = dsx[re * NSH + rj];
+            const int d = dsx[re * NSH + rj] & kDofMask;
             reduce_lane<SHAPE, FMT, H2>(lds, re, 2 * rj - H, d, y0, y);
             reduce_lane<SHAPE, FMT, H2>(lds, re, 2 * rj - H + 1, d + 1, y1, y);
         }
     } else if (task) {
-        reduce_lane<SHAPE, FMT, H2>(lds, re, rj, dsx[re * C::CE + H + rj], y1, y);
+        reduce_lane<SHAPE, FMT, H2>(lds, re, rj, dsx[re * C::CE + H + rj] & kDofMask, y1, y);
     }
 }
 
@@ -563,14 +568,14 @@ void residual_fmt(int shape, bool commit, const GroupDev& g, const void* Bv, con
 
 template <int FMT>
 void tangent_fmt(int shape, const GroupDev& g, const void* Bv, const double* v, double* y, const double* dep,
-                 int64_t npad, hipStream_t s) {
+                 int64_t npad, bool zf, hipStream_t s) {
     const dim3 grid((unsigned)g.ntiles), block(kWave);
     dispatch_shape(shape, [&](auto sc) {
         constexpr int S = decltype(sc)::value;
         if (g.node)
-            hipLaunchKernelGGL((k_tangent_fmt<S, FMT, true>), grid, block, 0, s, g, Bv, v, y, dep, npad);
+            hipLaunchKernelGGL((k_tangent_fmt<S, FMT, true>), grid, block, 0, s, g, Bv, v, y, dep, npad, zf);
         else
-            hipLaunchKernelGGL((k_tangent_fmt<S, FMT, false>), grid, block, 0, s, g, Bv, v, y, dep, npad);
+            hipLaunchKernelGGL((k_tangent_fmt<S, FMT, false>), grid, block, 0, s, g, Bv, v, y, dep, npad, zf);
     });
 }
 
@@ -599,14 +604,15 @@ void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, con
 }
 
 void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, const double* v, double* y,
-                        const double* dep, int64_t npad, hipStream_t s) {
+                        const double* dep, int64_t npad, bool zero_first, hipStream_t s) {
     if (g.ntiles == 0) return;
+    const bool zf = zero_first;
     switch (fmt) {
-        case FMT_D64: tangent_fmt<FMT_D64>(shape, g, Bv, v, y, dep, npad, s); break;
-        case FMT_C64: tangent_fmt<FMT_C64>(shape, g, Bv, v, y, dep, npad, s); break;
-        case FMT_G64: tangent_fmt<FMT_G64>(shape, g, Bv, v, y, dep, npad, s); break;
-        case FMT_D32: tangent_fmt<FMT_D32>(shape, g, Bv, v, y, dep, npad, s); break;
-        case FMT_C32: tangent_fmt<FMT_C32>(shape, g, Bv, v, y, dep, npad, s); break;
+        case FMT_D64: tangent_fmt<FMT_D64>(shape, g, Bv, v, y, dep, npad, zf, s); break;
+        case FMT_C64: tangent_fmt<FMT_C64>(shape, g, Bv, v, y, dep, npad, zf, s); break;
+        case FMT_G64: tangent_fmt<FMT_G64>(shape, g, Bv, v, y, dep, npad, zf, s); break;
+        case FMT_D32: tangent_fmt<FMT_D32>(shape, g, Bv, v, y, dep, npad, zf, s); break;
+        case FMT_C32: tangent_fmt<FMT_C32>(shape, g, Bv, v, y, dep, npad, zf, s); break;
         default: break;
     }
 }

@@ -178,15 +178,7 @@ k_cg_update_dot(int64_t n, const double* __restrict__ scal, const double* __rest
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-// p = z + beta p, and q = 0 for the next product
-__global__ void k_cg_direction_clear(int64_t n, const double* __restrict__ scal, const double* __restrict__ z,
-                                     double* __restrict__ p, double* __restrict__ q) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    p[i] = z[i] + scal[5] * p[i];
-    q[i] = 0.0;
-}
-
+// p = z + beta p
 __global__ void k_cg_direction(int64_t n, const double* __restrict__ scal, const double* __restrict__ z,
                                double* __restrict__ p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,11 +258,6 @@ void launch_cg_update_dot(int64_t n, const double* scal, const double* p, const 
                           hipStream_t s) {
     hipLaunchKernelGGL(k_cg_update_dot, dim3(nb), dim3(kDotThreads), 0, s, n, scal, p, q, x, r, pa, pb, z, partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, nb, partial, out);
-}
-
-void launch_cg_direction_clear(int64_t n, const double* scal, const double* z, double* p, double* q, hipStream_t s) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_cg_direction_clear, grid1(n), dim3(256), 0, s, n, scal, z, p, q);
 }
 
 int dot_blocks(int64_t n) {

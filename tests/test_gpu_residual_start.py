@@ -8,6 +8,11 @@ into r first.  Bars: both forms give bit-identical residuals on the shared
 DoFs, the unowned DoFs hold f_lin exactly, the result does not depend on what
 r held before the call (NaN-filled here), and the residual matches the oracle
 within 1e-12 of max(|f_int|, |f_lin|) (color accumulation order).
+
+The tangent product y = K v starts the same way: the first writes start
+from 0 when every DoF belongs to an element, else y is cleared first.  Bars:
+both forms give bit-identical K v on the shared DoFs, exact zeros on the
+unowned ones, whatever y held before (NaN-filled).
 """
 import numpy as np
 import pytest
@@ -34,7 +39,8 @@ def test_unowned_dofs_keep_f_lin(gpu, mesh, fmt):
     rng = np.random.default_rng(7)
     flin = np.concatenate([ix["rhs_linear"], rng.standard_normal(extra)])
     u = np.concatenate([wb.lame_displacement(m.nodes(), wb.normal_stress(10)), rng.standard_normal(extra)])
-    out = {}
+    v = np.concatenate([u[:neq] * 3.0 - 1e-4, rng.standard_normal(extra)])
+    out, kv = {}, {}
     (nip, nsh), tab = m.shape_table()
     corners = m.corners()
     ci = np.asarray(ix["color_indexes"], dtype=np.int64)
@@ -55,9 +61,17 @@ def test_unowned_dofs_keep_f_lin(gpu, mesh, fmt):
         ctx.residual(torch.as_tensor(u[:n], device=gpu), r)
         ctx.synchronize()
         out[n] = r.cpu().numpy()
+        ctx.set_b_format(fmt, tangent=fmt)
+        ctx.residual_tangent(torch.as_tensor(u[:n], device=gpu), torch.empty_like(r))
+        y = torch.full((n,), float("nan"), dtype=torch.float64, device=gpu)
+        ctx.tangent_apply(torch.as_tensor(v[:n], device=gpu), y)
+        ctx.synchronize()
+        kv[n] = y.cpu().numpy()
     folded, copied = out[neq], out[neq + extra]
     assert np.array_equal(folded, copied[:neq])
     assert np.array_equal(copied[neq:], flin[neq:])
+    assert np.array_equal(kv[neq], kv[neq + extra][:neq])
+    assert (kv[neq + extra][neq:] == 0).all() and np.abs(kv[neq]).max() > 0
     B = O.Blocks(b["row_sizes"], b["col_sizes"], b["matrix_position"], b["row_first"], b["col_first"], b["storage"])
     r_o, *_ = O.residual(B, ix["dof_indexes"], ix["color_indexes"], s["ncolor"], neq, ix["weight"], *mat, u[:neq],
                          f_lin=flin[:neq])
