@@ -479,11 +479,9 @@ void cg_iterations(ipf_ctx* ctx, double* x, int count, hipStream_t s) {
     const int nb = dot_blocks(n);
     for (int k = 0; k < count; k++) {
         tangent_product(ctx, ctx->dCp, ctx->dCq, s);
-        launch_cg_mask_dot(n, ctx->dMask, ctx->dCp, ctx->dCq, ctx->dPartial, nb, ctx->dScal + 1, s);
-        launch_cg_scalars(ctx->dScal, 0, s);
-        launch_cg_update_dot(n, ctx->dScal, ctx->dCp, ctx->dCq, x, ctx->dCr, ctx->dPa, ctx->dPb, ctx->dCz, ctx->dPartial, nb,
-                             ctx->dScal + 2, s);
-        launch_cg_scalars(ctx->dScal, 1, s);
+        launch_cg_mask_dot(n, ctx->dMask, ctx->dCp, ctx->dCq, ctx->dPartial, nb, ctx->dScal, s);
+        launch_cg_update_dot(n, ctx->dScal, ctx->dCp, ctx->dCq, x, ctx->dCr, ctx->dPa, ctx->dPb, ctx->dCz, ctx->dPartial,
+                             nb, s);
         launch_cg_direction(n, ctx->dScal, ctx->dCz, ctx->dCp, s);
     }
 }

@@ -179,13 +179,13 @@ void launch_cg_mask(int64_t n, const signed char* mask, double* q, hipStream_t s
 void launch_cg_update(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
                       const double* pa, const double* pb, double* z, hipStream_t stream);
 void launch_cg_direction(int64_t n, const double* scal, const double* z, double* p, hipStream_t stream);
-void launch_cg_scalars(double* scal, int stage, hipStream_t stream);
-// fused forms: q masked + p.q into *out; update + r.z into *out
+// fused forms on the CG scalars scal (rz, pq, rz_new, ., alpha, beta):
+// q masked, pq = p.q and alpha = rz / pq; update with alpha, rz_new = r.z,
+// beta = rz_new / rz and rz = rz_new
 void launch_cg_mask_dot(int64_t n, const signed char* mask, const double* p, double* q, double* partial, int nb,
-                        double* out, hipStream_t stream);
-void launch_cg_update_dot(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
-                          const double* pa, const double* pb, double* z, double* partial, int nb, double* out,
-                          hipStream_t stream);
+                        double* scal, hipStream_t stream);
+void launch_cg_update_dot(int64_t n, double* scal, const double* p, const double* q, double* x, double* r,
+                          const double* pa, const double* pb, double* z, double* partial, int nb, hipStream_t stream);
 void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t stream);
 void launch_mask_copy(int64_t n, const signed char* mask, const double* src, double* dst, hipStream_t stream);
 int dot_blocks(int64_t n);

@@ -186,14 +186,24 @@ __global__ void k_cg_direction(int64_t n, const double* __restrict__ scal, const
     p[i] = z[i] + scal[5] * p[i];
 }
 
-// stage 0: alpha = rz / pq;  stage 1: beta = rz_new / rz, rz = rz_new
-__global__ void k_cg_scalars(double* scal, int stage) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The block partials summed as k_dot_final does, then the CG scalars
+// (scal: rz, pq, rz_new, ., alpha, beta).  stage 0: pq = the sum, alpha =
+// rz / pq;  stage 1: rz_new = the sum, beta = rz_new / rz, rz = rz_new.
+__global__ void __launch_bounds__(kDotThreads)
+k_cg_final(int nb, const double* __restrict__ partial, double* __restrict__ scal, int stage) {
+    __shared__ double sh[kDotThreads];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kDotThreads) s = s + partial[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x != 0) return;
     if (stage == 0) {
-        scal[4] = scal[1] != 0.0 ? scal[0] / scal[1] : 0.0;
+        scal[1] = t;
+        scal[4] = t != 0.0 ? scal[0] / t : 0.0;
     } else {
-        scal[5] = scal[0] != 0.0 ? scal[2] / scal[0] : 0.0;
-        scal[0] = scal[2];
+        const double rz = scal[0];
+        scal[2] = t;
+        scal[5] = rz != 0.0 ? t / rz : 0.0;
+        scal[0] = t;
     }
 }
 
@@ -248,16 +258,15 @@ void launch_segment_sumsq(const double* x, const signed char* mask, const int64_
 }
 
 void launch_cg_mask_dot(int64_t n, const signed char* mask, const double* p, double* q, double* partial, int nb,
-                        double* out, hipStream_t s) {
+                        double* scal, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_mask_dot, dim3(nb), dim3(kDotThreads), 0, s, n, mask, p, q, partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, nb, partial, out);
+    hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(kDotThreads), 0, s, nb, partial, scal, 0);
 }
 
-void launch_cg_update_dot(int64_t n, const double* scal, const double* p, const double* q, double* x, double* r,
-                          const double* pa, const double* pb, double* z, double* partial, int nb, double* out,
-                          hipStream_t s) {
+void launch_cg_update_dot(int64_t n, double* scal, const double* p, const double* q, double* x, double* r,
+                          const double* pa, const double* pb, double* z, double* partial, int nb, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_update_dot, dim3(nb), dim3(kDotThreads), 0, s, n, scal, p, q, x, r, pa, pb, z, partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotThreads), 0, s, nb, partial, out);
+    hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(kDotThreads), 0, s, nb, partial, scal, 1);
 }
 
 int dot_blocks(int64_t n) {
@@ -296,9 +305,6 @@ void launch_cg_direction(int64_t n, const double* scal, const double* z, double*
     if (n) hipLaunchKernelGGL(k_cg_direction, grid1(n), dim3(256), 0, s, n, scal, z, p);
 }
 
-void launch_cg_scalars(double* scal, int stage, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1), 0, s, scal, stage);
-}
 
 void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_axpy, grid1(n), dim3(256), 0, s, n, a, x, y);
