@@ -130,7 +130,31 @@ k_cg_mask_dot(int64_t n, const signed char* __restrict__ mask, const double* __r
               double* __restrict__ partial) {
     __shared__ double sh[kDotThreads];
     double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kDotThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kDotThreads) {
+    const int64_t S = (int64_t)gridDim.x * kDotThreads;
+    int64_t i = (int64_t)blockIdx.x * kDotThreads + threadIdx.x;
+    // batches of U grid strides: every load of the batch in flight at once,
+    // then the products added in stride order (the order of the loop below)
+    constexpr int U = 4;
+    for (; i + (U - 1) * S < n; i += U * S) {
+        double pv[U], qv[U];
+        signed char mv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            pv[u] = p[i + u * S];
+            qv[u] = q[i + u * S];
+            mv[u] = mask[i + u * S];
+        }
+        asm volatile("" ::: "memory");  // the loads above are issued before any use
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (mv[u]) {
+                qv[u] = 0.0;
+                q[i + u * S] = 0.0;
+            }
+            s = s + pv[u] * qv[u];
+        }
+    }
+    for (; i < n; i += S) {
         double qi = q[i];
         if (mask[i]) {
             qi = 0.0;
@@ -151,9 +175,44 @@ k_cg_update_dot(int64_t n, const double* __restrict__ scal, const double* __rest
     __shared__ double sh[kDotThreads];
     const double alpha = scal[4];
     double s = 0.0;
-    // one DoF pair per thread and step (see k_cg_init)
-    for (int64_t k = (int64_t)blockIdx.x * kDotThreads + threadIdx.x; 2 * k < n;
-         k += (int64_t)gridDim.x * kDotThreads) {
+    // one DoF pair per thread and step (see k_cg_init); batches of U whole
+    // pairs with every load in flight at once, then the same arithmetic and
+    // the same order of the r.z terms as the loop below
+    const int64_t S = (int64_t)gridDim.x * kDotThreads;
+    int64_t k = (int64_t)blockIdx.x * kDotThreads + threadIdx.x;
+    constexpr int U = 2;
+    for (; 2 * (k + (U - 1) * S) + 1 < n; k += U * S) {
+        double xv[2 * U], pv[2 * U], qv[2 * U], rv[2 * U], av[2 * U], bv[2 * U];
+#pragma unroll
+        for (int j = 0; j < 2 * U; j++) {
+            const int64_t i = 2 * (k + (j >> 1) * S) + (j & 1);
+            xv[j] = x[i];
+            pv[j] = p[i];
+            qv[j] = q[i];
+            rv[j] = r[i];
+            av[j] = pa[i];
+            bv[j] = pb[i];
+        }
+        asm volatile("" ::: "memory");  // the loads above are issued before any use
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t i0 = 2 * (k + u * S), i1 = i0 + 1;
+            const int j0 = 2 * u, j1 = j0 + 1;
+            x[i0] = xv[j0] + alpha * pv[j0];
+            const double r0 = rv[j0] - alpha * qv[j0];
+            r[i0] = r0;
+            x[i1] = xv[j1] + alpha * pv[j1];
+            const double r1 = rv[j1] - alpha * qv[j1];
+            r[i1] = r1;
+            const double z0 = av[j0] * r0 + bv[j0] * r1;
+            z[i0] = z0;
+            s = s + r0 * z0;
+            const double z1 = av[j1] * r1 + bv[j1] * r0;
+            z[i1] = z1;
+            s = s + r1 * z1;
+        }
+    }
+    for (; 2 * k < n; k += S) {
         const int64_t i0 = 2 * k, i1 = i0 + 1;
         const bool has1 = i1 < n;
         x[i0] = x[i0] + alpha * p[i0];
