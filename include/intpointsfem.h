@@ -298,7 +298,8 @@ int ipf_set_tangent_pattern(ipf_ctx *ctx, const int *el_color_index, const int64
 int ipf_assemble_tangent(ipf_ctx *ctx, const double *u_dev, double *r_dev, double *kg_dev);
 
 /* y = K v with K = sum_e B_e^T (w Dep) B_e from the last ipf_residual_tangent
- * (all DoFs, no mask), device pointers [neq]. */
+ * (all DoFs, no mask), device pointers [neq].  y is overwritten entirely
+ * (its contents before the call are not read); v and y must not overlap. */
 int ipf_tangent_apply(ipf_ctx *ctx, const double *v_dev, double *y_dev);
 
 /* Block-Jacobi-preconditioned CG on the free DoFs (each node's 2x2 diagonal

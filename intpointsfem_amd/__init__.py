@@ -469,6 +469,8 @@ class DeviceContext:
         self._c(lib().ipf_assemble_tangent(self._h, _ptr(u), _ptr(r), _ptr(kg)), "ipf_assemble_tangent")
 
     def tangent_apply(self, v, y):
+        """y = K v from the last residual_tangent (ipf_tangent_apply); y is
+        overwritten whatever it held."""
         self._c(lib().ipf_tangent_apply(self._h, _ptr(v), _ptr(y)), "ipf_tangent_apply")
 
     def autotune(self, u, reps=20, allow_fp32=False):
