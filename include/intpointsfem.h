@@ -127,6 +127,15 @@ int ipf_set_material(ipf_ctx *ctx, double lambda, double mu, double mc_phi, doub
  * covers all materials).  Without it every element has material 0. */
 int ipf_set_material_table(ipf_ctx *ctx, int nmat, const double *params);
 int ipf_set_element_materials(ipf_ctx *ctx, int64_t nblocks, const int *element_material);
+/* Order of the elements inside each (color, shape, material) run of the next
+ * ipf_transfer_data / ipf_transfer_geometry: ascending keys [nblocks], then
+ * element index (default: all keys 0, index order).  The order decides which
+ * elements share a wavefront tile -- e.g. predicted-elastic elements
+ * together, so that whole tiles take the certified elastic path -- and
+ * nothing else: within a color every DoF has one element, so residuals,
+ * tangent products and states are bit-identical for any keys.  No reference
+ * counterpart. */
+int ipf_set_element_order_keys(ipf_ctx *ctx, int64_t nblocks, const int *keys);
 
 /* fRhsLinear / dRhsLinear (TPZElastoPlasticIntPointsStructMatrix.cpp:63-68,156-167), [neq] host array. */
 int ipf_set_rhs_linear(ipf_ctx *ctx, const double *f_lin_host);
@@ -152,6 +161,81 @@ int ipf_info(const ipf_ctx *ctx, int64_t out[8]);
  * overlap.  Asynchronous on the context's stream. */
 int ipf_residual(ipf_ctx *ctx, const double *u_dev, double *r_dev);
 int ipf_residual_host(ipf_ctx *ctx, const double *u_host, double *r_host);
+
+/* How non-committing fused residuals (ipf_residual, ipf_residual_timed with
+ * UpdateMem off) evaluate the stress.  No reference counterpart.
+ *   IPF_RESIDUAL_EXACT (default): the reference's return map at every point;
+ *     branch flags, stresses and committed state bit-identical to the oracle.
+ *   IPF_RESIDUAL_CERTIFIED_ELASTIC: a point whose trial stress is elastic
+ *     with a margin -- its closed-form principal stresses below the main
+ *     Mohr-Coulomb plane by 2e-3 of its largest stress entry, more than the
+ *     reference's eigenvalue solver (20 Newton steps, |f| <= 1e-11 on the
+ *     normalised cubic, FunctionsSpectralDecomp.h:61-111) can be off -- is
+ *     elastic in the reference too, and takes its trial stress instead of
+ *     the eigen-reconstruction of it (FunctionsStressStrain.h:60-68): the
+ *     same stress up to that reconstruction's rounding and the solver's
+ *     tolerance.  Every other point takes the exact return map; a wavefront
+ *     of certified points skips the spectral decomposition altogether.
+ * Committing residuals (UpdateMem on) always run the exact path, so the
+ * committed state never depends on the mode; neither do ipf_residual_tangent,
+ * the separate steps or the Newton solver's residuals. */
+enum { IPF_RESIDUAL_EXACT = 0, IPF_RESIDUAL_CERTIFIED_ELASTIC = 1 };
+int ipf_set_residual_mode(ipf_ctx *ctx, int mode);
+int ipf_get_residual_mode(const ipf_ctx *ctx, int *mode);
+/* How non-committing fused residuals accumulate r.  No reference
+ * counterpart (the reference scatters into color slabs,
+ * TPZNumericalIntegrator.cpp:81-105).
+ *   IPF_ASSEMBLY_COLORED (default): one launch per element color, each
+ *     adding its elements' forces to r in color order -- the reference's
+ *     per-DoF summation order.
+ *   IPF_ASSEMBLY_PATCH: compact patches of elements (a Morton order of the
+ *     element centroids, cut per element shape and material), one
+ *     workgroup per patch: each patch's nodes are gathered once, its
+ *     elements' forces summed per node in LDS, nodes owned by the patch
+ *     written once (r = f_lin - sum, r never read), nodes shared with other
+ *     patches finished by a fix-up pass over per-patch partial sums in a
+ *     fixed order.  Deterministic, atomic-free; the same residual up to the
+ *     order of the sums (a few ulps of the element forces).  Needs the
+ *     geometric setup (ipf_transfer_geometry) and every element's DoFs in
+ *     (x, y) node pairs (IPF_EUNSUPPORTED otherwise); built on the first
+ *     selection after each transfer.
+ * Committing residuals always take the colored path. */
+enum { IPF_ASSEMBLY_COLORED = 0, IPF_ASSEMBLY_PATCH = 1 };
+int ipf_set_residual_assembly(ipf_ctx *ctx, int assembly);
+int ipf_get_residual_assembly(const ipf_ctx *ctx, int *assembly);
+/* out[0..7] = built (0/1), patches, patch nodes, partial slots, shared
+ * nodes, DoFs of no element, patch points (padded), element shapes. */
+int ipf_patch_info(const ipf_ctx *ctx, int64_t out[8]);
+
+/* The patch layout of IPF_ASSEMBLY_PATCH planned on the host (no GPU), for
+ * inspection and tests: n elements with their shape (kernel shape index:
+ * 0..2 Q1..Q3, 3..5 T1..T3), material, first integration point, centroid
+ * and node x-DoFs (shape_ce / 2 per element, concatenated; y DoF = x + 1),
+ * neq equations.  sizes[12] = ranges, patches, tiles, slots, local node
+ * entries, patch nodes, contributions, patch points, shared nodes, partial
+ * slot entries of the shared nodes, orphan DoFs, partial slots.  Each output
+ * array may be NULL (call once with NULL outputs for the sizes): ranges
+ * [7 per range] = (shape, patches, first patch, first tile, first slot,
+ * first point, first local node entry); tmeta [tiles]; slot_elem [slots]
+ * (element index, -1 padding); lnode [local node entries]; node_off
+ * [patches + 1]; pnode, pslot [patch nodes]; cbeg [patch nodes + 1];
+ * contrib [contributions]; pt_map [patch points] (the element's first point
+ * + point, -1 padding); fix_dof [shared], fix_ptr [shared + 1], fix_slot;
+ * orphan [orphan DoFs]. */
+int ipf_patch_plan_host(int64_t n, const int *shape, const int *mat, const int64_t *first_point, const double *cx,
+                        const double *cy, const int *node_xdof, int64_t neq, int64_t *sizes, int64_t *ranges,
+                        int *tmeta, int64_t *slot_elem, unsigned short *lnode, int64_t *node_off, int *pnode,
+                        int *pslot, int *cbeg, unsigned short *contrib, int64_t *pt_map, int *fix_dof, int *fix_ptr,
+                        int *fix_slot, int *orphan);
+
+/* The certificate of IPF_RESIDUAL_CERTIFIED_ELASTIC on the host, point by
+ * point (no GPU): strain3 [3n] ([xx, xy, yy], engineering shear),
+ * plastic_strain4 [4n] ([XX, XY, YY, ZZ]); certified [n] = 1 where the
+ * point passes, and then sigma3 [3n] (optional) = its trial stress
+ * [sxx, sxy, syy], else 0. */
+int ipf_certify_elastic_host(int64_t n, const double *strain3, const double *plastic_strain4, double lambda,
+                             double mu, double mc_phi, double mc_psi, double mc_cohesion, int *certified,
+                             double *sigma3);
 
 /* ipf_residual with a HIP event pair around every kernel launch on the
  * context's stream; waits, then writes the duration of each launch in
@@ -251,6 +335,7 @@ int ipf_point_features(ipf_ctx *ctx, const double *u_dev, float *feat_dev, int *
 int ipf_branch_order(ipf_ctx *ctx, const int *branch_dev, int *order_dev);
 int ipf_compute_sigma_ordered(ipf_ctx *ctx, const double *strain_dev, double *wsigma_dev, const int *order_dev);
 
+
 typedef struct ipf_mlp ipf_mlp;
 int ipf_mlp_create(int device, uint64_t seed, ipf_mlp **out);
 int ipf_mlp_destroy(ipf_mlp *mlp);
@@ -311,12 +396,22 @@ int ipf_cg_solve(ipf_ctx *ctx, const double *b_dev, double *x_dev, double rtol, 
                  double *relres);
 
 /* Newton for the current load (f_lin): u <- u + du until the free-DoF residual
- * satisfies ||r|| <= rtol * ||r0|| (r0: residual at the entry u), then commits
- * the state at the converged u (AcceptPseudoTimeStepSolution, main.cpp:487-522).
- * Solution / Newton loop of main.cpp:160-262 with CG in place of LDL^T.
+ * satisfies ||r|| <= rtol * ||r0|| (r0: residual at the entry u) or ||r|| <=
+ * atol (ipf_set_newton_atol, default 0), then commits the state at the
+ * converged u (AcceptPseudoTimeStepSolution, main.cpp:487-522).
+ * Solution / Newton loop of main.cpp:160-262 with CG in place of LDL^T; the
+ * reference stops on absolute norms, ||r|| < tol and ||du|| < tol
+ * (main.cpp:235) -- the relative test here, the absolute floor for a call
+ * that starts at an already converged u (its r0 is at rounding level, so the
+ * relative test alone could not be met).  The caller's UpdateMem switch is
+ * the same after the call, whatever its status.
  * stats[0..7] = Newton iterations, converged (0/1), ||r||/||r0||, ||r0||,
  * ||r||, total CG iterations, worst CG relative residual, 0. */
 int ipf_newton(ipf_ctx *ctx, double *u_dev, double rtol, int maxit, double cg_rtol, int cg_maxit, double *stats);
+
+/* Absolute residual floor of ipf_newton / ipf_newton_segmented (per
+ * segment): also converged when ||r|| <= atol.  atol >= 0; 0 turns it off. */
+int ipf_set_newton_atol(ipf_ctx *ctx, double atol);
 
 /* ipf_newton for independent blocks of equations (batched instances): segment
  * k holds the equations [seg_first[k], seg_first[k+1]) (seg_first [nseg+1],

@@ -107,6 +107,30 @@ int ipf_sm_create_tangent(ipf_sm *sm, int64_t *nnz);
  * fFirstColorLIndex [ncolor+1] of that pattern; any may be NULL. */
 int ipf_sm_export_tangent_pattern(const ipf_sm *sm, int64_t *ia, int64_t *ja, int64_t *color_l_sequence,
                                   int64_t *first_color_l_index);
+/* Clone() (TPZElastoPlasticIntPointsStructMatrix.h:33, .cpp:30-32): a second
+ * structural matrix on the same mesh that shares the first one's device
+ * context (buffers, committed state, stream) by reference count, so either
+ * may be destroyed first -- the reference's copy duplicates its TPZVecGPU
+ * handles and frees them twice (TPZVecGPU.h:24-27).  The mesh must outlive
+ * both. */
+int ipf_sm_clone(const ipf_sm *sm, ipf_sm **out);
+/* CreateAssemble(rhs, gui) (TPZElastoPlasticIntPointsStructMatrix.h:37,
+ * .cpp:71-77) at u (host array [neq], becomes the mesh's solution): a new
+ * matrix from Create() (CSR pattern, FillLIndexes, upload), rhs [neq] and the
+ * matrix's values from Assemble(K, rhs).  *nnz = values of the new matrix,
+ * read with ipf_sm_tangent_values; the matrix replaces any earlier one of
+ * this structural matrix.  Commits the state when the mesh's UpdateMem is
+ * set.  The reference's gui argument (a progress hook) has no use here. */
+int ipf_sm_create_assemble(ipf_sm *sm, const double *u_host, double *rhs_host, int64_t *nnz);
+/* Values [nnz] of the matrix the last ipf_sm_create_assemble /
+ * ipf_sm_assemble_tangent filled (IPF_ESTATE before any). */
+int ipf_sm_tangent_values(const ipf_sm *sm, double *kg_host);
+/* TPZIrregularBlocksMatrix::CSRVectors (TPZIrregularBlocksMatrix.cpp:177-195):
+ * the block matrix as CSR over its stored values -- row_ptr [rows+1] (row r
+ * of block e starts at fMatrixPosition[e] + r * C_e), col_ind [stored values]
+ * (fColFirstIndex[e] + c); the values are fStorage (ipf_sm_export_blocks).
+ * int indexes, as the reference's.  Host work. */
+int ipf_sm_export_csr(ipf_sm *sm, int *row_ptr, int *col_ind);
 /* Assemble(K, rhs) for u (TPZElastoPlasticIntPointsStructMatrix.cpp:98-135):
  * the tangent's CSR values kg [nnz] and rhs [neq], host arrays; commits the
  * state when the mesh's UpdateMem is set. */

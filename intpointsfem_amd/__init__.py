@@ -29,6 +29,8 @@ _LIB_PATH = os.path.join(_HERE, "libintpointsfem.so")
 IPF_OK = 0
 BRANCH_NAMES = ("elastic", "main", "right_edge", "left_edge", "apex")
 B_FORMATS = ("dense-fp64", "compact-fp64", "dense-fp32", "compact-fp32", "geometry-fp64")
+RESIDUAL_MODES = ("exact", "certified-elastic")
+ASSEMBLIES = ("colored", "patch")
 
 
 class IPFError(RuntimeError):
@@ -79,6 +81,7 @@ def lib():
         "ipf_set_material": (_int, [_vp, _d, _d, _d, _d, _d]),
         "ipf_set_material_table": (_int, [_vp, _int, _dp]),
         "ipf_set_element_materials": (_int, [_vp, _i64, _ip]),
+        "ipf_set_element_order_keys": (_int, [_vp, _i64, _ip]),
         "ipf_set_rhs_linear": (_int, [_vp, _dp]),
         "ipf_set_update_mem": (_int, [_vp, _int]),
         "ipf_info": (_int, [_vp, _lp]),
@@ -100,6 +103,15 @@ def lib():
         "ipf_cg_solve": (_int, [_vp, _vp, _vp, _d, _int, _ip, _dp]),
         "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
         "ipf_set_cg_graph": (_int, [_vp, _int]),
+        "ipf_set_newton_atol": (_int, [_vp, _d]),
+        "ipf_set_residual_mode": (_int, [_vp, _int]),
+        "ipf_get_residual_mode": (_int, [_vp, _ip]),
+        "ipf_certify_elastic_host": (_int, [_i64, _dp, _dp, _d, _d, _d, _d, _d, _ip, _dp]),
+        "ipf_set_residual_assembly": (_int, [_vp, _int]),
+        "ipf_get_residual_assembly": (_int, [_vp, _ip]),
+        "ipf_patch_info": (_int, [_vp, _lp]),
+        "ipf_patch_plan_host": (_int, [_i64, _ip, _ip, _lp, _dp, _dp, _ip, _i64, _lp, _lp, _ip, _lp, _vp, _lp,
+                                       _ip, _ip, _ip, _vp, _lp, _ip, _ip, _ip, _ip]),
         "ipf_newton_segmented": (_int, [_vp, _vp, _int, _lp, _d, _int, _d, _int, _dp, _dp, _ip]),
         "ipf_autotune": (_int, [_vp, _vp, _int, _int, _dp]),
         "ipf_get_state": (_int, [_vp, _dp, _dp, _ip, _ip]),
@@ -143,6 +155,10 @@ def lib():
         "ipf_sm_create_tangent": (_int, [_vp, _lp]),
         "ipf_sm_export_tangent_pattern": (_int, [_vp, _lp, _lp, _lp, _lp]),
         "ipf_sm_assemble_tangent": (_int, [_vp, _dp, _dp, _dp]),
+        "ipf_sm_clone": (_int, [_vp, ctypes.POINTER(_vp)]),
+        "ipf_sm_create_assemble": (_int, [_vp, _dp, _dp, _lp]),
+        "ipf_sm_tangent_values": (_int, [_vp, _dp]),
+        "ipf_sm_export_csr": (_int, [_vp, _ip, _ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -150,6 +166,47 @@ def lib():
         f.argtypes = args
     _lib = L
     return L
+
+
+def patch_plan(shape, mat, first_point, cx, cy, node_xdof, neq):
+    """The patch layout planned on the host (ipf_patch_plan_host): a dict of numpy arrays."""
+    n = len(shape)
+    sh = np.ascontiguousarray(shape, dtype=np.int32)
+    mt = np.ascontiguousarray(mat, dtype=np.int32)
+    fp = np.ascontiguousarray(first_point, dtype=np.int64)
+    x = np.ascontiguousarray(cx, dtype=np.float64)
+    y = np.ascontiguousarray(cy, dtype=np.float64)
+    nd = np.ascontiguousarray(node_xdof, dtype=np.int32)
+    sizes = np.zeros(12, dtype=np.int64)
+    args = [n, _p(sh, _ip), _p(mt, _ip), _p(fp, _lp), _p(x, _dp), _p(y, _dp), _p(nd, _ip), int(neq), _p(sizes, _lp)]
+    _check(lib().ipf_patch_plan_host(*args, *([None] * 14)), what="ipf_patch_plan_host")
+    nr, npch, nt, ns, nl, npn, nc, npt, nsh, nfs, nor, npart = (int(v) for v in sizes)
+    out = dict(ranges=np.zeros(7 * nr, np.int64), tmeta=np.zeros(nt, np.int32), slot_elem=np.zeros(ns, np.int64),
+               lnode=np.zeros(nl, np.uint16), node_off=np.zeros(npch + 1, np.int64), pnode=np.zeros(npn, np.int32),
+               pslot=np.zeros(npn, np.int32), cbeg=np.zeros(npn + 1, np.int32), contrib=np.zeros(nc, np.uint16),
+               pt_map=np.zeros(npt, np.int64), fix_dof=np.zeros(nsh, np.int32), fix_ptr=np.zeros(nsh + 1, np.int32),
+               fix_slot=np.zeros(nfs, np.int32), orphan=np.zeros(nor, np.int32))
+    ptrs = [_p(out["ranges"], _lp), _p(out["tmeta"], _ip), _p(out["slot_elem"], _lp), out["lnode"].ctypes.data,
+            _p(out["node_off"], _lp), _p(out["pnode"], _ip), _p(out["pslot"], _ip), _p(out["cbeg"], _ip),
+            out["contrib"].ctypes.data, _p(out["pt_map"], _lp), _p(out["fix_dof"], _ip), _p(out["fix_ptr"], _ip),
+            _p(out["fix_slot"], _ip), _p(out["orphan"], _ip)]
+    _check(lib().ipf_patch_plan_host(*args, *ptrs), what="ipf_patch_plan_host")
+    out["ranges"] = out["ranges"].reshape(-1, 7)
+    out["npartial"] = npart
+    return out
+
+
+def certify_elastic(strain3, plastic_strain4, lam, mu, phi, psi, cohesion):
+    """Host evaluation of the certified-elastic test (ipf_certify_elastic_host):
+    returns (certified [n] bool, trial stress [n, 3])."""
+    e = np.ascontiguousarray(strain3, dtype=np.float64).reshape(-1, 3)
+    ep = np.ascontiguousarray(plastic_strain4, dtype=np.float64).reshape(-1, 4)
+    n = e.shape[0]
+    cert = np.zeros(n, dtype=np.int32)
+    sig = np.zeros((n, 3))
+    _check(lib().ipf_certify_elastic_host(n, _p(e, _dp), _p(ep, _dp), lam, mu, phi, psi, cohesion, _p(cert, _ip),
+                                          _p(sig, _dp)), what="ipf_certify_elastic_host")
+    return cert.astype(bool), sig
 
 
 def exported_symbols():
@@ -375,6 +432,11 @@ class DeviceContext:
         a = np.ascontiguousarray(ids, dtype=np.int32)
         self._c(lib().ipf_set_element_materials(self._h, a.size, _p(a, _ip)), "ipf_set_element_materials")
 
+    def set_element_order_keys(self, keys):
+        """Order key per block inside each (color, shape, material) run; applies to the next transfer."""
+        a = np.ascontiguousarray(keys, dtype=np.int32)
+        self._c(lib().ipf_set_element_order_keys(self._h, a.size, _p(a, _ip)), "ipf_set_element_order_keys")
+
     def set_rhs_linear(self, f):
         f = np.ascontiguousarray(f, dtype=np.float64)
         self._c(lib().ipf_set_rhs_linear(self._h, _p(f, _dp)), "ipf_set_rhs_linear")
@@ -512,6 +574,35 @@ class DeviceContext:
                 "ipf_cg_solve")
         return it.value, rel.value
 
+    def set_residual_mode(self, mode):
+        """'exact' (0) or 'certified-elastic' (1): see ipf_set_residual_mode."""
+        code = RESIDUAL_MODES.index(mode) if isinstance(mode, str) else int(mode)
+        self._c(lib().ipf_set_residual_mode(self._h, code), "ipf_set_residual_mode")
+
+    def residual_mode(self):
+        m = ctypes.c_int()
+        self._c(lib().ipf_get_residual_mode(self._h, ctypes.byref(m)), "ipf_get_residual_mode")
+        return RESIDUAL_MODES[m.value]
+
+    def set_residual_assembly(self, assembly):
+        """'colored' (0) or 'patch' (1): see ipf_set_residual_assembly."""
+        code = ASSEMBLIES.index(assembly) if isinstance(assembly, str) else int(assembly)
+        self._c(lib().ipf_set_residual_assembly(self._h, code), "ipf_set_residual_assembly")
+
+    def residual_assembly(self):
+        m = ctypes.c_int()
+        self._c(lib().ipf_get_residual_assembly(self._h, ctypes.byref(m)), "ipf_get_residual_assembly")
+        return ASSEMBLIES[m.value]
+
+    def patch_info(self):
+        out = np.zeros(8, dtype=np.int64)
+        self._c(lib().ipf_patch_info(self._h, _p(out, _lp)), "ipf_patch_info")
+        keys = ("built", "patches", "patch_nodes", "partials", "shared_nodes", "orphan_dofs", "points", "shapes")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def set_newton_atol(self, atol):
+        self._c(lib().ipf_set_newton_atol(self._h, float(atol)), "ipf_set_newton_atol")
+
     def newton(self, u, rtol=1e-10, maxit=50, cg_rtol=1e-10, cg_maxit=20000):
         """Newton for the current load; commits the state. Returns a stats dict."""
         st = np.zeros(8)
@@ -607,12 +698,40 @@ class StructMatrix:
     ``device >= 0`` uploads to that GPU; ``device < 0`` builds the host data
     only (exports work without a GPU)."""
 
-    def __init__(self, mesh, device=0):
-        h = _vp()
-        _check(lib().ipf_sm_create(mesh.handle, device, ctypes.byref(h)), what="ipf_sm_create")
+    def __init__(self, mesh, device=0, _handle=None):
+        if _handle is None:
+            h = _vp()
+            _check(lib().ipf_sm_create(mesh.handle, device, ctypes.byref(h)), what="ipf_sm_create")
+        else:
+            h = _handle
         self._h = h
         self.mesh = mesh
         self.device = device
+
+    def clone(self):
+        """Clone(): a structural matrix sharing this one's device context."""
+        h = _vp()
+        _check(lib().ipf_sm_clone(self._h, ctypes.byref(h)), what="ipf_sm_clone")
+        return StructMatrix(self.mesh, self.device, _handle=h)
+
+    def create_assemble(self, u):
+        """CreateAssemble(rhs, gui) at u: returns (rhs, kg) of a new matrix."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.zeros_like(u)
+        n = np.zeros(1, dtype=np.int64)
+        _check(lib().ipf_sm_create_assemble(self._h, _p(u, _dp), _p(r, _dp), _p(n, _lp)),
+               what="ipf_sm_create_assemble")
+        kg = np.zeros(int(n[0]))
+        _check(lib().ipf_sm_tangent_values(self._h, _p(kg, _dp)), what="ipf_sm_tangent_values")
+        return r, kg
+
+    def export_csr(self):
+        """CSRVectors(): (row_ptr [rows+1], col_ind [stored values]) of the block matrix."""
+        s = self.sizes()
+        rp = np.zeros(s["rows"] + 1, np.int32)
+        ci = np.zeros(s["nstorage"], np.int32)
+        _check(lib().ipf_sm_export_csr(self._h, _p(rp, _ip), _p(ci, _ip)), what="ipf_sm_export_csr")
+        return rp, ci
 
     def __del__(self):
         h = getattr(self, "_h", None)

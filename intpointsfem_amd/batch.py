@@ -174,17 +174,12 @@ class Batch:
             raise ValueError(f"setup must be 'device' or 'host', not {setup!r}")
         h = concat_geometry(meshes) if setup == "device" else concat_instances(meshes)
         self.ctx = DeviceContext(device=device)
-        self.ctx.set_element_materials(h["element_material"])
-        if setup == "device":
-            self.ctx.transfer_geometry(h["shape_sizes"], h["shape_tables"], h["elem_shape"], h["corners"],
-                                       h["dof_indexes"], h["color_indexes"], h["ncolor"], h["neq"])
-        else:
-            self.ctx.transfer_data(h["row_sizes"], h["col_sizes"], h["matrix_position"], h["row_first"],
-                                   h["col_first"], h["storage"], h["dof_indexes"], h["color_indexes"], h["ncolor"],
-                                   h["neq"], h["weight"])
-        self.ctx.set_material_table(h["material_table"])
-        self.ctx.set_rhs_linear(h["rhs_linear"])
+        self._h = h
+        self._setup = setup
+        self.mask = None
+        self._transfer()
         self.n = len(meshes)
+        self.element_offsets = h["block_offsets"]
         self.neq = h["neq"]
         self.npts = int(h["pt_offsets"][-1])
         self.ncolor = h["ncolor"]
@@ -194,6 +189,44 @@ class Batch:
         self.row_sizes = h["row_sizes"]
         self.col_sizes = h["col_sizes"]
         self.meshes = list(meshes)
+
+    def _transfer(self, keys=None):
+        h = self._h
+        self.ctx.set_element_materials(h["element_material"])
+        if keys is not None:
+            self.ctx.set_element_order_keys(keys)
+        if self._setup == "device":
+            self.ctx.transfer_geometry(h["shape_sizes"], h["shape_tables"], h["elem_shape"], h["corners"],
+                                       h["dof_indexes"], h["color_indexes"], h["ncolor"], h["neq"])
+        else:
+            self.ctx.transfer_data(h["row_sizes"], h["col_sizes"], h["matrix_position"], h["row_first"],
+                                   h["col_first"], h["storage"], h["dof_indexes"], h["color_indexes"], h["ncolor"],
+                                   h["neq"], h["weight"])
+        self.ctx.set_material_table(h["material_table"])
+        self.ctx.set_rhs_linear(h["rhs_linear"])
+        if self.mask is not None:
+            self.ctx.set_dirichlet_mask(self.mask)
+
+    def set_dirichlet_mask(self, mask):
+        """The batch's constrained DoFs (kept across relayout)."""
+        self.mask = np.ascontiguousarray(mask, dtype=np.int8)
+        self.ctx.set_dirichlet_mask(self.mask)
+
+    def dirichlet_mask(self):
+        """The instances' symmetry constraints, stacked."""
+        return np.concatenate([m.dirichlet_mask() for m in self.meshes])
+
+    def relayout(self, keys):
+        """Re-transfers the batch with the elements of every (color, shape,
+        material) run ordered by `keys` [elements] (ipf_set_element_order_keys:
+        e.g. predicted-plastic elements last, so that whole tiles take the
+        certified elastic path).  The committed state and the Dirichlet mask
+        carry over; f_lin returns to the instances' own loads (rhs_linear) and
+        the B formats to the library's default.  Results are bit-identical for
+        any keys (within a color every DoF has one element)."""
+        st = self.ctx.get_state()
+        self._transfer(keys)
+        self.ctx.set_state(st[0], st[1])
 
     def newton(self, u, rtol=1e-10, maxit=80, cg_rtol=1e-12, cg_maxit=40000):
         """Newton over the batch with per-instance convergence

@@ -19,6 +19,7 @@ SOURCES = [
     "ipf_kernels_fmt.hip",
     "ipf_assemble.hip",
     "ipf_learned.hip",
+    "ipf_patch.hip",
     "ipf_mlp.hip",
     "ipf_device.hip",
     "TPZCompMesh.cpp",

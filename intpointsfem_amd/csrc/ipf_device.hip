@@ -20,6 +20,7 @@
 #include "ipf_geometry.h"
 #include "ipf_kernels.h"
 #include "ipf_learned.h"
+#include "ipf_patch.h"
 
 namespace ipf {
 
@@ -170,11 +171,40 @@ struct ipf_ctx {
     int64_t nblocks = 0, npts = 0, rows = 0, cols = 0, neq = 0, ncolor = 0, nstorage = 0, npad = 0;
     bool have_data = false;
     bool update_mem = false;
+    double newton_atol = 0.0;  // absolute residual floor of ipf_newton[_segmented]
+    int residual_mode = IPF_RESIDUAL_EXACT;  // ipf_set_residual_mode
+    int assembly = IPF_ASSEMBLY_COLORED;     // ipf_set_residual_assembly
+    // every call that may change the committed state bumps this; the patch
+    // layout's copy of the plastic strain is refreshed when it is behind
+    uint64_t state_version = 1;
+    // patch layout: the builder's input kept from the transfer (geometric
+    // data with (x, y) DoF pairs only), the device arrays built on first use
+    std::vector<PatchElemIn> patch_in;
+    std::vector<int> patch_in_nodes;
+    bool patch_built = false;
+    PatchHost patch;
+    uint64_t patch_state_version = 0;
+    int* dPtmeta = nullptr;
+    double* dPgeo = nullptr;
+    unsigned short* dPlnode = nullptr;
+    int64_t* dPnodeOff = nullptr;
+    int* dPnode = nullptr;
+    int* dPslot = nullptr;
+    int* dPcbeg = nullptr;
+    unsigned short* dPcontrib = nullptr;
+    int64_t* dPptmap = nullptr;
+    double* dPep[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* dPpartial = nullptr;
+    int* dPfixDof = nullptr;
+    int* dPfixPtr = nullptr;
+    int* dPfixSlot = nullptr;
+    int* dPorphan = nullptr;
     // material table (index = element material id) and the per-element ids
     // that the next ipf_transfer_data groups by
     std::vector<MatConst> mats;
     MatConst* dMats = nullptr;
     std::vector<int> elem_mat_next;
+    std::vector<int> elem_key_next;  // ipf_set_element_order_keys, for the next transfer
     int max_mat = 0;              // largest material id of the transferred data
     int* dTmeta = nullptr;
     int64_t* dNewOf = nullptr;    // reference point -> new point index
@@ -301,8 +331,32 @@ void drop_cg_graph(ipf_ctx* ctx) {
     ctx->cg_tfmt = -1;
 }
 
+void release_patch(ipf_ctx* ctx) {
+    dev_free(ctx->dPtmeta);
+    dev_free(ctx->dPgeo);
+    dev_free(ctx->dPlnode);
+    dev_free(ctx->dPnodeOff);
+    dev_free(ctx->dPnode);
+    dev_free(ctx->dPslot);
+    dev_free(ctx->dPcbeg);
+    dev_free(ctx->dPcontrib);
+    dev_free(ctx->dPptmap);
+    for (auto& q : ctx->dPep) dev_free(q);
+    dev_free(ctx->dPpartial);
+    dev_free(ctx->dPfixDof);
+    dev_free(ctx->dPfixPtr);
+    dev_free(ctx->dPfixSlot);
+    dev_free(ctx->dPorphan);
+    ctx->patch = PatchHost();
+    ctx->patch_built = false;
+}
+
 void release_data(ipf_ctx* ctx) {
     drop_cg_graph(ctx);
+    release_patch(ctx);
+    ctx->patch_in.clear();
+    ctx->patch_in_nodes.clear();
+    ctx->state_version++;
     dev_free(ctx->dB);
     dev_free(ctx->dDof);
     dev_free(ctx->dCol);
@@ -417,6 +471,7 @@ GroupDev group_dev(const ipf_ctx* ctx, const Group& g, bool columns = false) {
 }
 
 PointState point_state(ipf_ctx* ctx) {
+    if (ctx->update_mem) ctx->state_version++;  // the launches it feeds may commit
     PointState s;
     s.w = ctx->dW;
     s.ep0 = ctx->dEp[0];
@@ -442,8 +497,121 @@ void launch_group_residual(ipf_ctx* ctx, const Group& g, const double* u, double
     GroupDev d = group_dev(ctx, g);
     if (flin) d.dof = ctx->dDofR + g.dof_off;
     if (ctx->dNodeR) d.node = ctx->dNodeR + g.dof_off / 2;  // flagged only when f_lin is folded in
-    launch_residual_fmt(ctx->rfmt, g.shape, ctx->update_mem, d, group_fmt_B(ctx, g, ctx->rfmt), u, r, flin, st,
-                        ctx->stream);
+    launch_residual_fmt(ctx->rfmt, g.shape, ctx->update_mem, ctx->residual_mode == IPF_RESIDUAL_CERTIFIED_ELASTIC, d,
+                        group_fmt_B(ctx, g, ctx->rfmt), u, r, flin, st, ctx->stream);
+}
+
+int launch_status(ipf_ctx* ctx, const char* what);
+
+// Builds and uploads the patch layout (ipf_patch.h) from the transfer's
+// input, once per transfer.
+int ensure_patch(ipf_ctx* ctx) {
+    if (ctx->patch_built) return IPF_OK;
+    if (ctx->patch_in.empty())
+        return fail(ctx, IPF_EUNSUPPORTED,
+                    "patch assembly needs the geometric setup (ipf_transfer_geometry) with every element's DoFs "
+                    "in (x, y) node pairs");
+    PatchHost& h = ctx->patch;
+    const std::string err = build_patches(ctx->patch_in, ctx->patch_in_nodes, ctx->neq, h);
+    if (!err.empty()) {
+        ctx->patch = PatchHost();
+        return fail(ctx, IPF_EUNSUPPORTED, "%s", err.c_str());
+    }
+    const int64_t b0 = ctx->bytes;
+    int rc;
+    auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (bytes == 0) return IPF_OK;
+        IPF_HIP(ctx, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return IPF_OK;
+    };
+    const int64_t npn = (int64_t)h.pnode.size();
+    if ((rc = dev_alloc(ctx, &ctx->dPtmeta, h.ntiles)) || (rc = dev_alloc(ctx, &ctx->dPgeo, 8 * h.nslots)) ||
+        (rc = dev_alloc(ctx, &ctx->dPlnode, (int64_t)h.lnode.size())) ||
+        (rc = dev_alloc(ctx, &ctx->dPnodeOff, (int64_t)h.node_off.size())) ||
+        (rc = dev_alloc(ctx, &ctx->dPnode, npn)) || (rc = dev_alloc(ctx, &ctx->dPslot, npn)) ||
+        (rc = dev_alloc(ctx, &ctx->dPcbeg, npn + 1)) ||
+        (rc = dev_alloc(ctx, &ctx->dPcontrib, (int64_t)h.contrib.size())) ||
+        (rc = dev_alloc(ctx, &ctx->dPptmap, h.npts)) || (rc = dev_alloc(ctx, &ctx->dPep[0], h.npts)) ||
+        (rc = dev_alloc(ctx, &ctx->dPep[1], h.npts)) || (rc = dev_alloc(ctx, &ctx->dPep[2], h.npts)) ||
+        (rc = dev_alloc(ctx, &ctx->dPep[3], h.npts)) ||
+        (rc = dev_alloc(ctx, &ctx->dPpartial, 2 * std::max<int64_t>(1, h.npartial))) ||
+        (rc = dev_alloc(ctx, &ctx->dPfixDof, (int64_t)h.fix_dof.size())) ||
+        (rc = dev_alloc(ctx, &ctx->dPfixPtr, (int64_t)h.fix_ptr.size())) ||
+        (rc = dev_alloc(ctx, &ctx->dPfixSlot, (int64_t)h.fix_slot.size())) ||
+        (rc = dev_alloc(ctx, &ctx->dPorphan, (int64_t)h.orphan.size())) ||
+        (rc = up(ctx->dPtmeta, h.tmeta.data(), sizeof(int) * h.tmeta.size())) ||
+        (rc = up(ctx->dPlnode, h.lnode.data(), sizeof(unsigned short) * h.lnode.size())) ||
+        (rc = up(ctx->dPnodeOff, h.node_off.data(), sizeof(int64_t) * h.node_off.size())) ||
+        (rc = up(ctx->dPnode, h.pnode.data(), sizeof(int) * npn)) ||
+        (rc = up(ctx->dPslot, h.pslot.data(), sizeof(int) * npn)) ||
+        (rc = up(ctx->dPcbeg, h.cbeg.data(), sizeof(int) * (npn + 1))) ||
+        (rc = up(ctx->dPcontrib, h.contrib.data(), sizeof(unsigned short) * h.contrib.size())) ||
+        (rc = up(ctx->dPptmap, h.pt_map.data(), sizeof(int64_t) * h.npts)) ||
+        (rc = up(ctx->dPfixDof, h.fix_dof.data(), sizeof(int) * h.fix_dof.size())) ||
+        (rc = up(ctx->dPfixPtr, h.fix_ptr.data(), sizeof(int) * h.fix_ptr.size())) ||
+        (rc = up(ctx->dPfixSlot, h.fix_slot.data(), sizeof(int) * h.fix_slot.size())) ||
+        (rc = up(ctx->dPorphan, h.orphan.data(), sizeof(int) * h.orphan.size()))) {
+        release_patch(ctx);
+        return rc;
+    }
+    // corners gathered on the device from the colored layout's slots
+    int64_t* dSlotGeo = nullptr;
+    if ((rc = dev_alloc(ctx, &dSlotGeo, h.nslots)) ||
+        (rc = up(dSlotGeo, h.slot_geo.data(), sizeof(int64_t) * h.nslots))) {
+        dev_free(dSlotGeo);
+        release_patch(ctx);
+        return rc;
+    }
+    launch_patch_geo(h.nslots, dSlotGeo, ctx->dGeo, ctx->dPgeo, ctx->stream);
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    dev_free(dSlotGeo);
+    if (e != hipSuccess || (rc = launch_status(ctx, "patch geometry"))) {
+        release_patch(ctx);
+        return rc ? rc : fail(ctx, IPF_EHIP, "patch geometry: %s", hipGetErrorString(e));
+    }
+    (void)b0;
+    ctx->patch_built = true;
+    ctx->patch_state_version = 0;
+    return IPF_OK;
+}
+
+// The non-committing residual over the patch layout: refresh the patch copy
+// of the plastic strain when the committed state changed, one launch per
+// element shape, then the shared nodes' fix-up.
+int patch_residual(ipf_ctx* ctx, const double* u, double* r) {
+    int rc = ensure_patch(ctx);
+    if (rc) return rc;
+    const PatchHost& h = ctx->patch;
+    if (ctx->patch_state_version != ctx->state_version) {
+        launch_patch_state(h.npts, ctx->dPptmap, ctx->dEp, ctx->dPep, ctx->stream);
+        ctx->patch_state_version = ctx->state_version;
+    }
+    const bool fast = ctx->residual_mode == IPF_RESIDUAL_CERTIFIED_ELASTIC;
+    for (const auto& R : h.ranges) {
+        PatchDev p;
+        const int ET = shape_et(R.shape), TP = shape_tp(R.shape);
+        p.npatch = R.npatch;
+        p.tmeta = ctx->dPtmeta + R.tile0;
+        p.mats = ctx->dMats;
+        p.geo = ctx->dPgeo + 8 * R.slot0;
+        p.lnode = ctx->dPlnode + R.lnode0;
+        p.node_off = ctx->dPnodeOff + R.patch0;
+        p.pnode = ctx->dPnode;
+        p.pslot = ctx->dPslot;
+        p.cbeg = ctx->dPcbeg;
+        p.contrib = ctx->dPcontrib;
+        p.ep0 = ctx->dPep[0] + R.pt0;
+        p.ep1 = ctx->dPep[1] + R.pt0;
+        p.ep2 = ctx->dPep[2] + R.pt0;
+        p.ep3 = ctx->dPep[3] + R.pt0;
+        p.tab = ctx->dTab + ctx->tab_off[R.shape];
+        (void)ET;
+        (void)TP;
+        launch_residual_patch(R.shape, fast, p, u, r, ctx->dFlin, ctx->dPpartial, ctx->stream);
+    }
+    launch_patch_fixup((int64_t)h.fix_dof.size(), ctx->dPfixDof, ctx->dPfixPtr, ctx->dPfixSlot, ctx->dPpartial,
+                       (int64_t)h.orphan.size(), ctx->dPorphan, ctx->dFlin, r, ctx->stream);
+    return launch_status(ctx, "patch residual");
 }
 
 // r = f_lin (TPZElastoPlasticIntPointsStructMatrix.cpp:147) before the
@@ -720,6 +888,14 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
     }
     int max_mat = 0;
     for (int64_t e = 0; e < nblocks; e++) max_mat = std::max(max_mat, emat[e]);
+    // --- order keys inside a material run (ipf_set_element_order_keys; all 0 otherwise)
+    std::vector<int> ekey(nblocks, 0);
+    if (!ctx->elem_key_next.empty()) {
+        if ((int64_t)ctx->elem_key_next.size() != nblocks)
+            return fail(ctx, IPF_EINVAL, "element order keys were set for %lld blocks, the data has %lld",
+                        (long long)ctx->elem_key_next.size(), (long long)nblocks);
+        ekey = ctx->elem_key_next;
+    }
 
     // --- groups: color-major, then shape; inside a group elements by
     // (material, index), every material run padded to whole tiles
@@ -731,7 +907,9 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
             for (int64_t e = 0; e < nblocks; e++)
                 if (elem_color[e] == c && elem_shape[e] == s) es.push_back(e);
             if (es.empty()) continue;
-            std::stable_sort(es.begin(), es.end(), [&](int64_t a, int64_t b) { return emat[a] < emat[b]; });
+            std::stable_sort(es.begin(), es.end(), [&](int64_t a, int64_t b) {
+                return emat[a] != emat[b] ? emat[a] < emat[b] : ekey[a] < ekey[b];
+            });
             Group g;
             g.color = c;
             g.shape = s;
@@ -885,6 +1063,35 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
         hNodeR.resize(nd / 2);
         for (int64_t i = 0; i < nd / 2; i++) hNodeR[i] = fold ? hDofR[2 * i] : hDof[2 * i];
     }
+    // input of the patch layout (built on demand): every real element slot
+    // with its node x-DoFs, corner centroid, colored geometry slot and points
+    std::vector<PatchElemIn> pin;
+    std::vector<int> pin_nodes;
+    if (geo && pairs) {
+        try {
+            for (const auto& g : groups) {
+                const int NIP = shape_nip(g.shape), CE = shape_ce(g.shape), ET = shape_et(g.shape),
+                          TP = shape_tp(g.shape);
+                for (int64_t k = 0; k < (int64_t)g.elems.size(); k++) {
+                    if (g.elems[k] < 0) continue;
+                    PatchElemIn E;
+                    E.shape = g.shape;
+                    E.mat = g.tmeta[k / ET] >> 8;
+                    E.geo_slot = g.slot0 + k;
+                    E.P0 = g.pt0 + (k / ET) * TP + (k % ET) * NIP;
+                    E.node0 = (int64_t)pin_nodes.size();
+                    const double* X = &hGeo[8 * (g.slot0 + k)];
+                    E.cx = 0.25 * (X[0] + X[2] + X[4] + X[6]);
+                    E.cy = 0.25 * (X[1] + X[3] + X[5] + X[7]);
+                    for (int a = 0; a < CE / 2; a++) pin_nodes.push_back(hDof[g.dof_off + k * CE + 2 * a]);
+                    pin.push_back(E);
+                }
+            }
+        } catch (const std::bad_alloc&) {
+            pin.clear();
+            pin_nodes.clear();
+        }
+    }
 
     // --- device buffers
     int rc;
@@ -996,6 +1203,8 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
         ctx->have_geometry = true;
     }
     ctx->have_data = true;
+    ctx->patch_in = std::move(pin);
+    ctx->patch_in_nodes = std::move(pin_nodes);
     // Blocks in the 2D elasticity pattern default to compact FP64: the same
     // results from a third of the B bytes; with the geometry at hand, to the
     // geometric form (the same results from no per-point B bytes at all).
@@ -1115,6 +1324,13 @@ int ipf_set_element_materials(ipf_ctx* ctx, int64_t nblocks, const int* element_
     return IPF_OK;
 }
 
+int ipf_set_element_order_keys(ipf_ctx* ctx, int64_t nblocks, const int* keys) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    if (nblocks < 0 || (nblocks > 0 && !keys)) return fail(ctx, IPF_EINVAL, "ipf_set_element_order_keys: bad arguments");
+    ctx->elem_key_next.assign(keys, keys + nblocks);
+    return IPF_OK;
+}
+
 int ipf_set_rhs_linear(ipf_ctx* ctx, const double* f_lin_host) {
     int rc = check_ready(ctx, false);
     if (rc) return rc;
@@ -1122,6 +1338,132 @@ int ipf_set_rhs_linear(ipf_ctx* ctx, const double* f_lin_host) {
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     IPF_HIP(ctx, hipMemcpyAsync(ctx->dFlin, f_lin_host, sizeof(double) * ctx->neq, hipMemcpyHostToDevice, ctx->stream));
     IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return IPF_OK;
+}
+
+int ipf_set_residual_assembly(ipf_ctx* ctx, int assembly) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_residual_assembly: NULL context");
+    if (assembly != IPF_ASSEMBLY_COLORED && assembly != IPF_ASSEMBLY_PATCH)
+        return fail(ctx, IPF_EINVAL, "ipf_set_residual_assembly: unknown assembly %d", assembly);
+    if (assembly == IPF_ASSEMBLY_PATCH) {
+        int rc = check_ready(ctx, false);
+        if (rc) return rc;
+        IPF_HIP(ctx, hipSetDevice(ctx->device));
+        if ((rc = ensure_patch(ctx))) return rc;
+    }
+    ctx->assembly = assembly;
+    return IPF_OK;
+}
+
+int ipf_get_residual_assembly(const ipf_ctx* ctx, int* assembly) {
+    if (!ctx || !assembly) return fail(nullptr, IPF_EINVAL, "ipf_get_residual_assembly: NULL argument");
+    *assembly = ctx->assembly;
+    return IPF_OK;
+}
+
+int ipf_patch_info(const ipf_ctx* ctx, int64_t out[8]) {
+    if (!ctx || !out) return fail(nullptr, IPF_EINVAL, "ipf_patch_info: NULL argument");
+    const PatchHost& h = ctx->patch;
+    int64_t np = 0;
+    for (const auto& R : h.ranges) np += R.npatch;
+    out[0] = ctx->patch_built ? 1 : 0;
+    out[1] = np;
+    out[2] = (int64_t)h.pnode.size();
+    out[3] = h.npartial;
+    out[4] = (int64_t)h.fix_dof.size();
+    out[5] = (int64_t)h.orphan.size();
+    out[6] = h.npts;
+    out[7] = (int64_t)h.ranges.size();
+    return IPF_OK;
+}
+
+int ipf_patch_plan_host(int64_t n, const int* shape, const int* mat, const int64_t* first_point, const double* cx,
+                        const double* cy, const int* node_xdof, int64_t neq, int64_t* sizes, int64_t* ranges,
+                        int* tmeta, int64_t* slot_elem, unsigned short* lnode, int64_t* node_off, int* pnode,
+                        int* pslot, int* cbeg, unsigned short* contrib, int64_t* pt_map, int* fix_dof, int* fix_ptr,
+                        int* fix_slot, int* orphan) {
+    if (n < 0 || !sizes || (n > 0 && (!shape || !mat || !first_point || !cx || !cy || !node_xdof)))
+        return fail(nullptr, IPF_EINVAL, "ipf_patch_plan_host: bad arguments");
+    std::vector<PatchElemIn> el(n);
+    std::vector<int> nodes;
+    for (int64_t e = 0; e < n; e++) {
+        if (shape[e] < 0 || shape[e] >= SHAPE_COUNT) return fail(nullptr, IPF_EINVAL, "ipf_patch_plan_host: shape");
+        el[e].shape = shape[e];
+        el[e].mat = mat[e];
+        el[e].geo_slot = e;
+        el[e].P0 = first_point[e];
+        el[e].cx = cx[e];
+        el[e].cy = cy[e];
+        el[e].node0 = (int64_t)nodes.size();
+        for (int a = 0; a < shape_ce(shape[e]) / 2; a++) nodes.push_back(node_xdof[el[e].node0 + a]);
+    }
+    PatchHost h;
+    const std::string err = build_patches(el, nodes, neq, h);
+    if (!err.empty()) return fail(nullptr, IPF_EUNSUPPORTED, "%s", err.c_str());
+    int64_t np = 0;
+    for (const auto& R : h.ranges) np += R.npatch;
+    const int64_t sz[12] = {(int64_t)h.ranges.size(), np, h.ntiles, h.nslots, (int64_t)h.lnode.size(),
+                            (int64_t)h.pnode.size(), (int64_t)h.contrib.size(), h.npts, (int64_t)h.fix_dof.size(),
+                            (int64_t)h.fix_slot.size(), (int64_t)h.orphan.size(), h.npartial};
+    std::memcpy(sizes, sz, sizeof(sz));
+    auto cp = [](auto* dst, const auto& v) {
+        if (dst) std::memcpy(dst, v.data(), sizeof(v[0]) * v.size());
+    };
+    if (ranges)
+        for (size_t i = 0; i < h.ranges.size(); i++) {
+            const auto& R = h.ranges[i];
+            const int64_t v[7] = {R.shape, R.npatch, R.patch0, R.tile0, R.slot0, R.pt0, R.lnode0};
+            std::memcpy(ranges + 7 * i, v, sizeof(v));
+        }
+    cp(tmeta, h.tmeta);
+    cp(slot_elem, h.slot_geo);
+    cp(lnode, h.lnode);
+    cp(node_off, h.node_off);
+    cp(pnode, h.pnode);
+    cp(pslot, h.pslot);
+    cp(cbeg, h.cbeg);
+    cp(contrib, h.contrib);
+    cp(pt_map, h.pt_map);
+    cp(fix_dof, h.fix_dof);
+    cp(fix_ptr, h.fix_ptr);
+    cp(fix_slot, h.fix_slot);
+    cp(orphan, h.orphan);
+    return IPF_OK;
+}
+
+int ipf_certify_elastic_host(int64_t n, const double* strain3, const double* plastic_strain4, double lambda,
+                             double mu, double mc_phi, double mc_psi, double mc_cohesion, int* certified,
+                             double* sigma3) {
+    if (n < 0 || (n > 0 && (!strain3 || !plastic_strain4 || !certified)))
+        return fail(nullptr, IPF_EINVAL, "ipf_certify_elastic_host: bad arguments");
+    const MatConst m = MatConst::make(lambda, mu, mc_phi, mc_psi, mc_cohesion);
+    for (int64_t i = 0; i < n; i++) {
+        double ws[3] = {0.0, 0.0, 0.0};
+        certified[i] = certified_elastic(strain3 + 3 * i, plastic_strain4 + 4 * i, m, 1.0, ws) ? 1 : 0;
+        if (sigma3)
+            for (int k = 0; k < 3; k++) sigma3[3 * i + k] = ws[k];
+    }
+    return IPF_OK;
+}
+
+int ipf_set_residual_mode(ipf_ctx* ctx, int mode) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_residual_mode: NULL context");
+    if (mode != IPF_RESIDUAL_EXACT && mode != IPF_RESIDUAL_CERTIFIED_ELASTIC)
+        return fail(ctx, IPF_EINVAL, "ipf_set_residual_mode: unknown mode %d", mode);
+    ctx->residual_mode = mode;
+    return IPF_OK;
+}
+
+int ipf_get_residual_mode(const ipf_ctx* ctx, int* mode) {
+    if (!ctx || !mode) return fail(nullptr, IPF_EINVAL, "ipf_get_residual_mode: NULL argument");
+    *mode = ctx->residual_mode;
+    return IPF_OK;
+}
+
+int ipf_set_newton_atol(ipf_ctx* ctx, double atol) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_newton_atol: NULL context");
+    if (!(atol >= 0.0)) return fail(ctx, IPF_EINVAL, "ipf_set_newton_atol: atol must be >= 0");
+    ctx->newton_atol = atol;
     return IPF_OK;
 }
 
@@ -1136,6 +1478,7 @@ int ipf_residual(ipf_ctx* ctx, const double* u_dev, double* r_dev) {
     if (rc) return rc;
     if (!u_dev || !r_dev) return fail(ctx, IPF_EINVAL, "ipf_residual: NULL pointer");
     IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if (ctx->assembly == IPF_ASSEMBLY_PATCH && !ctx->update_mem) return patch_residual(ctx, u_dev, r_dev);
     // r = f_lin, then each color adds -B^T(w sigma)
     const double* flin = residual_start(ctx, r_dev);
     const PointState st = point_state(ctx);
@@ -1572,30 +1915,34 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
     if (!u_dev) return fail(ctx, IPF_EINVAL, "ipf_newton: NULL pointer");
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_solver_buffers(ctx))) return rc;
+    // The caller's UpdateMem switch is restored on every exit: the Newton
+    // residuals must not commit, only the final AcceptPseudoTimeStepSolution.
     const bool commit_flag = ctx->update_mem;
     ctx->update_mem = false;
     double r0 = 0.0, rn = 0.0, worst_cg = 0.0;
     int it = 0, cg_total = 0;
     bool converged = false;
     for (it = 0; it <= maxit; it++) {
-        if ((rc = ipf_residual_tangent(ctx, u_dev, ctx->dNr))) return rc;
-        if ((rc = masked_norm(ctx, ctx->dNr, &rn))) return rc;
+        if ((rc = ipf_residual_tangent(ctx, u_dev, ctx->dNr))) break;
+        if ((rc = masked_norm(ctx, ctx->dNr, &rn))) break;
         if (it == 0) r0 = rn;
-        if (rn <= rtol * r0 || rn == 0.0) {
+        if (rn <= rtol * r0 || rn <= ctx->newton_atol || rn == 0.0) {
             converged = true;
             break;
         }
         if (it == maxit) break;
         int cgi = 0;
         double cgr = 0.0;
-        if ((rc = ipf_cg_solve(ctx, ctx->dNr, ctx->dNd, cg_rtol, cg_maxit, &cgi, &cgr))) return rc;
+        if ((rc = ipf_cg_solve(ctx, ctx->dNr, ctx->dNd, cg_rtol, cg_maxit, &cgi, &cgr))) break;
         cg_total += cgi;
         worst_cg = std::max(worst_cg, cgr);
         launch_axpy(ctx->neq, 1.0, ctx->dNd, u_dev, ctx->stream);
     }
     // AcceptPseudoTimeStepSolution: one residual with UpdateMem on
-    ctx->update_mem = true;
-    rc = ipf_residual(ctx, u_dev, ctx->dNr);
+    if (!rc) {
+        ctx->update_mem = true;
+        rc = ipf_residual(ctx, u_dev, ctx->dNr);
+    }
     ctx->update_mem = commit_flag;
     if (rc) return rc;
     if ((rc = ipf_synchronize(ctx))) return rc;
@@ -1688,7 +2035,7 @@ int ipf_newton_segmented(ipf_ctx* ctx, double* u_dev, int nseg, const int64_t* s
         for (int k = 0; k < nseg; k++) {
             rn[k] = std::sqrt(rn[k]);
             if (it == 0) r0[k] = rn[k];
-            if (!done[k] && (rn[k] <= rtol * r0[k] || rn[k] == 0.0)) {
+            if (!done[k] && (rn[k] <= rtol * r0[k] || rn[k] <= ctx->newton_atol || rn[k] == 0.0)) {
                 done[k] = 1;
                 its[k] = it;
                 // hold the segment: its DoFs leave the CG and the update
@@ -1804,6 +2151,7 @@ int ipf_set_state(ipf_ctx* ctx, const double* plastic_strain_host, const double*
     }
     for (int64_t P = 0; P < n; P++) v[P] = (alpha_host && orig[P] >= 0) ? alpha_host[orig[P]] : 0.0;
     IPF_HIP(ctx, hipMemcpy(ctx->dAlpha, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    ctx->state_version++;
     return IPF_OK;
 }
 

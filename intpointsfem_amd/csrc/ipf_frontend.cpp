@@ -303,7 +303,7 @@ int ipf_sm_assemble_rhs(ipf_sm* sm, const double* u_host, double* rhs_host) {
 int ipf_sm_create_tangent(ipf_sm* sm, int64_t* nnz) {
     if (!sm || !nnz) return null_arg("ipf_sm_create_tangent");
     return guarded([&] {
-        if (!sm->K) sm->K.reset(sm->sm->Create());
+        if (!sm->K) sm->K.reset(static_cast<TPZSYsmpMatrix<STATE>*>(sm->sm->Create()));
         *nnz = sm->K->JA().size();
         return IPF_OK;
     });
@@ -325,6 +325,51 @@ int ipf_sm_export_tangent_pattern(const ipf_sm* sm, int64_t* ia, int64_t* ja, in
         std::memcpy(first_color_l_index, in.FirstColorLIndex().data(),
                     sizeof(int64_t) * in.FirstColorLIndex().size());
     return IPF_OK;
+}
+
+int ipf_sm_clone(const ipf_sm* sm, ipf_sm** out) {
+    if (!sm || !out) return null_arg("ipf_sm_clone");
+    *out = nullptr;
+    return guarded([&] {
+        auto s = std::make_unique<ipf_sm>();
+        s->mesh = sm->mesh;
+        s->sm.reset(static_cast<TPZElastoPlasticIntPointsStructMatrix*>(sm->sm->Clone()));
+        *out = s.release();
+        return IPF_OK;
+    });
+}
+
+int ipf_sm_create_assemble(ipf_sm* sm, const double* u_host, double* rhs_host, int64_t* nnz) {
+    if (!sm || !u_host || !rhs_host || !nnz) return null_arg("ipf_sm_create_assemble");
+    return guarded([&] {
+        TPZCompMesh* m = sm->mesh->mesh.get();
+        std::memcpy(m->Solution().data(), u_host, sizeof(double) * m->NEquations());
+        TPZFMatrix<STATE> rhs;
+        std::unique_ptr<TPZMatrix<STATE>> K(sm->sm->CreateAssemble(rhs, TPZAutoPointer<TPZGuiInterface>()));
+        std::memcpy(rhs_host, rhs.data(), sizeof(double) * m->NEquations());
+        sm->K.reset(static_cast<TPZSYsmpMatrix<STATE>*>(K.release()));
+        *nnz = sm->K->JA().size();
+        return IPF_OK;
+    });
+}
+
+int ipf_sm_tangent_values(const ipf_sm* sm, double* kg_host) {
+    if (!sm || !kg_host) return null_arg("ipf_sm_tangent_values");
+    if (!sm->K) return fail_status(IPF_ESTATE, "ipf_sm_tangent_values: no matrix (ipf_sm_create_tangent or ipf_sm_create_assemble)");
+    std::memcpy(kg_host, sm->K->A().data(), sizeof(double) * sm->K->A().size());
+    return IPF_OK;
+}
+
+int ipf_sm_export_csr(ipf_sm* sm, int* row_ptr, int* col_ind) {
+    if (!sm) return null_arg("ipf_sm_export_csr");
+    return guarded([&] {
+        auto& bm = sm->sm->Integrator().BlockMatrix();
+        bm.CSRVectors();
+        auto& B = bm.Blocks();
+        if (row_ptr) std::memcpy(row_ptr, B.fRowPtr.data(), sizeof(int) * B.fRowPtr.size());
+        if (col_ind) std::memcpy(col_ind, B.fColInd.data(), sizeof(int) * B.fColInd.size());
+        return IPF_OK;
+    });
 }
 
 int ipf_sm_assemble_tangent(ipf_sm* sm, const double* u_host, double* rhs_host, double* kg_host) {

@@ -147,8 +147,10 @@ void launch_assemble_K(int shape, const GroupDev& g, const double* dep_soa, int6
 // With flin set, g.dof's entries flagged kFirstWrite start from flin[d]
 // instead of r[d] (each DoF's first write in launch order): r = flin then
 // needs no copy before the launches.
-void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
-                         double* r, const double* flin, const PointState& st, hipStream_t stream);
+// fast: non-committing launches take the certified elastic path
+// (certified_elastic, ipf_pointmath.h); ignored when commit is set.
+void launch_residual_fmt(int fmt, int shape, bool commit, bool fast, const GroupDev& g, const void* Bv,
+                         const double* u, double* r, const double* flin, const PointState& st, hipStream_t stream);
 // The tangent launches accumulate y += B^T (w Dep) B v; with zero_first,
 // g.dof's (g.node's) entries flagged kFirstWrite start from 0 instead of y[d],
 // so the launches of all groups overwrite y with K v and need no clear.

@@ -20,6 +20,7 @@
 // the return map, the assembly into r and everything else stay FP64.
 #include "ipf_geometry.h"
 #include "ipf_kernels.h"
+#include "ipf_patch.h"
 
 namespace ipf {
 
@@ -203,10 +204,16 @@ __device__ __forceinline__ void reduce_lane(const typename FC<SHAPE, FMT>::T* L,
 // no other wave of the launch writes them.  R_EARLY costs 7 VGPRs; it is on
 // where that keeps the waves per SIMD (Q3: LDS already limits them to 4), off
 // where it would cost one wave (measured slower at Q2 on MI355X).
-template <int SHAPE, int FMT, bool COMMIT, bool PAIR>
+//
+// FAST (never with COMMIT): a point whose trial stress passes
+// certified_elastic takes its trial stress and skips the spectral
+// decomposition; a wavefront whose points all pass skips the return map's
+// instructions altogether.  The other points take the exact return map.
+template <int SHAPE, int FMT, bool COMMIT, bool PAIR, bool FAST>
 __global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
 k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ u, double* __restrict__ r,
                const double* __restrict__ flin, PointState st) {
+    static_assert(!(FAST && COMMIT), "the certified elastic path commits nothing");
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
     constexpr int NSH = C::CE / 2;
@@ -310,7 +317,8 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     if (valid) {
         int mt, br;
         const double w = FmtTraits<FMT>::GEOM ? wgeo : wst;
-        point_sigma<false>(e3, ep, alpha, mt, br, tile_material(g, tile), w, ws, nullptr);
+        const MatConst& mc = tile_material(g, tile);
+        if (!(FAST && certified_elastic(e3, ep, mc, w, ws))) point_sigma<false>(e3, ep, alpha, mt, br, mc, w, ws, nullptr);
         if (COMMIT) {
             st.ep0[P] = ep[0];
             st.ep1[P] = ep[1];
@@ -547,23 +555,144 @@ __global__ void k_build_B(GroupDev g, const double* __restrict__ corners, double
     w[g.pt0 + tile * C::TP + lane] = wt;
 }
 
+// The non-committing residual over the patch layout (ipf_patch.h): one
+// workgroup of patch_waves(SHAPE) wavefronts per patch.  Per point the
+// colored kernel's arithmetic (geometric FP64 B, compact strain sums, return
+// map or FAST's certified elastic path, B^T products); then per element the
+// column sums over its points (point order) into LDS, and per patch node
+// the sum of its elements' contributions (ascending element slot, then
+// local node), written to r = f_lin + (-sum) where the patch owns the node,
+// else to the node's partial slot for the fix-up pass.
+template <int SHAPE, bool FAST>
+__global__ void __launch_bounds__(kWave * patch_waves(SHAPE))
+k_residual_patch(PatchDev p, const double* __restrict__ u, double* __restrict__ r, const double* __restrict__ flin,
+                 double* __restrict__ partial) {
+    using C = FC<SHAPE, FMT_G64>;
+    constexpr int NW = patch_waves(SHAPE), ET = C::ET, NIP = C::NIP, CE = C::CE, TP = C::TP;
+    constexpr int NSH = CE / 2, PE = NW * ET, MAXN = patch_max_nodes(SHAPE);
+    constexpr int H = CE / 2, H2 = CE - H;
+    static_assert(ET * H <= kWave && ET * H2 <= kWave, "one column sum per lane and half");
+    __shared__ double us[2 * MAXN];
+    __shared__ double es[PE * CE];
+    __shared__ double stg[NW * TP * H];
+    const int tid = threadIdx.x, w = tid / kWave, lane = tid - w * kWave;
+    const int64_t pa = blockIdx.x;
+    const int64_t n0 = p.node_off[pa];
+    const int nn = (int)(p.node_off[pa + 1] - n0);
+    const int64_t tile = pa * NW + w;
+    const int tm = p.tmeta[tile];
+    const int nv = tm & 255;
+    const int el = lane / NIP;
+    const bool slot = lane < TP;
+    const bool valid = slot && el < nv;
+    // requests first: the point's plastic strain (padding points hold zeros)
+    // and the element's local node indexes, then the patch's nodes into LDS
+    const int64_t P = tile * TP + (slot ? lane : 0);
+    double ep[4] = {ld_stream(&p.ep0[P]), ld_stream(&p.ep1[P]), ld_stream(&p.ep2[P]), ld_stream(&p.ep3[P])};
+    const int64_t sl = tile * ET + (valid ? el : 0);
+    int ln[NSH];
+#pragma unroll
+    for (int a = 0; a < NSH; a++) ln[a] = p.lnode[sl * NSH + a];
+    for (int i = tid; i < nn; i += NW * kWave) {
+        const int d = p.pnode[n0 + i];
+        us[2 * i] = u[d];
+        us[2 * i + 1] = u[d + 1];
+    }
+    double b[C::NV];
+    double wgeo = 0.0;
+    if (valid) {
+        wgeo = geom_values<SHAPE>(p.geo, p.tab, tile, lane, b);
+    } else {
+#pragma unroll
+        for (int k = 0; k < C::NV; k++) b[k] = 0.0;
+    }
+    __syncthreads();
+    double ws[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < NSH; a++) {
+            const double dx = b[2 * a], dy = b[2 * a + 1];
+            const double ux = us[2 * ln[a]], uy = us[2 * ln[a] + 1];
+            a0 = a0 + dx * ux;
+            a1 = a1 + dy * ux;
+            a1 = a1 + dx * uy;
+            a2 = a2 + dy * uy;
+        }
+        const double e3[3] = {a0, a1, a2};
+        const MatConst& mc = p.mats[tm >> 8];
+        if (!(FAST && certified_elastic(e3, ep, mc, wgeo, ws))) {
+            double alpha = 0.0;
+            int mt, br;
+            point_sigma<false>(e3, ep, alpha, mt, br, mc, wgeo, ws, nullptr);
+        }
+    }
+    // element column sums, half the columns at a time through the wave's staging
+    double* L = stg + w * TP * H;
+    const int re = lane / H, rj = lane - re * H;
+    if (slot) bt_cols<SHAPE, FMT_G64, 0, H>(b, ws, L + lane * H);
+    __syncthreads();
+    if (lane < ET * H && re < nv) {
+        double f = L[(re * NIP) * H + rj];
+#pragma unroll
+        for (int ip = 1; ip < NIP; ip++) f = f + L[(re * NIP + ip) * H + rj];
+        es[(w * ET + re) * CE + rj] = f;
+    }
+    __syncthreads();
+    const int re2 = lane / H2, rj2 = lane - re2 * H2;
+    if (slot) bt_cols<SHAPE, FMT_G64, H, CE>(b, ws, L + lane * H2);
+    __syncthreads();
+    if (lane < ET * H2 && re2 < nv) {
+        double f = L[(re2 * NIP) * H2 + rj2];
+#pragma unroll
+        for (int ip = 1; ip < NIP; ip++) f = f + L[(re2 * NIP + ip) * H2 + rj2];
+        es[(w * ET + re2) * CE + H + rj2] = f;
+    }
+    __syncthreads();
+    // node sums: owned nodes straight into r (never read), shared ones to their partial slot
+    for (int i = tid; i < nn; i += NW * kWave) {
+        const int64_t k = n0 + i;
+        double fx = 0.0, fy = 0.0;
+        const int c1 = p.cbeg[k + 1];
+        for (int c = p.cbeg[k]; c < c1; c++) {
+            const int v = p.contrib[c];
+            const int s = v / NSH, a = v - s * NSH;
+            fx = fx + es[s * CE + 2 * a];
+            fy = fy + es[s * CE + 2 * a + 1];
+        }
+        const int d = p.pnode[k], ps = p.pslot[k];
+        if (ps < 0) {
+            r[d] = flin[d] + (-fx);
+            r[d + 1] = flin[d + 1] + (-fy);
+        } else {
+            partial[2 * (int64_t)ps] = fx;
+            partial[2 * (int64_t)ps + 1] = fy;
+        }
+    }
+}
+
 template <int SHAPE, int FMT>
-void residual_fmt_shape(bool commit, const GroupDev& g, const void* Bv, const double* u, double* r,
+void residual_fmt_shape(bool commit, bool fast, const GroupDev& g, const void* Bv, const double* u, double* r,
                         const double* flin, const PointState& st, hipStream_t s) {
     const dim3 grid((unsigned)g.ntiles), block(kWave);
     if (commit)
-        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, true, false>), grid, block, 0, s, g, Bv, u, r, flin, st);
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, true, false, false>), grid, block, 0, s, g, Bv, u, r, flin, st);
+    else if (g.node && fast)
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false, true, true>), grid, block, 0, s, g, Bv, u, r, flin, st);
     else if (g.node)
-        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false, true>), grid, block, 0, s, g, Bv, u, r, flin, st);
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false, true, false>), grid, block, 0, s, g, Bv, u, r, flin, st);
+    else if (fast)
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false, false, true>), grid, block, 0, s, g, Bv, u, r, flin, st);
     else
-        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false, false>), grid, block, 0, s, g, Bv, u, r, flin, st);
+        hipLaunchKernelGGL((k_residual_fmt<SHAPE, FMT, false, false, false>), grid, block, 0, s, g, Bv, u, r, flin, st);
 }
 
 template <int FMT>
-void residual_fmt(int shape, bool commit, const GroupDev& g, const void* Bv, const double* u, double* r,
+void residual_fmt(int shape, bool commit, bool fast, const GroupDev& g, const void* Bv, const double* u, double* r,
                   const double* flin, const PointState& st, hipStream_t s) {
-    dispatch_shape(shape,
-                   [&](auto sc) { residual_fmt_shape<decltype(sc)::value, FMT>(commit, g, Bv, u, r, flin, st, s); });
+    dispatch_shape(shape, [&](auto sc) {
+        residual_fmt_shape<decltype(sc)::value, FMT>(commit, fast, g, Bv, u, r, flin, st, s);
+    });
 }
 
 template <int FMT>
@@ -590,15 +719,16 @@ void convert_fmt(int shape, const double* dense, void* out, int64_t ntiles, hipS
 
 }  // namespace
 
-void launch_residual_fmt(int fmt, int shape, bool commit, const GroupDev& g, const void* Bv, const double* u,
-                         double* r, const double* flin, const PointState& st, hipStream_t s) {
+void launch_residual_fmt(int fmt, int shape, bool commit, bool fast, const GroupDev& g, const void* Bv,
+                         const double* u, double* r, const double* flin, const PointState& st, hipStream_t s) {
     if (g.ntiles == 0) return;
+    const bool f = fast && !commit;
     switch (fmt) {
-        case FMT_D64: residual_fmt<FMT_D64>(shape, commit, g, Bv, u, r, flin, st, s); break;
-        case FMT_C64: residual_fmt<FMT_C64>(shape, commit, g, Bv, u, r, flin, st, s); break;
-        case FMT_G64: residual_fmt<FMT_G64>(shape, commit, g, Bv, u, r, flin, st, s); break;
-        case FMT_D32: residual_fmt<FMT_D32>(shape, commit, g, Bv, u, r, flin, st, s); break;
-        case FMT_C32: residual_fmt<FMT_C32>(shape, commit, g, Bv, u, r, flin, st, s); break;
+        case FMT_D64: residual_fmt<FMT_D64>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
+        case FMT_C64: residual_fmt<FMT_C64>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
+        case FMT_G64: residual_fmt<FMT_G64>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
+        case FMT_D32: residual_fmt<FMT_D32>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
+        case FMT_C32: residual_fmt<FMT_C32>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
         default: break;
     }
 }
@@ -615,6 +745,19 @@ void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, c
         case FMT_C32: tangent_fmt<FMT_C32>(shape, g, Bv, v, y, dep, npad, zf, s); break;
         default: break;
     }
+}
+
+void launch_residual_patch(int shape, bool fast, const PatchDev& p, const double* u, double* r, const double* flin,
+                           double* partial, hipStream_t s) {
+    if (p.npatch == 0) return;
+    dispatch_shape(shape, [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        const dim3 grid((unsigned)p.npatch), block(kWave * patch_waves(S));
+        if (fast)
+            hipLaunchKernelGGL((k_residual_patch<S, true>), grid, block, 0, s, p, u, r, flin, partial);
+        else
+            hipLaunchKernelGGL((k_residual_patch<S, false>), grid, block, 0, s, p, u, r, flin, partial);
+    });
 }
 
 void launch_convert_B(int fmt, int shape, const double* dense, void* out, int64_t ntiles, hipStream_t s) {

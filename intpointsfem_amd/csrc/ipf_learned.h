@@ -54,6 +54,7 @@ void launch_point_features(int shape, const GroupDev& g, const double* strain, c
 // sort last): order[] = indexes grouped by key, index order inside a key.
 // hist holds key_sort_blocks(n) * IPF_NBRANCH ints.
 constexpr int IPF_NBRANCH = 5;
+constexpr int IPF_SORT_BLOCK = 256;  // keys per histogram block
 int key_sort_blocks(int64_t n);
 void launch_branch_order(const int* key, int64_t n, int* hist, int* order, hipStream_t stream);
 // The exact return map of the points in `order` (reference indexes; new_of

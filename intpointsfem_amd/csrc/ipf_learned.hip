@@ -312,34 +312,109 @@ __global__ void k_mlp_refresh(const float* __restrict__ par, short* __restrict__
     if (p < NPAR) wbf[p] = f2bf(par[p]);
 }
 
-// Inference: branch = argmax of the 5 logits, out[q] = the 6 outputs.
+// GELU as x * sigmoid(2 y), y = k0 (x + k1 x^3) -- the tanh form above
+// rewritten (0.5 (1 + tanh y) = sigmoid(2 y)) -- with the hardware exp2 and
+// reciprocal: a few ulps of FP32 from gelu(), far below the BF16 rounding
+// the activations get next.
+__device__ __forceinline__ float gelu_fast(float x) {
+    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+    const float y = k0 * x * (1.f + k1 * x * x);
+    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * y);  // exp(-2 y)
+    return x * __builtin_amdgcn_rcpf(1.f + e);
+}
+
+__device__ __forceinline__ bf16x4 to_bf16x4(f32x4 v) {
+    bf16x4 b;
+#pragma unroll
+    for (int i = 0; i < 4; i++) b[i] = f2bf(v[i]);
+    return b;
+}
+
+// Inference over n samples, 16 per wavefront step, every layer computed
+// transposed (Z^T = W^T A^T): then the C fragment of one layer's 16 x 16
+// output tile -- lane = sample + 16 (hidden / 4), value hidden % 4 -- is
+// exactly the B fragment of the next layer's K step over those 16 hidden
+// units, so activations stay in registers from input to output, and the
+// weights sit in registers as A fragments for the whole grid-stride loop
+// (W1^T 4 tiles, W2^T 16, W3^T 4: 48 VGPRs).  Biases start the
+// accumulators.  Inputs are read as one 16-byte vector per lane (features
+// 4g..4g+3 of the lane's sample, g < 2), outputs written by the lanes that
+// hold them.
 __global__ void __launch_bounds__(4 * kWave)
 k_mlp_predict(const short* __restrict__ wbf, const float* __restrict__ par, const float* __restrict__ feat, int64_t n,
               int* __restrict__ branch, float* __restrict__ out) {
-    __shared__ WeightsLDS w;
-    __shared__ WaveLDS wl[4];
-    load_weights(w, wbf, par);
-    __syncthreads();
-    const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x & 63;
-    WaveLDS& s = wl[wave];
-    const int64_t gw = (int64_t)blockIdx.x * 4 + wave;
-    const int64_t nw = (int64_t)gridDim.x * 4;
-    const int64_t ntile = (n + 15) / 16;
-    const int64_t rounds = (ntile + nw - 1) / nw;
-    for (int64_t rd = 0; rd < rounds; rd++) {
-        const int64_t s0 = (rd * nw + gw) * 16;
-        forward16(w, s, feat, s0, n);
-        if (lane < 16 && s0 + lane < n) {
-            const int64_t q = s0 + lane;
-            int best = 0;
-            for (int j = 1; j < NBR; j++)
-                if (s.Z3[lane][j] > s.Z3[lane][best]) best = j;
-            if (branch) branch[q] = best;
-            if (out)
-                for (int j = 0; j < OUT; j++) out[q * OUT + j] = s.Z3[lane][j];
+    const int s = lane & 15, g = lane >> 4;
+    bf16x4 a1[H / 16], a2[H / 16][H / 16], a3[H / 16];
+    f32x4 c1[H / 16], c2[H / 16], c3;
+#pragma unroll
+    for (int t = 0; t < H / 16; t++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int k = 4 * g + i;
+            a1[t][i] = wbf[OFF_W1 + k * H + 16 * t + s];
+            a3[t][i] = wbf[OFF_W3 + (16 * t + k) * OUTP + s];
+            c1[t][i] = par[OFF_B1 + 16 * t + k];
+            c2[t][i] = par[OFF_B2 + 16 * t + k];
+#pragma unroll
+            for (int t2 = 0; t2 < H / 16; t2++) a2[t2][t][i] = wbf[OFF_W2 + (16 * t + k) * H + 16 * t2 + s];
         }
-        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) c3[i] = par[OFF_B3 + 4 * g + i];
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    for (int64_t s0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16; s0 < n; s0 += nw * 16) {
+        const int64_t q = s0 + s;
+        bf16x4 xb = {0, 0, 0, 0};
+        if (g < 2 && q < n) {
+            const f32x4 f = *reinterpret_cast<const f32x4*>(feat + q * FEAT_STRIDE + 4 * g);
+            xb = to_bf16x4(f);
+        }
+        bf16x4 h1[H / 16];
+#pragma unroll
+        for (int t = 0; t < H / 16; t++) {
+            f32x4 z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[t], xb, c1[t], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) z[i] = gelu_fast(z[i]);
+            h1[t] = to_bf16x4(z);
+        }
+        bf16x4 h2[H / 16];
+#pragma unroll
+        for (int t2 = 0; t2 < H / 16; t2++) {
+            f32x4 z = c2[t2];
+#pragma unroll
+            for (int t = 0; t < H / 16; t++) z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t2][t], h1[t], z, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) z[i] = gelu_fast(z[i]);
+            h2[t2] = to_bf16x4(z);
+        }
+        f32x4 o = c3;
+#pragma unroll
+        for (int t = 0; t < H / 16; t++) o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[t], h2[t], o, 0, 0, 0);
+        // lane (s, g) holds outputs 4g..4g+3 of sample s; logit 4 sits in lane s + 16
+        const float o4 = __shfl(o[0], s + 16, kWave);
+        if (q < n) {
+            if (g == 0) {
+                if (branch) {
+                    int best = 0;
+                    float bv = o[0];
+#pragma unroll
+                    for (int j = 1; j < 4; j++)
+                        if (o[j] > bv) {
+                            bv = o[j];
+                            best = j;
+                        }
+                    if (o4 > bv) best = 4;
+                    branch[q] = best;
+                }
+                if (out)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) out[q * OUT + j] = o[j];
+            } else if (g == 1 && out) {
+                out[q * OUT + 4] = o[0];
+                out[q * OUT + 5] = o[1];
+            }
+        }
     }
 }
 
@@ -434,7 +509,7 @@ namespace ipf {
 
 namespace {
 
-constexpr int kSortBlock = 256;
+constexpr int kSortBlock = IPF_SORT_BLOCK;
 
 // counting sort of keys in [0, NK): per-block histograms
 template <int NK>

@@ -605,6 +605,59 @@ IPF_HD void tangent3(const double* grad, const double* sev, const double* eev, c
     for (int i = 0; i < 9; i++) dep9[i] = dep9[i] * w;
 }
 
+// ---------------------------------------------- certified elastic points
+//
+// A test, from the trial stress alone, that the reference's return map would
+// leave the point elastic: the closed-form principal stresses of the
+// plane-strain trial stress (szz and the in-plane pair c +- R) put the main
+// Mohr-Coulomb plane below zero by a margin that covers how far the
+// reference's eigenvalues can sit from the exact ones.  Those come from at
+// most 20 Newton steps on the characteristic cubic of the tensor normalised
+// by its largest entry M (FunctionsSpectralDecomp.h:61-111), started outside
+// the spectrum at the (quirked) Gershgorin bounds, at most M beyond the
+// outermost eigenvalue: the iterates approach the largest / smallest root
+// monotonically and shrink the error by 2/3 or better per step (the
+// logarithmic derivative of a cubic with real roots), and the loop stops
+// when |f| <= 1e-11 >= e^3.  So each computed eigenvalue (the third one by
+// the trace included, after the sort) is within max((2/3)^20, 1e-11^(1/3)) M
+// < 6e-4 M of an exact one, and the plane value within 2 * 6e-4 M.  A point
+// with phi + 2e-3 M < 0 is therefore elastic in the reference (PhiPlane,
+// FunctionsSigmaProjection.h:6-20: phi < 0 or |phi| < 1e-12): mtype 0, its
+// stress the eigen-reconstruction of the trial stress, which equals the
+// trial stress up to that reconstruction's rounding and the Newton residual
+// of its eigenvalues.  Where two principal stresses nearly coincide the
+// reference's eigenvector formulas (FunctionsSpectralDecomp.h:116-256)
+// amplify that residual by the inverse gap and its stress departs from the
+// trial stress (measured on the oracle: up to half of M for coincident
+// eigenvalues, ~1e-9 M at gaps of 1e-2 M, ~1e-13 M typical), so the test
+// also asks for gaps of at least 1e-2 M: such points keep the reference's
+// values through the exact path.  Used only where nothing is committed.
+// Returns true and ws3 = w * [sxx, sxy, syy] of the trial stress for a
+// certified point; false (ws3 untouched) otherwise, NaN included.
+IPF_HD bool certified_elastic(const double* e3, const double* ep4, const MatConst& m, double w, double* ws3) {
+    const double exx = e3[0] - ep4[0];
+    const double exy = e3[1] - ep4[1];
+    const double eyy = e3[2] - ep4[2];
+    const double ezz = 0 - ep4[3];
+    const double sxx = m.lam2mu * exx + m.lambda * eyy + m.lambda * ezz;
+    const double syy = m.lambda * exx + m.lam2mu * eyy + m.lambda * ezz;
+    const double szz = m.lambda * exx + m.lambda * eyy + m.lam2mu * ezz;
+    const double sxy = m.mu * exy;
+    const double M = fmax(fmax(fabs(sxx), fabs(syy)), fmax(fabs(szz), fabs(sxy)));
+    const double c = 0.5 * (sxx + syy), d = 0.5 * (sxx - syy);
+    const double R = sqrt(d * d + sxy * sxy);
+    if (!(c - R <= c + R) || !(szz == szz)) return false;  // NaN anywhere
+    const double s1 = fmax(c + R, szz), s3 = fmin(c - R, szz);
+    const double phi = (s1 - s3) + (s1 + s3) * m.sinphi - m.phi_c;
+    if (!(phi + 2e-3 * M + 1e-12 * (M + m.phi_c) < 0.0)) return false;
+    const double gap = fmin(2.0 * R, fmin(fabs(c + R - szz), fabs(c - R - szz)));
+    if (!(gap >= 1e-2 * M)) return false;
+    ws3[0] = sxx * w;
+    ws3[1] = sxy * w;
+    ws3[2] = syy * w;
+    return true;
+}
+
 // ---------------------------------------------------------------- one point
 
 // One integration point of ComputeSigma / ComputeSigmaDep

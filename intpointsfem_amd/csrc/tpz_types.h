@@ -101,15 +101,25 @@ private:
     std::vector<T> fData;
 };
 
+// Base of NeoPZ's matrices: what TPZStructMatrix::Create() returns and
+// Assemble(mat, rhs, gui) takes.
+template <class T>
+class TPZMatrix {
+public:
+    virtual ~TPZMatrix() = default;
+    virtual int64_t Rows() const = 0;
+    virtual int64_t Cols() const { return Rows(); }
+};
+
 // Symmetric sparse matrix, upper triangle in CSR (NeoPZ TPZSYsmpMatrix:
 // IA row pointers, JA column indexes, A values).
 template <class T>
-class TPZSYsmpMatrix {
+class TPZSYsmpMatrix : public TPZMatrix<T> {
 public:
     TPZVec<int64_t>& IA() { return fIA; }
     TPZVec<int64_t>& JA() { return fJA; }
     TPZVec<T>& A() { return fA; }
-    int64_t Rows() const { return fIA.size() ? fIA.size() - 1 : 0; }
+    int64_t Rows() const override { return fIA.size() ? fIA.size() - 1 : 0; }
     T GetVal(int64_t r, int64_t c) const {
         if (r > c) std::swap(r, c);
         for (int64_t k = fIA[r]; k < fIA[r + 1]; k++)
@@ -129,6 +139,27 @@ public:
 private:
     TPZVec<int64_t> fIA, fJA;
     TPZVec<T> fA;
+};
+
+// NeoPZ's progress / cancel hook passed through the assembly calls; the
+// integration-point path never consults it.
+class TPZGuiInterface {
+public:
+    virtual ~TPZGuiInterface() = default;
+};
+
+// NeoPZ's reference-counted pointer (shared ownership).
+template <class T>
+class TPZAutoPointer {
+public:
+    TPZAutoPointer() = default;
+    explicit TPZAutoPointer(T* p) : fRef(p) {}
+    T* operator->() const { return fRef.get(); }
+    T& operator*() const { return *fRef; }
+    explicit operator bool() const { return (bool)fRef; }
+
+private:
+    std::shared_ptr<T> fRef;
 };
 
 // RAII device vector, move-only.

@@ -257,3 +257,46 @@ def test_blocks_from_corners_and_shape_table(p):
             assert np.array_equal(blk[3 * ip + 1, 0::2], dy) and np.array_equal(blk[3 * ip + 1, 1::2], dx)
             assert not blk[3 * ip, 1::2].any() and not blk[3 * ip + 2, 0::2].any()
             assert ix["weight"][b["row_first"][e] // 3 + ip] == wq[ip] * abs(det)
+
+
+def test_clone_keeps_host_data_after_original_is_gone():
+    """Clone() (TPZElastoPlasticIntPointsStructMatrix.cpp:30-32) copies the
+    host data; the original may be destroyed first (no shared handle is
+    freed twice, unlike TPZVecGPU's implicit copy, TPZVecGPU.h:24-27)."""
+    m = ipf.CompMesh(5, 4, 2, 0.1, 4.0, 1.1)
+    m.set_material(5000, 0.2, 10, 0.5, 0.5)
+    m.set_normal_stress(3.0, 0.0)
+    sm = ipf.StructMatrix(m, device=-1)
+    ref = sm.export_indexes()
+    refb = sm.export_blocks()
+    c = sm.clone()
+    del sm
+    got = c.export_indexes()
+    for k in ref:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    np.testing.assert_array_equal(c.export_blocks()["storage"], refb["storage"])
+    assert c.create_tangent() > 0
+
+
+@pytest.mark.parametrize("p", [1, 3])
+def test_csr_vectors_describe_the_block_matrix(p):
+    """CSRVectors (TPZIrregularBlocksMatrix.cpp:177-195): (row_ptr, col_ind)
+    over fStorage is the block-diagonal matrix of the blocks -- checked
+    against scipy's block_diag of the exported blocks, entry by entry, and
+    as an operator on a random vector."""
+    import scipy.sparse as sp
+    m = ipf.CompMesh(3, 4, p, 0.1, 4.0, 1.2)
+    m.set_material(5000, 0.2, 10, 0.5, 0.5)
+    sm = ipf.StructMatrix(m, device=-1)
+    s = sm.sizes()
+    b = sm.export_blocks()
+    rp, ci = sm.export_csr()
+    assert rp[0] == 0 and rp[-1] == s["nstorage"]
+    A = sp.csr_matrix((b["storage"], ci, rp), shape=(s["rows"], s["cols"]))
+    blocks = [b["storage"][b["matrix_position"][e]:b["matrix_position"][e + 1]].reshape(b["row_sizes"][e],
+                                                                                         b["col_sizes"][e])
+              for e in range(s["nblocks"])]
+    D = sp.block_diag(blocks, format="csr")
+    assert abs(A - D).max() == 0
+    x = np.random.default_rng(p).standard_normal(s["cols"])
+    np.testing.assert_allclose(A @ x, D @ x, rtol=0, atol=0)

@@ -95,3 +95,48 @@ def test_assemble_tangent_errors(gpu):
         ctx.set_tangent_pattern(ix["el_color_index"], bad, len(pat["ja"]))
     with pytest.raises(ipf.IPFError, match="color order"):
         ctx.set_tangent_pattern(ix["el_color_index"][::-1].copy(), pat["color_l_sequence"], len(pat["ja"]))
+
+
+def test_create_assemble_equals_create_then_assemble(gpu):
+    """CreateAssemble(rhs, gui) (TPZElastoPlasticIntPointsStructMatrix.cpp:
+    71-77) = Create() then Assemble(K, rhs): the same CSR values and rhs bit
+    for bit as the two calls on a second structural matrix of an identical
+    mesh, at a plastic state."""
+    mesh = (12, 10, 2, 1.1)
+    m1, m2 = wb.make_mesh(mesh), wb.make_mesh(mesh)
+    u = wb.lame_displacement(m1.nodes(), wb.normal_stress(10), scale=6.0)
+    sm1 = ipf.StructMatrix(m1, device=0)
+    r1, k1 = sm1.assemble_tangent(u)
+    sm2 = ipf.StructMatrix(m2, device=0)
+    r2, k2 = sm2.create_assemble(u)
+    assert len(k1) == len(k2)
+    np.testing.assert_array_equal(k2, k1)
+    np.testing.assert_array_equal(r2, r1)
+    # a second call builds a new matrix with the same values
+    r3, k3 = sm2.create_assemble(u)
+    np.testing.assert_array_equal(k3, k1)
+    np.testing.assert_array_equal(r3, r1)
+
+
+def test_clone_shares_the_device_context(gpu):
+    """Clone() shares the device context by reference count: residuals
+    through the clone equal the original's, a commit through one is seen by
+    the other, and the clone keeps working after the original is destroyed."""
+    m = wb.make_mesh((10, 8, 2, 1.1))
+    u = wb.lame_displacement(m.nodes(), wb.normal_stress(10), scale=6.0)
+    sm = ipf.StructMatrix(m, device=0)
+    c = sm.clone()
+    r0 = sm.assemble_rhs(u)
+    np.testing.assert_array_equal(c.assemble_rhs(u), r0)
+    m.set_update_mem(1)
+    c.assemble_rhs(u)  # commits the plastic strain at u through the clone
+    m.set_update_mem(0)
+    ep_c = c.context().get_state()[0]
+    assert np.abs(ep_c).max() > 0
+    np.testing.assert_array_equal(sm.context().get_state()[0], ep_c)
+    r_committed = sm.assemble_rhs(u)
+    assert np.abs(r_committed - r0).max() > 0  # the original sees the committed state
+    del sm
+    import gc
+    gc.collect()
+    np.testing.assert_array_equal(c.assemble_rhs(u), r_committed)

@@ -1,0 +1,329 @@
+"""The production workloads against the CPU oracle, at their full sizes.
+
+  * 800 x 800 Q2 (5.76 M points) and Q3 (10.24 M points), FP64 (library
+    default B format, recomputed from the element corners): two states --
+    a commit at u1 from zero plastic strain, then u2 from that committed
+    state -- with branch flags, mtype and plastic strain bit-identical to the
+    oracle's and residuals within 1e-12 of max|f_int| (the GPU accumulates
+    the colors in sequence, the oracle in the reference's pairwise slab
+    order: TPZNumericalIntegrator.cpp:81-105).
+  * the FP32 B formats on the same states against the same oracle residual.
+    B.u in FP32 cancels nodal displacements about r / h larger than the
+    strain (h the cell size), so the error grows with the mesh: measured
+    5.1e-4 (Q2) and 9.0e-4 (Q3) of max|f_int| at 800 x 800 (8.2e-6 at 40 x
+    40).  Bars 1e-3 (Q2) and 2e-3 (Q3); branch flags compared point by point,
+    the mismatch fraction printed and bounded by 1e-3 (points whose trial
+    stress sits within FP32 rounding of a branch switch).
+  * mixed-precision Newton at 800 x 800 Q2 (FP32 tangent products in the CG,
+    FP64 residuals deciding convergence: iterative refinement) against the
+    FP64 Newton, one load step to the full wellbore pressure from rest, both
+    to ||r|| <= 1e-8 ||r0||.
+  * the 64 uneven instances of the seed-0 table (tests/golden/
+    batch64_instances.json) in one batched context, every instance's slice
+    against that instance's oracle residual, flags and plastic strain.
+  * the learned branch predictor trained on pairs of the exact return map
+    from the 800 x 800 Q2 / Q3 meshes and the 64-instance batch (held-out
+    accuracy printed; bar: above the majority class), then used to order the
+    batch's points: the branch-sorted ComputeSigma gives the unsorted
+    results bit for bit.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import intpointsfem_amd as ipf
+from intpointsfem_amd import batch as bt
+from intpointsfem_amd import learned
+from intpointsfem_amd import wellbore as wb
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NT = 16  # oracle threads: the job's share of the GPU box's host
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _oracle_inputs(sm):
+    b = sm.export_blocks()
+    ix = sm.export_indexes()
+    B = O.Blocks(b["row_sizes"], b["col_sizes"], b["matrix_position"], b["row_first"], b["col_first"], b["storage"])
+    return B, ix
+
+
+def _oracle_residual(B, ix, s, mat, u, ep=None, al=None):
+    return O.residual(B, ix["dof_indexes"], ix["color_indexes"], s["ncolor"], s["neq"], ix["weight"], mat["lambda"],
+                      mat["mu"], mat["phi"], mat["psi"], mat["cohesion"], u, f_lin=ix["rhs_linear"],
+                      plastic_strain6=ep, alpha=al, update_mem=True, nthreads=NT)
+
+
+def _gpu_residual(ctx, u, neq, dev, commit):
+    import torch
+    r = torch.full((neq,), float("nan"), dtype=torch.float64, device=dev)
+    ctx.set_update_mem(commit)
+    ctx.residual(u, r)
+    ctx.set_update_mem(False)
+    ctx.synchronize()
+    return r.cpu().numpy()
+
+
+def _rel(r, ro, flin):
+    return np.abs(r - ro).max() / np.abs(ro - flin).max()
+
+
+@pytest.mark.parametrize("mesh,fp32_bar", [("800x800-q2", 1e-3), ("800x800-q3", 2e-3)])
+def test_production_mesh_matches_oracle(gpu, mesh, fp32_bar):
+    import torch
+    m = wb.make_mesh(mesh)
+    sm = ipf.StructMatrix(m, device=0)
+    ctx = sm.context()
+    s = sm.sizes()
+    neq, npts = s["neq"], s["npts"]
+    mat = m.material()
+    _log(f"{mesh}: {s['nblocks']} elements, {npts} points, neq {neq}; exporting the host blocks")
+    B, ix = _oracle_inputs(sm)
+    flin = ix["rhs_linear"]
+    u1h = wb.lame_displacement(m.nodes(), wb.normal_stress(10), scale=2.0)
+    u2h = wb.lame_displacement(m.nodes(), wb.normal_stress(10), scale=3.0)
+    u1 = torch.as_tensor(u1h, device=gpu)
+    u2 = torch.as_tensor(u2h, device=gpu)
+    fmt0 = ctx.b_format()[0]
+    # state 1: commit at u1 from zero plastic strain
+    r1 = _gpu_residual(ctx, u1, neq, gpu, True)
+    ep1, al1, mt1, br1 = ctx.get_state()
+    ro1, epo1, alo1, mto1, bro1 = _oracle_residual(B, ix, s, mat, u1h)
+    e1 = _rel(r1, ro1, flin)
+    _log(f"{mesh} ({fmt0}) state 1: {int((mto1 != 0).sum())} plastic points; residual {e1:.2e} of max|f_int|, "
+         f"flag mismatches {int((br1 != bro1).sum())}, plastic strain mismatches {int((ep1 != epo1).sum())}")
+    assert np.array_equal(br1, bro1) and np.array_equal(mt1, mto1)
+    assert np.array_equal(ep1, epo1) and np.array_equal(al1, alo1)
+    assert e1 <= 1e-12
+    # state 2: u2 from the committed state 1 -- non-committing, then committing
+    ro2, epo2, alo2, mto2, bro2 = _oracle_residual(B, ix, s, mat, u2h, epo1, alo1)
+    r2 = _gpu_residual(ctx, u2, neq, gpu, False)
+    e2 = _rel(r2, ro2, flin)
+    r2c = _gpu_residual(ctx, u2, neq, gpu, True)
+    ep2, al2, mt2, br2 = ctx.get_state()
+    _log(f"{mesh} state 2: {int((mto2 != 0).sum())} plastic points ({len(np.unique(bro2))} branches); "
+         f"residual {e2:.2e}, flag mismatches {int((br2 != bro2).sum())}")
+    assert np.array_equal(r2, r2c)
+    assert np.array_equal(br2, bro2) and np.array_equal(mt2, mto2)
+    assert np.array_equal(ep2, epo2) and np.array_equal(al2, alo2)
+    assert e2 <= 1e-12
+    # FP32 B formats on the same states
+    for fmt in ("compact-fp32", "dense-fp32"):
+        ctx.set_b_format(fmt)
+        ctx.set_state(epo1, alo1)
+        rf = _gpu_residual(ctx, u2, neq, gpu, False)
+        ef = _rel(rf, ro2, flin)
+        _gpu_residual(ctx, u2, neq, gpu, True)
+        _, _, _, brf = ctx.get_state()
+        mism = int((brf != bro2).sum())
+        _log(f"{mesh} {fmt}: residual {ef:.2e} of max|f_int| (bar {fp32_bar:.0e}), "
+             f"flag mismatches {mism} of {npts} ({mism / npts:.2e})")
+        assert ef <= fp32_bar
+        assert mism <= 1e-3 * npts
+    ctx.set_b_format(fmt0)
+    # the certified elastic path: non-committing residuals within 1e-10 of the
+    # oracle's; a committing residual in that mode runs the exact path
+    ctx.set_state(epo1, alo1)
+    ctx.set_residual_mode("certified-elastic")
+    rfast = _gpu_residual(ctx, u2, neq, gpu, False)
+    efast = _rel(rfast, ro2, flin)
+    rfc = _gpu_residual(ctx, u2, neq, gpu, True)
+    ep3, al3, mt3, br3 = ctx.get_state()
+    ctx.set_residual_mode("exact")
+    _log(f"{mesh} certified-elastic residual: {efast:.2e} of max|f_int| against the oracle, "
+         f"{np.abs(rfast - r2).max() / np.abs(r2 - flin).max():.2e} against the exact GPU residual")
+    assert efast <= 1e-10
+    assert np.array_equal(rfc, r2c)
+    assert np.array_equal(br3, bro2) and np.array_equal(ep3, epo2) and np.array_equal(al3, alo2)
+    # the patch assembly (exact and certified-elastic) on the same state
+    ctx.set_state(epo1, alo1)
+    ctx.set_residual_assembly("patch")
+    rpe = _gpu_residual(ctx, u2, neq, gpu, False)
+    ctx.set_residual_mode("certified-elastic")
+    rpf = _gpu_residual(ctx, u2, neq, gpu, False)
+    ctx.set_residual_mode("exact")
+    ctx.set_residual_assembly("colored")
+    epe, epf = _rel(rpe, ro2, flin), _rel(rpf, ro2, flin)
+    _log(f"{mesh} patch assembly: exact {epe:.2e}, certified-elastic {epf:.2e} of max|f_int| against the oracle; "
+         f"{ctx.patch_info()}")
+    assert epe <= 1e-12 and epf <= 1e-10
+
+
+def test_mixed_precision_newton_800x800_q2(gpu):
+    """FP32 tangent products (compact FP32 B, FP32 accumulation) inside the
+    CG, FP64 residuals: iterative refinement reaches the FP64 Newton's
+    solution.  One load step to the full pressure from rest (a plastic zone
+    forms in the step); both solves to ||r|| <= 1e-8 ||r0||."""
+    import time
+
+    import torch
+    out = {}
+    for tfmt in ("geometry-fp64", "compact-fp32"):
+        m = wb.make_mesh("800x800-q2")
+        sm = ipf.StructMatrix(m, device=0)
+        ctx = sm.context()
+        neq = sm.sizes()["neq"]
+        ctx.set_dirichlet_mask(m.dirichlet_mask())
+        ctx.set_b_format(ctx.b_format()[0], tangent=tfmt)
+        u = torch.zeros(neq, dtype=torch.float64, device=gpu)
+        t0 = time.time()
+        st = ctx.newton(u, rtol=1e-8, maxit=60, cg_rtol=1e-1, cg_maxit=20000)
+        el = time.time() - t0
+        ep, al, mt, br = ctx.get_state()
+        _log(f"newton, tangent {tfmt}: {int(st['iterations'])} iterations, {int(st['cg_iterations'])} CG iterations, "
+             f"|r|/|r0| {st['rel_residual']:.2e}, {el:.1f} s, plastic points {int((mt != 0).sum())}")
+        assert st["converged"] == 1.0
+        out[tfmt] = (u.cpu().numpy(), br)
+        del ctx, sm
+    (u64, b64), (u32, b32) = out["geometry-fp64"], out["compact-fp32"]
+    du = np.abs(u32 - u64).max() / np.abs(u64).max()
+    mism = int((b32 != b64).sum())
+    _log(f"mixed vs FP64 Newton: max |du| / max |u| {du:.2e}, flag mismatches {mism} of {len(b64)}")
+    assert du <= 1e-6
+    assert mism <= 1e-4 * len(b64)
+
+
+def _batch64():
+    with open(os.path.join(HERE, "golden", "batch64_instances.json")) as f:
+        table = json.load(f)
+    return table["instances"] if isinstance(table, dict) else table
+
+
+@pytest.fixture(scope="module")
+def batch64(gpu):
+    import torch
+    table = _batch64()
+    assert len(table) == 64 and table == bt.draw_instances(64, seed=0)
+    _log("batch64: building the 64 instance meshes and the batched context")
+    meshes = [bt.instance_mesh(i) for i in table]
+    B = bt.Batch(meshes, device=0)
+    us = [wb.lame_displacement(m.nodes(), wb.normal_stress(wb.NSTEPS, wb.NSTEPS, i["pw_final"], wb.SIGMA0),
+                               E_=i["E"], nu=i["nu"]) for i, m in zip(table, meshes)]
+    u1 = np.concatenate([2.0 * x for x in us])
+    u2 = np.concatenate([3.0 * x for x in us])
+    _log(f"batch64: {B.npts} points, neq {B.neq}; residuals at u1 (commit) and u2")
+    r1 = _gpu_residual(B.ctx, torch.as_tensor(u1, device=gpu), B.neq, gpu, True)
+    st1 = B.ctx.get_state()
+    r2 = _gpu_residual(B.ctx, torch.as_tensor(u2, device=gpu), B.neq, gpu, False)
+    B.ctx.set_residual_mode("certified-elastic")
+    r2f = _gpu_residual(B.ctx, torch.as_tensor(u2, device=gpu), B.neq, gpu, False)
+    B.ctx.set_residual_assembly("patch")
+    r2p = _gpu_residual(B.ctx, torch.as_tensor(u2, device=gpu), B.neq, gpu, False)
+    B.ctx.set_residual_assembly("colored")
+    B.ctx.set_residual_mode("exact")
+    _gpu_residual(B.ctx, torch.as_tensor(u2, device=gpu), B.neq, gpu, True)
+    st2 = B.ctx.get_state()
+    return dict(table=table, meshes=meshes, B=B, u1=u1, u2=u2, r1=r1, r2=r2, r2f=r2f, r2p=r2p, st1=st1, st2=st2)
+
+
+@pytest.mark.parametrize("chunk", range(4))
+def test_batch64_instances_match_oracle(gpu, batch64, chunk):
+    d = batch64
+    Bt = d["B"]
+    worst = 0.0
+    for i in range(16 * chunk, 16 * chunk + 16):
+        m = d["meshes"][i]
+        sm = ipf.StructMatrix(m, device=-1)
+        s = sm.sizes()
+        Bo, ix = _oracle_inputs(sm)
+        mat = m.material()
+        e0, e1 = Bt.eq_offsets[i], Bt.eq_offsets[i + 1]
+        p0, p1 = Bt.pt_offsets[i], Bt.pt_offsets[i + 1]
+        ro1, epo1, alo1, mto1, bro1 = _oracle_residual(Bo, ix, s, mat, d["u1"][e0:e1])
+        ro2, epo2, alo2, mto2, bro2 = _oracle_residual(Bo, ix, s, mat, d["u2"][e0:e1], epo1, alo1)
+        ep1, al1, mt1, br1 = d["st1"]
+        ep2, al2, mt2, br2 = d["st2"]
+        flin = ix["rhs_linear"]
+        ea = _rel(d["r1"][e0:e1], ro1, flin)
+        eb = _rel(d["r2"][e0:e1], ro2, flin)
+        ef = _rel(d["r2f"][e0:e1], ro2, flin)
+        ep_ = _rel(d["r2p"][e0:e1], ro2, flin)
+        worst = max(worst, ea, eb)
+        inst = d["table"][i]
+        _log(f"instance {i} (N {inst['N']}, p {inst['p']}): {p1 - p0} points, plastic {int((mto2 != 0).sum())}, "
+             f"residuals {ea:.1e} / {eb:.1e}, certified-elastic {ef:.1e}, patch + certified-elastic {ep_:.1e}")
+        assert ef <= 1e-10 and ep_ <= 1e-10
+        assert np.array_equal(br1[p0:p1], bro1) and np.array_equal(mt1[p0:p1], mto1)
+        assert np.array_equal(ep1[6 * p0:6 * p1], epo1) and np.array_equal(al1[p0:p1], alo1)
+        assert np.array_equal(br2[p0:p1], bro2) and np.array_equal(mt2[p0:p1], mto2)
+        assert np.array_equal(ep2[6 * p0:6 * p1], epo2)
+        assert ea <= 1e-12 and eb <= 1e-12
+    _log(f"chunk {chunk}: worst residual deviation {worst:.2e} of max|f_int|")
+
+
+def test_learned_predictor_trained_on_production_pairs(gpu, batch64):
+    """Pairs of the exact return map from the 800 x 800 Q2 and Q3 meshes and
+    the 64-instance batch (trial states at multiples of each problem's
+    linear-elastic displacement), training on the GPU, then the predictor
+    orders the batch's points for ComputeSigma: identical results."""
+    import torch
+    parts = []
+    for mesh in ("800x800-q2", "800x800-q3"):
+        m = wb.make_mesh(mesh)
+        sm = ipf.StructMatrix(m, device=0)
+        ctx = sm.context()
+        npts = sm.sizes()["npts"]
+        base = wb.lame_displacement(m.nodes(), wb.normal_stress(10))
+        for scale in (1.5, 3.0):
+            parts.append(learned.point_features(ctx, torch.as_tensor(base * scale, device=gpu), npts, gpu))
+        ctx.synchronize()
+        del ctx, sm
+    d = batch64
+    Bt = d["B"]
+    for u in (d["u1"], d["u2"]):
+        parts.append(learned.point_features(Bt.ctx, torch.as_tensor(u, device=gpu), Bt.npts, gpu))
+    Bt.ctx.synchronize()
+    feat, br, tg = (torch.cat([p[k] for p in parts]) for k in range(3))
+    n = br.shape[0]
+    counts = torch.bincount(br.long(), minlength=5).cpu().numpy()
+    _log(f"training pairs: {n} ({counts.tolist()} by branch)")
+    assert n >= 20_000_000
+    # a seeded 90 / 10 split; the held-out tenth of the batch's u2 pairs is the test set
+    nb = Bt.npts
+    g = torch.Generator(device="cpu").manual_seed(0)
+    held = torch.rand(n, generator=g) < 0.1
+    held[: n - nb] = False  # test pairs come from the batch at u2 only
+    train_idx = torch.nonzero(~held).squeeze(1).to(gpu)
+    test_idx = torch.nonzero(held).squeeze(1).to(gpu)
+    mlp = learned.new_mlp(seed=0)
+    ft, bt_, tt = feat.index_select(0, train_idx), br.index_select(0, train_idx), tg.index_select(0, train_idx)
+    learned.train(mlp, ft, bt_, tt, steps=3000, batch=4096, lr=3e-3, seed=0)
+    del ft, bt_, tt
+    pred = learned.predict(mlp, feat.index_select(0, test_idx).contiguous())
+    be_t = br.index_select(0, test_idx)
+    torch.cuda.synchronize()
+    acc = float((pred == be_t).float().mean())
+    major = float(torch.bincount(be_t.long(), minlength=5).max()) / be_t.shape[0]
+    _log(f"held-out tenth of the batch's u2 pairs ({be_t.shape[0]}): accuracy {acc:.4f}, majority class {major:.4f}")
+    # states beyond the first commit hold many points near the yield surface,
+    # where the branch hinges on the last bits of the trial stress (BF16
+    # inputs cannot resolve it): the bar is that it beats the majority class
+    assert acc > major + 0.02
+    fe, be = feat[n - nb:], br[n - nb:]
+    # the predictor orders the batch's points; the exact return map gives the unsorted results
+    ctx = Bt.ctx
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    u2 = torch.as_tensor(d["u2"], device=gpu)
+    strain = torch.zeros(3 * nb, dtype=torch.float64, device=gpu)
+    ws_a = torch.zeros(3 * nb, dtype=torch.float64, device=gpu)
+    ws_b = torch.zeros(3 * nb, dtype=torch.float64, device=gpu)
+    order = torch.zeros(nb, dtype=torch.int32, device=gpu)
+    ctx.multiply(u2, strain)
+    ctx.compute_sigma(strain, ws_a)
+    learned.compute_sigma_sorted(ctx, mlp, strain, ws_b, fe, order)
+    torch.cuda.synchronize()
+    assert torch.equal(ws_a, ws_b)
+    o = order.cpu().numpy()
+    assert np.array_equal(np.sort(o), np.arange(nb))
+    visit = be.cpu().numpy()[o]
+    _log(f"mixed-branch 64-point groups: index order {learned.divergence(be.cpu().numpy()):.4f}, "
+         f"predicted order {learned.divergence(visit):.4f}")
+    ctx.set_stream(None)

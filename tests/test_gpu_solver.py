@@ -182,3 +182,51 @@ def test_cg_graph_matches_single_launches(gpu):
     for (ug, ig, cg), (ue, ie, ce) in zip(res[True][0], res[False][0]):
         assert np.array_equal(ug, ue) and ig == ie and cg == ce
     assert all(np.array_equal(a, b) for a, b in zip(res[True][2], res[False][2]))
+
+
+def test_newton_at_a_converged_u_stops_on_the_absolute_floor(gpu):
+    """A second ipf_newton on the same load starts at the converged u, where
+    r0 is at rounding level and ||r|| <= rtol ||r0|| cannot be met.  With the
+    absolute floor (ipf_set_newton_atol; the reference stops on absolute
+    norms, main.cpp:235) it returns at once, converged, u unchanged."""
+    import torch
+    m, sm, ctx, B, ix, mask = _setup((16, 12, 2, 1.1))
+    neq = sm.sizes()["neq"]
+    f = ix["rhs_linear"] * (wb.normal_stress(3) / wb.normal_stress(wb.NSTEPS))  # load step 3
+    ctx.set_rhs_linear(f)
+    u = torch.zeros(neq, dtype=torch.float64, device=gpu)
+    st = ctx.newton(u, rtol=1e-10, maxit=80, cg_rtol=1e-13, cg_maxit=40000)
+    assert st["converged"] == 1.0 and st["iterations"] >= 1, st
+    u1 = u.cpu().numpy().copy()
+    fnorm = float(np.linalg.norm(f[mask == 0]))
+    ctx.set_newton_atol(1e-10 * fnorm)
+    st2 = ctx.newton(u, rtol=1e-12, maxit=40, cg_rtol=1e-13, cg_maxit=40000)
+    print(f"first call: {int(st['iterations'])} iterations, |r| {st['r']:.3e}; second call: "
+          f"{int(st2['iterations'])} iterations, |r| {st2['r']:.3e}, floor {1e-10 * fnorm:.3e}")
+    assert st2["converged"] == 1.0 and st2["iterations"] == 0
+    np.testing.assert_array_equal(u.cpu().numpy(), u1)
+    ctx.set_newton_atol(0.0)
+    with pytest.raises(ipf.IPFError):
+        ctx.set_newton_atol(-1.0)
+
+
+def test_newton_restores_update_mem_after_an_error(gpu):
+    """ipf_newton turns the caller's UpdateMem off for its Newton residuals;
+    an error inside (here: cg_maxit < 1, rejected by ipf_cg_solve) must leave
+    it as the caller set it -- a later residual still commits."""
+    import torch
+    m, sm, ctx, B, ix, mask = _setup((12, 10, 2, 1.1))
+    neq = sm.sizes()["neq"]
+    ctx.set_rhs_linear(ix["rhs_linear"])
+    ctx.set_update_mem(1)
+    u = torch.zeros(neq, dtype=torch.float64, device=gpu)
+    with pytest.raises(ipf.IPFError, match="maxit"):
+        ctx.newton(u, rtol=1e-12, maxit=5, cg_rtol=1e-13, cg_maxit=0)
+    ep, _, mt, _ = ctx.get_state()
+    assert not ep.any()  # nothing committed by the failed call
+    up = _t(wb.lame_displacement(m.nodes(), wb.normal_stress(10), scale=6.0), gpu)
+    r = torch.zeros(neq, dtype=torch.float64, device=gpu)
+    ctx.residual(up, r)
+    ctx.synchronize()
+    ep, _, mt, _ = ctx.get_state()
+    assert (mt != 0).any() and ep.any()  # UpdateMem was still on

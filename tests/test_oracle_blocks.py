@@ -127,11 +127,13 @@ def test_coloring_irregular_graph():
             nodes = set(conn[ptr[e]:ptr[e + 1]])
             assert not (nodes & seen)
             seen |= nodes
-    # greedy first-fit: an element of color c conflicts with some element of every lower color
+    # multi-pass greedy (TPZNumericalIntegrator.cpp:463-493): pass c visits the
+    # uncolored elements in index order, so an element left out of pass c
+    # conflicts with an element of color c that precedes it
     for e in range(200):
         ne = set(conn[ptr[e]:ptr[e + 1]])
         for c in range(col[e]):
-            assert any(ne & set(conn[ptr[f]:ptr[f + 1]]) for f in np.nonzero(col == c)[0] if f < e or True)
+            assert any(ne & set(conn[ptr[f]:ptr[f + 1]]) for f in np.nonzero(col == c)[0] if f < e)
 
 
 def test_element_tangents_match_numpy():

@@ -236,3 +236,108 @@ def test_plastic_tangent_shear_uses_trial_eigenvalues():
     h = 1e-8
     fd = (_sigma([e3[0], h, e3[2]], np.zeros(6)) - _sigma([e3[0], -h, e3[2]], np.zeros(6)))[1] / (2 * h)
     assert fd < 0.99 * G  # the exact derivative is softer
+
+
+# ------------------------------------------- tangent on every branch (FD + covariance)
+
+def _coaxial_states(psi, rng, per_branch=3):
+    """Seeded coaxial strains (zero shear, so the principal axes are x, y, z)
+    whose branch survives a 1e-3 relative change of either normal strain:
+    each state lies inside its branch region, away from the switches."""
+    found = {}
+    while any(len(found.get(b, [])) < per_branch for b in (1, 2, 3, 4)):
+        exx, eyy = rng.uniform(-0.03, 0.03, 2)
+        b = O.point_sigma([exx, 0.0, eyy], np.zeros(6), LAM, MU, PHI, psi, C)["branch"]
+        if b == 0 or len(found.get(b, [])) >= per_branch:
+            continue
+        if all(O.point_sigma([exx * (1 + dx), 0.0, eyy * (1 + dy)], np.zeros(6), LAM, MU, PHI, psi, C)["branch"] == b
+               for dx, dy in ((1e-3, 0), (-1e-3, 0), (0, 1e-3), (0, -1e-3))):
+            found.setdefault(b, []).append([exx, 0.0, eyy])
+    return found
+
+
+@pytest.mark.parametrize("psi_deg", [30.0, 10.0])
+def test_tangent_normal_columns_match_fd_on_every_plastic_branch(psi_deg):
+    """Main plane, right edge, left edge and apex (FunctionsSigmaProjection.h:
+    68-88, 163-188, 262-287, 328-332) through TangentOperator
+    (FunctionsTangentOperator.h:37-123): with principal axes along x, y, z the
+    xx and yy columns of Dep carry no eigenvector-rotation term, so they are
+    the exact derivative of the stress update, which is piecewise linear
+    inside a branch (H = 0).  Central differences with h = 1e-6 on strains of
+    order 1e-2 agree to rounding (measured <= 3e-11 of max|Dep|); a dropped
+    term, swapped sin(phi) / sin(psi) or a wrong edge formula moves an entry
+    by a sizeable fraction of it.  psi = 10 deg separates the phi and psi
+    terms of the closed forms.  Apex: both columns are zero."""
+    psi = math.radians(psi_deg)
+    states = _coaxial_states(psi, np.random.default_rng(int(psi_deg)))
+    h = 1e-6
+    for b, lst in states.items():
+        for e3 in lst:
+            out = O.point_sigma(e3, np.zeros(6), LAM, MU, PHI, psi, C, dep=True)
+            assert out["branch"] == b
+            scale = np.abs(out["wdep"]).max()
+            for j in (0, 2):
+                ep = np.array(e3)
+                em = np.array(e3)
+                ep[j] += h
+                em[j] -= h
+                fd = (O.point_sigma(ep, np.zeros(6), LAM, MU, PHI, psi, C)["wsigma"]
+                      - O.point_sigma(em, np.zeros(6), LAM, MU, PHI, psi, C)["wsigma"]) / (2 * h)
+                np.testing.assert_allclose(out["wdep"][:, j], fd, rtol=0, atol=1e-8 * scale,
+                                           err_msg=f"branch {b} strain {e3} column {j}")
+            if b == 4:
+                np.testing.assert_array_equal(out["wdep"][:, [0, 2]], 0.0)
+
+
+def _t_sigma(th):
+    """[sxx, sxy, syy] of R s R^T for a rotation by th about z."""
+    c, s = math.cos(th), math.sin(th)
+    return np.array([[c * c, -2 * c * s, s * s], [c * s, c * c - s * s, -c * s], [s * s, 2 * c * s, c * c]])
+
+
+def _t_eps(th):
+    """[exx, gxy, eyy] (engineering shear) of R e R^T."""
+    c, s = math.cos(th), math.sin(th)
+    return np.array([[c * c, -c * s, s * s], [2 * c * s, c * c - s * s, -2 * c * s], [s * s, c * s, c * c]])
+
+
+@pytest.mark.parametrize("psi_deg", [30.0, 10.0])
+def test_tangent_is_frame_covariant_on_every_branch(psi_deg):
+    """The stress update is an isotropic function of the strain, so its
+    tangent must transform as Dep(R e R^T) = T_s Dep(e) T_e^-1 for a rotation
+    R about z.  Starting from the coaxial states above (normal columns pinned
+    by finite differences, the xy-xy entry = G from the trial-eigenvalue
+    factor, test above), this pins the eigenvector-rotation term of
+    FunctionsTangentOperator.h:79-121 -- eigenvector products, the factor 2
+    of the off-diagonal Voigt columns, signs -- for every orientation, on
+    every branch and the elastic one.  In plane strain the in-plane block
+    does not see the accumulated tempMat quirk (the extra pairs all involve
+    the z eigenvector, whose products with in-plane columns vanish).
+    Measured agreement <= 3e-9 of max|Dep| (the eigenvalues carry the
+    Newton tolerance, amplified by 1 / gap in the rotation factor where two
+    strain eigenvalues are close); a tangent with the xy-xy entry halved
+    misses by ~0.1."""
+    psi = math.radians(psi_deg)
+    states = _coaxial_states(psi, np.random.default_rng(10 + int(psi_deg)), per_branch=2)
+    states[0] = [[-0.012, 0.0, -0.0046], [0.001, 0.0, -0.002]]
+    for b, lst in states.items():
+        for e3 in lst:
+            out = O.point_sigma(e3, np.zeros(6), LAM, MU, PHI, psi, C, dep=True)
+            assert out["branch"] == b
+            D0 = out["wdep"]
+            scale = np.abs(D0).max()
+            wrong = D0.copy()
+            wrong[1, 1] *= 0.5
+            for th in np.linspace(0.15, 3.0, 9):
+                E = np.array([[e3[0], e3[1] / 2], [e3[1] / 2, e3[2]]])
+                R = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+                Er = R @ E @ R.T
+                o2 = O.point_sigma([Er[0, 0], 2 * Er[0, 1], Er[1, 1]], np.zeros(6), LAM, MU, PHI, psi, C, dep=True)
+                assert o2["branch"] == b
+                Ti = np.linalg.inv(_t_eps(th))
+                np.testing.assert_allclose(o2["wdep"], _t_sigma(th) @ D0 @ Ti, rtol=0, atol=1e-8 * scale,
+                                           err_msg=f"branch {b} strain {e3} theta {th}")
+                np.testing.assert_allclose(o2["wsigma"], _t_sigma(th) @ out["wsigma"], rtol=0,
+                                           atol=1e-9 * np.abs(out["wsigma"]).max())
+                if abs(math.sin(2 * th)) > 0.3:  # the check above would catch a halved shear entry
+                    assert np.abs(o2["wdep"] - _t_sigma(th) @ wrong @ Ti).max() > 1e-2 * scale
