@@ -9,7 +9,8 @@ with block-Jacobi PCG inner solves, FP64 throughout -- runs first, untimed).  Th
 data is set up on the GPU from the element corners, and the library's B
 format is the geometric one: B and the weights recomputed inside the kernel
 from 8 corner coordinates per element, bit-identical to the stored dense and
-compact FP64 blocks built from the same geometry.
+compact FP64 blocks built from the same geometry.  The plain residual is the
+drop-in's: colored assembly, the reference's return map at every point.
 
 Before the W warm-up steps, --settle-ms of untimed back-to-back residuals
 bring the GPU to its sustained clocks (config.settle); the timed region is
@@ -22,21 +23,32 @@ Prints ONE JSON line on stdout (progress goes to stderr):
               coordinates per element; compact: dN/dx, dN/dy per node and
               point, + a weight per point), a 4-byte DoF index per element
               column, 4 plastic-strain components per point, u read + r
-              read-modify-written once -- over the kernel's average duration
-              from HIP events around each launch; `traffic` = HBM bytes per
-              launch measured by rocprofv3 (profiles/), `dense_fp64_bytes_rate`
-              = the rate on the dense blocks' byte count, bytes this kernel
-              does not move
+              written once -- over the kernel's average duration from HIP
+              events around each launch (dispatch gap removed); `traffic` =
+              HBM bytes per launch measured by rocprofv3 PMC (profiles/),
+              `traffic_over_algorithmic` its ratio to the compulsory bytes,
+              `second_bound` the VALU issue fraction (same profile)
   cpu_baseline the CPU oracle (same algorithm, OpenMP) on the host cores
+  residual_paths  the same residual with the patch assembly and / or the
+              certified-elastic stress path (ipf_set_residual_assembly /
+              ipf_set_residual_mode): time and deviation
   b_format_variants  the same residual with the other B formats (dense and
               compact FP64 and FP32; ipf_set_b_format): time, rate on each
               format's stored bytes, deviation from the FP64 residual
+  learned_mixed_batched  the 64 uneven instances at their converged
+              mixed-precision Newton state: the plain batched residual and the
+              learned + mixed one (predictor-ordered elements, certified-elastic
+              path, the faster assembly), speed-up, deviation from the plain
+              residual and from the CPU oracle, the predictor's figures and the
+              ablation of each ingredient
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -137,29 +149,37 @@ def kernel_name(fmt, shape, pairs=True):
 
 
 def launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt, step_ms=None):
-    """Per-launch HBM rates of the color kernels: each launch is credited with
-    its points' share of the plain FP64 algorithmic bytes (dense B) and of the
-    bytes the format actually stores.  Events around every launch add the
-    dispatch gap that back-to-back launches hide; with `step_ms` (the
-    back-to-back residual time) that gap, (sum of the per-launch times -
-    step_ms) / launches, is taken off each launch, so the durations are those
-    of the timed region."""
+    """Per-launch HBM rates of the residual kernels: each launch is credited
+    with its points' share of the compulsory bytes of what the format reads
+    (algorithmic_bytes).  Events around every launch add the dispatch gap
+    that back-to-back launches hide; with `step_ms` (the back-to-back residual
+    time) that gap -- (sum of the intervals that hold a kernel - step_ms) /
+    their number -- is taken off each kernel interval, so the durations are
+    those of the timed region.  The interval before the kernels (the f_lin
+    copy) holds no kernel when the launches read f_lin themselves (colored
+    assembly with every DoF in some element): it is left out of the gap and
+    reported as measured."""
     avg, pts = launch_times(ctx, u, r, nrep)
+    kern = pts > 0  # the intervals of the residual kernels proper
+    first_real = float(avg[0]) > 0.002  # a copy / refresh kernel ran in interval 0
+    has = kern.copy()
+    has[0] = first_real
+    tail = np.zeros(len(avg), dtype=bool)
+    tail[1:] = ~kern[1:]  # e.g. the patch fix-up: a kernel without points of its own
+    has |= tail
     gap = 0.0
     if step_ms is not None:
-        gap = max(0.0, (float(avg.sum()) - step_ms) / len(avg))
-    isolated = float(avg[1:].mean())
-    avg = avg - gap
-    color_ms = avg[1:]
-    share = pts[1:] / npts
-    alg = algorithmic_bytes(blocks, npts, neq) * share
+        gap = max(0.0, (float(avg[has].sum()) - step_ms) / int(has.sum()))
+    isolated = float(avg[kern].mean())
+    adj = np.where(has, avg - gap, avg)
+    main_ms = adj[kern]
+    share = pts[kern] / npts
     sto = algorithmic_bytes(blocks, npts, neq, fmt) * share
-    t = color_ms.sum() / 1e3
-    return {"copy_ms": float(avg[0]), "color_ms": color_ms, "avg_launch_ms": float(color_ms.mean()),
-            "avg_launch_ms_isolated": isolated, "dispatch_gap_ms": gap,
-            "bytes_per_launch": float(alg.mean()), "stored_bytes_per_launch": float(sto.mean()),
-            "stored_bytes": algorithmic_bytes(blocks, npts, neq, fmt),
-            "achieved": float(alg.sum() / t / 1e9), "achieved_stored": float(sto.sum() / t / 1e9)}
+    t = main_ms.sum() / 1e3
+    return {"copy_ms": float(adj[0]), "color_ms": main_ms, "other_ms": [float(x) for x in adj[tail]],
+            "avg_launch_ms": float(main_ms.mean()), "avg_launch_ms_isolated": isolated, "dispatch_gap_ms": gap,
+            "stored_bytes_per_launch": float(sto.mean()), "stored_bytes": algorithmic_bytes(blocks, npts, neq, fmt),
+            "achieved_stored": float(sto.sum() / t / 1e9)}
 
 
 def launch_times(ctx, u, r, nrep):
@@ -189,139 +209,218 @@ def event_ms(fn, stream, reps=10):
     return a.elapsed_time(b) / reps
 
 
-def measure_learned(ctx, u, r, npts, stream, dev, fused_ms):
-    """Trains the branch predictor on pairs from a batch of small instances,
-    then times, at the bench state, the reference's unfused steps with and
-    without branch sorting (ComputeSigma on points grouped by predicted
-    branch); the results of both are identical."""
-    import numpy as np
+
+
+def profile_figures(workload, kname):
+    """HBM bytes per launch (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE, calibrated
+    on a copy kernel of known size) and VALU issue fraction of kernel `kname`
+    for `workload`, from the committed profile summary (profiles/); (None,
+    None) when the profile does not cover this kernel."""
+    tfile = os.path.join(ROOT, "profiles", f"k_residual_{workload}_traffic.json")
+    if not os.path.exists(tfile):
+        return None, None
+    with open(tfile) as f:
+        tk = json.load(f)["kernels"]
+    if kname not in tk:
+        return None, None
+    k = tk[kname]
+    return (int(k["total_bytes"]) if "total_bytes" in k else None), k.get("valu_busy")
+
+
+def roofline_entry(roof, kname, traffic, valu_busy, npts):
+    """The line's roofline object: compulsory bytes per launch over the
+    launch's average duration, against HBM; the measured HBM traffic per
+    launch (rocprofv3 PMC) and its ratio to the compulsory bytes; the VALU
+    issue fraction as the second bound."""
+    alg = roof["stored_bytes_per_launch"]
+    return {"bound": "hbm", "achieved": round(roof["achieved_stored"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
+            "algorithmic_bytes_per_launch": int(alg),
+            "algorithmic_bytes_per_point": round(roof["stored_bytes"] / npts, 1),
+            "avg_launch_ms": round(roof["avg_launch_ms"], 5),
+            "avg_launch_ms_isolated": round(roof["avg_launch_ms_isolated"], 5),
+            "dispatch_gap_ms": round(roof["dispatch_gap_ms"], 5),
+            "traffic_over_algorithmic": None if traffic is None else round(traffic / alg, 3),
+            "measured_traffic_rate": None if traffic is None else round(traffic / (roof["avg_launch_ms"] / 1e3) / 1e9, 1),
+            # 4 cycles x VALU instructions / (1024 SIMDs x 2.4 GHz x duration), rocprofv3 PMC (profiles/)
+            "valu_busy": None if valu_busy is None else round(valu_busy, 3),
+            "second_bound": None if valu_busy is None else {"bound": "valu", "frac": round(valu_busy, 3)}}
+
+
+def residual_paths(ctx, u, r, stream, steps, warmup, npts, r_ref, fint, assemblies=("colored", "patch")):
+    """The non-committing residual in every (assembly, stress mode) the
+    library offers, at the same state and u: time per residual, rate, and
+    deviation from the reference-order exact residual r_ref."""
+    out = {}
+    for asm in assemblies:
+        try:
+            ctx.set_residual_assembly(asm)
+        except Exception as e:  # the patch assembly needs the geometric setup
+            out[asm] = {"unavailable": str(e)}
+            continue
+        for mode in ("exact", "certified-elastic"):
+            ctx.set_residual_mode(mode)
+            tms = time_residuals(ctx, u, r, stream, steps, warmup)
+            ctx.residual(u, r)
+            ctx.synchronize()
+            dev = float(np.abs(r.cpu().numpy() - r_ref).max()) / fint
+            out[f"{asm}/{mode}"] = {"ms_per_step": round(tms / steps, 5),
+                                    "value": round(npts * steps / (tms / 1e3) / 1e6, 2),
+                                    "max_dev_vs_exact_colored": dev}
+            log(f"[bench] residual {asm}/{mode}: {tms / steps:.4f} ms, {out[f'{asm}/{mode}']['value']:.1f} "
+                f"Mint-pts/s, deviation {dev:.2e} of max|f_int|")
+    ctx.set_residual_assembly("colored")
+    ctx.set_residual_mode("exact")
+    return out
+
+
+def train_predictor(ctx, u, npts, dev, seed=0, steps=3000):
+    """The branch predictor trained on the exact return map's pairs at this
+    state (a seeded 90 / 10 split), on the GPU.  Returns (mlp, features,
+    stats) with the held-out accuracy and the inference time over all points."""
     import torch
 
     from intpointsfem_amd import learned as L
     t0 = time.time()
-    f_tr, b_tr, t_tr = L.training_set(n_instances=24, seed=1)
-    mlp = L.new_mlp(seed=0)
-    hist = L.train(mlp, f_tr, b_tr, t_tr, steps=3000, batch=4096, lr=3e-3, log_every=1000)
+    feat, br, tg = L.point_features(ctx, u, npts, dev)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    held = (torch.rand(npts, generator=g) < 0.1).to(dev)
+    tr = torch.nonzero(~held).squeeze(1)
+    te = torch.nonzero(held).squeeze(1)
+    mlp = L.new_mlp(seed=seed)
+    L.train(mlp, feat.index_select(0, tr), br.index_select(0, tr), tg.index_select(0, tr), steps=steps,
+            batch=4096, lr=3e-3, seed=seed)
+    torch.cuda.synchronize()
     train_s = time.time() - t0
-    del f_tr, b_tr, t_tr
-    feat, br_exact, tg = L.point_features(ctx, u, npts, dev)
-    strain = torch.zeros(3 * npts, dtype=torch.float64, device=dev)
-    ws = torch.zeros(3 * npts, dtype=torch.float64, device=dev)
     pred = torch.zeros(npts, dtype=torch.int32, device=dev)
-    order = torch.zeros(npts, dtype=torch.int32, device=dev)
-    ctx.set_update_mem(False)
-    t = {
-        "multiply_ms": event_ms(lambda: ctx.multiply(u, strain), stream),
-        "features_ms": event_ms(lambda: ctx.point_features(u, feat, br_exact, tg), stream),
-        "predict_ms": event_ms(lambda: mlp.predict(feat, branch=pred, n=npts), stream),
-        "sort_ms": event_ms(lambda: ctx.branch_order(pred, order), stream),
-        "compute_sigma_ms": event_ms(lambda: ctx.compute_sigma(strain, ws), stream),
-        "compute_sigma_sorted_ms": event_ms(lambda: ctx.compute_sigma_ordered(strain, ws, order), stream),
-        "multiply_transpose_ms": event_ms(lambda: ctx.multiply_transpose(ws, r), stream),
-    }
-    acc = float((pred == br_exact).float().mean().item())
-    be = br_exact.cpu().numpy()
-    div_nat = L.divergence(be, 63)
-    div_sorted = L.divergence(be[order.cpu().numpy()], 64)
-    plain = t["multiply_ms"] + t["compute_sigma_ms"] + t["multiply_transpose_ms"]
-    learned = (t["multiply_ms"] + t["features_ms"] + t["predict_ms"] + t["sort_ms"] + t["compute_sigma_sorted_ms"]
-               + t["multiply_transpose_ms"])
-    log(f"[bench] learned: trained in {train_s:.1f}s (loss {hist[0][1]:.3f} -> {hist[-1][1]:.4f}), branch accuracy "
-        f"{acc:.4f}; mixed-branch groups {div_nat:.3f} -> {div_sorted:.4f}; ComputeSigma {t['compute_sigma_ms']:.3f} "
-        f"-> sorted {t['compute_sigma_sorted_ms']:.3f} ms; unfused residual plain {plain:.3f} ms, learned "
-        f"{learned:.3f} ms, fused {fused_ms:.3f} ms")
-    return dict({k: round(v, 4) for k, v in t.items()}, train_seconds=round(train_s, 1),
-                branch_accuracy=round(acc, 5), mixed_branch_groups_natural=round(div_nat, 4),
-                mixed_branch_groups_sorted=round(div_sorted, 5), unfused_residual_plain_ms=round(plain, 4),
-                unfused_residual_learned_ms=round(learned, 4), fused_residual_ms=round(fused_ms, 4),
-                learned_value=round(npts / learned / 1e3, 2))
+    stream = torch.cuda.current_stream(dev)
+    pms = event_ms(lambda: mlp.predict(feat, branch=pred, n=npts), stream)
+    acc = float((pred.index_select(0, te) == br.index_select(0, te)).float().mean())
+    major = float(torch.bincount(br.index_select(0, te).long(), minlength=5).max()) / max(1, te.shape[0])
+    stats = {"pairs": npts, "train_seconds": round(train_s, 1), "held_out_accuracy": round(acc, 4),
+             "majority_class": round(major, 4), "predict_ms": round(pms, 4),
+             "predict_ms_per_5.76M_points": round(pms * 5.76e6 / npts, 4),
+             "predict_rate_Gpts_s": round(npts / pms / 1e6, 2)}
+    log(f"[bench] predictor: {train_s:.1f} s to train on {npts} pairs, held-out accuracy {acc:.4f} (majority "
+        f"{major:.4f}); inference {pms:.4f} ms for {npts} points")
+    return mlp, pred, stats
 
 
-def run_batch(args):
-    """The 64 uneven wellbore instances (tests/golden/batch64_instances.json,
-    drawn with seed 0) in one context: one launch per (color, shape) for the
-    whole batch.  Same JSON line; cpu_baseline times the oracle on whole
-    instances, in table order, until --cpu-seconds."""
-    import numpy as np
+def element_keys(pred, row_sizes):
+    """Per element: 1 when the predictor puts any of its points off the
+    elastic branch (the element then cannot join a certified-elastic tile)."""
+    nip = (np.asarray(row_sizes) // 3).astype(np.int64)
+    p = pred.cpu().numpy()
+    first = np.concatenate([[0], np.cumsum(nip)[:-1]])
+    return (np.maximum.reduceat((p != 0).astype(np.int32), first) > 0).astype(np.int32)
+
+
+def learned_mixed_batched(args, dev):
+    """The learned + mixed-precision batched residual on the 64 uneven
+    instances (tests/golden/batch64_instances.json, drawn with seed 0):
+
+      * state: one load step to each instance's final wellbore pressure from
+        rest, solved by the segmented Newton with FP32 tangent products in the
+        CG and FP64 residuals (iterative refinement) to ||r_k|| / ||r0_k|| <=
+        newton_rtol per instance, then committed;
+      * plain: the FP64 residual of the drop-in (colored assembly, exact
+        return map), natural element order;
+      * learned + mixed: the predictor (trained on this state's pairs) orders
+        each color's / patch's elements so that predicted-elastic elements
+        share tiles, and the certified-elastic path skips the spectral
+        decomposition of every point it proves elastic; the assembly
+        (colored or patch) is the faster one at this state;
+      * deviation from the plain residual, and from the CPU oracle on whole
+        instances within --cpu-seconds.
+    Returns (dict for the JSON line, batch, u, plain_ms)."""
     import torch
 
-    import intpointsfem_amd as ipf
     from intpointsfem_amd import batch
-    from intpointsfem_amd import wellbore as wb
-
-    dev = torch.device("cuda:0")
-    t0 = time.time()
     table = batch.draw_instances(64, seed=0)
+    t0 = time.time()
     meshes = [batch.instance_mesh(i) for i in table]
     B = batch.Batch(meshes, device=0)
     ctx = B.ctx
     npts, neq = B.npts, B.neq
-    log(f"[bench] batch of {B.n} instances: {len(B.row_sizes)} elements, {npts} points, neq {neq}; setup + upload "
-        f"{time.time() - t0:.1f}s, {ctx.info()['device_bytes'] / 1e9:.2f} GB on device")
     stream = torch.cuda.current_stream(dev)
     ctx.set_stream(stream.cuda_stream)
-    us = [wb.lame_displacement(m.nodes(), wb.normal_stress(wb.NSTEPS, wb.NSTEPS, inst["pw_final"], wb.SIGMA0),
-                               E_=inst["E"], nu=inst["nu"]) for inst, m in zip(table, meshes)]
-    u_host = np.concatenate(us)
-    u = torch.as_tensor(u_host, dtype=torch.float64, device=dev)
-    r = torch.zeros(neq, dtype=torch.float64, device=dev)
-    ctx.set_update_mem(True)
-    ctx.residual(u, r)
-    ctx.set_update_mem(False)
-    ctx.synchronize()
+    B.set_dirichlet_mask(B.dirichlet_mask())
+    log(f"[bench] batch64: {len(B.row_sizes)} elements, {npts} points, neq {neq}; setup {time.time() - t0:.1f}s")
+    fmt0 = ctx.b_format()
+    tfmt = args.batch_tangent_format
+    ctx.set_b_format(fmt0[0], tangent=tfmt)
+    u = torch.zeros(neq, dtype=torch.float64, device=dev)
+    t1 = time.time()
+    st = B.newton(u, rtol=args.newton_rtol, maxit=80, cg_rtol=args.cg_rtol, cg_maxit=200000)
+    solve_s = time.time() - t1
+    ctx.set_b_format(*fmt0)
+    newton = {"tangent_format": tfmt, "iterations": int(st["iterations"]), "converged": bool(st["converged"] == 1.0),
+              "max_rel_residual": st["rel_residual"], "cg_iterations": int(st["cg_iterations"]),
+              "solve_seconds": round(solve_s, 1), "load": "one step from rest to each instance's final pressure"}
+    log(f"[bench] batch64 newton ({tfmt} tangent): {newton['iterations']} iterations, {newton['cg_iterations']} CG, "
+        f"worst ||r_k||/||r0_k|| {st['rel_residual']:.2e}, {solve_s:.1f} s")
     _, _, mt, br = ctx.get_state()
-    ctx.set_state(np.zeros(6 * npts))
-    branch_counts = {ipf.BRANCH_NAMES[i]: int((br == i).sum()) for i in range(5)}
-    log(f"[bench] branches: {branch_counts}")
-    tune = None
-    if not args.no_autotune:
-        tune = ctx.autotune(u, reps=5)
-        log(f"[bench] autotune: {', '.join(f'{k} {v:.4f} ms' for k, v in tune['residual'].items())} -> "
-            f"{tune['selected'][0]}")
-        tune = {"residual_ms": {k: round(v, 5) for k, v in tune["residual"].items()}, "selected": tune["selected"][0]}
-    fmt0 = ctx.b_format()[0]
-    nsettle = settle(ctx, u, r, args.settle_ms)
-    total_ms = time_residuals(ctx, u, r, stream, args.steps, args.warmup, roi="timed residuals")
-    ms_per_step = total_ms / args.steps
-    value = npts * args.steps / (total_ms / 1e3) / 1e6
-    log(f"[bench] {args.steps} batched residuals ({fmt0} B): {ms_per_step:.4f} ms, {value:.1f} Mint-pts/s")
-    blocks = {"row_sizes": B.row_sizes, "col_sizes": B.col_sizes}
-    nrep = max(3, min(args.steps, 20))
-    roof = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt0, step_ms=ms_per_step)
-    # the known-size k_copy that calibrates the PMC byte counters (see main)
-    ctx.residual_tangent(u, torch.empty_like(r))
+    r = torch.zeros(neq, dtype=torch.float64, device=dev)
+    settle(ctx, u, r, args.settle_ms)
+    plain_total = time_residuals(ctx, u, r, stream, args.steps, args.warmup)
+    plain_ms = plain_total / args.steps
+    ctx.residual(u, r)
     ctx.synchronize()
-    log(f"[bench] launches {', '.join(f'{x:.4f}' for x in roof['color_ms'])} ms; {roof['achieved']:.0f} GB/s on "
-        f"plain FP64 bytes, {roof['achieved_stored']:.0f} GB/s on stored bytes")
-    # HBM bytes per launch of each element order, from the committed rocprofv3
-    # profile of this workload (PMC FETCH_SIZE + WRITE_SIZE, calibrated on k_copy)
-    traffic, by_shape = None, None
-    tfile = os.path.join(ROOT, "profiles", "k_residual_batch64_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            tk = json.load(f)["kernels"]
-        by_shape = {}
-        for sh, nm in enumerate(("Q1", "Q2", "Q3")):
-            kn = kernel_name(fmt0, sh)
-            if kn in tk and "total_bytes" in tk[kn]:
-                by_shape[nm] = {"traffic": int(tk[kn]["total_bytes"]), "valu_busy": tk[kn].get("valu_busy")}
-        if len(by_shape) == 3 and len(roof["color_ms"]) == 3 * B.ncolor:
-            # every color holds all three orders: the mean over the launches
-            traffic = int(sum(v["traffic"] for v in by_shape.values()) / 3)
-    roofline = {"bound": "hbm", "achieved": round(roof["achieved_stored"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_residual_fmt, one launch per (color, element order) over all instances",
-                "algorithmic_bytes_per_point": round(roof["stored_bytes"] / npts, 1),
-                "avg_launch_ms": round(roof["avg_launch_ms"], 5),
-                "measured_traffic_rate": None if traffic is None else round(traffic / (roof["avg_launch_ms"] / 1e3) / 1e9, 1),
-                "traffic_by_order": by_shape,
-                "dense_fp64_bytes_rate": round(roof["achieved"], 1)}
-    cpu = None
+    r_plain = r.cpu().numpy()
+    fint = float(np.abs(r_plain - B.rhs_linear).max())
+    ablation = {}
+
+    def timed(label):
+        tms = time_residuals(ctx, u, r, stream, args.steps, args.warmup) / args.steps
+        ctx.residual(u, r)
+        ctx.synchronize()
+        dev_ = float(np.abs(r.cpu().numpy() - r_plain).max()) / fint
+        ablation[label] = {"ms_per_step": round(tms, 5), "speedup_vs_plain": round(plain_ms / tms, 3),
+                           "max_dev_vs_plain": dev_}
+        log(f"[bench] batch64 {label}: {tms:.4f} ms ({plain_ms / tms:.3f}x plain), deviation {dev_:.2e}")
+        return tms
+
+    for asm in ("colored", "patch"):
+        ctx.set_residual_assembly(asm)
+        ctx.set_residual_mode("exact")
+        if asm == "patch":
+            timed("patch/exact, index order")
+        ctx.set_residual_mode("certified-elastic")
+        timed(f"{asm}/certified-elastic, index order")
+    ctx.set_residual_assembly("colored")
+    ctx.set_residual_mode("exact")
+    # the predictor: trained on this state's pairs, then the element order
+    mlp, pred, pstats = train_predictor(ctx, u, npts, dev)
+    keys = element_keys(pred, B.row_sizes)
+    t2 = time.time()
+    B.relayout(keys)
+    ctx.set_stream(stream.cuda_stream)
+    relayout_s = time.time() - t2
+    best = None
+    for asm in ("colored", "patch"):
+        ctx.set_residual_assembly(asm)
+        ctx.set_residual_mode("certified-elastic")
+        tms = timed(f"{asm}/certified-elastic, predicted order")
+        if best is None or tms < best[1]:
+            best = (asm, tms)
+    ctx.set_residual_assembly(best[0])
+    ctx.set_residual_mode("certified-elastic")
+    settle(ctx, u, r, args.settle_ms)
+    lm_total = time_residuals(ctx, u, r, stream, args.steps, args.warmup, roi="timed learned+mixed residuals")
+    lm_ms = lm_total / args.steps
+    ctx.residual(u, r)
+    ctx.synchronize()
+    r_lm = r.cpu().numpy()
+    dev_plain = float(np.abs(r_lm - r_plain).max()) / fint
+    # the CPU oracle on whole instances (in table order) within the time bound
+    oracle_dev, checked = None, 0
     if not args.no_cpu_baseline:
+        import intpointsfem_amd as ipf
         from oracle import oracle as O
-        nthreads = max(1, min(16, os.cpu_count() or 1))
-        done_pts, spent, ninst = 0, 0.0, 0
-        for inst, m, ui in zip(table, meshes, us):
+        ep, al, _, _ = ctx.get_state()
+        uh = u.cpu().numpy()
+        spent, worst = 0.0, 0.0
+        for i, m in enumerate(meshes):
             sm = ipf.StructMatrix(m, device=-1)
             b = sm.export_blocks()
             ix = sm.export_indexes()
@@ -329,28 +428,40 @@ def run_batch(args):
             Bo = O.Blocks(b["row_sizes"], b["col_sizes"], b["matrix_position"], b["row_first"], b["col_first"],
                           b["storage"])
             d = m.material()
+            e0, e1 = B.eq_offsets[i], B.eq_offsets[i + 1]
+            p0, p1 = B.pt_offsets[i], B.pt_offsets[i + 1]
             t = time.time()
-            O.residual(Bo, ix["dof_indexes"], ix["color_indexes"], s["ncolor"], s["neq"], ix["weight"], d["lambda"],
-                       d["mu"], d["phi"], d["psi"], d["cohesion"], ui, f_lin=ix["rhs_linear"], nthreads=nthreads)
+            ro, *_ = O.residual(Bo, ix["dof_indexes"], ix["color_indexes"], s["ncolor"], s["neq"], ix["weight"],
+                                d["lambda"], d["mu"], d["phi"], d["psi"], d["cohesion"], uh[e0:e1],
+                                f_lin=ix["rhs_linear"], plastic_strain6=ep[6 * p0:6 * p1], alpha=al[p0:p1],
+                                nthreads=max(1, min(16, os.cpu_count() or 1)))
             spent += time.time() - t
-            done_pts += s["npts"]
-            ninst += 1
+            worst = max(worst, float(np.abs(r_lm[e0:e1] - ro).max() / np.abs(ro - ix["rhs_linear"]).max()))
+            checked += 1
             if spent >= args.cpu_seconds:
                 break
-        cpu = {"value": round(done_pts / spent / 1e6, 3), "unit": "Mint-pts/s", "cores": nthreads, "kind": "port",
-               "sample": f"one residual of each of the first {ninst} instances ({done_pts} points) by the C oracle "
-                         f"with {nthreads} OpenMP threads, {spent:.1f} s"}
-        log(f"[bench] cpu oracle: {cpu['value']:.2f} Mint-pts/s ({ninst} instances)")
-    out = {"metric": METRIC, "value": round(value, 2), "unit": "Mint-pts/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "plain FP64 batched residual, 64 uneven wellbore instances (seed-0 table)",
-                      "b_format": fmt0, "autotune": tune, "settle": {"ms": args.settle_ms, "residuals": nsettle},
-                      "instances": B.n, "elements": int(len(B.row_sizes)), "int_points": npts,
-                      "neq": neq, "colors": B.ncolor, "branches": branch_counts,
-                      "state": "u = each instance's linear-elastic thick-cylinder displacement, zero plastic strain"},
-           "roofline": roofline, "cpu_baseline": cpu}
-    print(json.dumps(out), flush=True)
+        oracle_dev = worst
+        log(f"[bench] batch64 learned+mixed vs the CPU oracle on {checked} instances: {worst:.2e} of max|f_int|")
+    ctx.set_residual_assembly("colored")
+    ctx.set_residual_mode("exact")
+    value = npts * args.steps / (lm_total / 1e3) / 1e6
+    log(f"[bench] batch64 learned+mixed ({best[0]} assembly): {lm_ms:.4f} ms, {value:.1f} Mint-pts/s, "
+        f"{plain_ms / lm_ms:.3f}x plain ({plain_ms:.4f} ms)")
+    out = {"workload": "64 uneven wellbore instances (seed-0 table) at the converged mixed-precision Newton state",
+           "value": round(value, 2), "unit": "Mint-pts/s", "ms_per_step": round(lm_ms, 5),
+           "speedup_vs_plain": round(plain_ms / lm_ms, 3),
+           "plain": {"ms_per_step": round(plain_ms, 5), "value": round(npts * args.steps / (plain_total / 1e3) / 1e6, 2),
+                     "path": "colored assembly, exact return map, index order (the drop-in's residual)"},
+           "path": {"assembly": best[0], "residual_mode": "certified-elastic", "order": "predicted branch",
+                    "state_solve": "FP32 tangent products in the CG, FP64 residuals"},
+           "max_dev_vs_plain": dev_plain, "max_dev_vs_oracle": oracle_dev, "oracle_instances_checked": checked,
+           "int_points": npts, "neq": neq, "newton": newton,
+           "branches": {n: int((br == i).sum()) for i, n in enumerate(("elastic", "main", "right_edge", "left_edge",
+                                                                       "apex"))},
+           "predictor": dict(pstats, elements_keyed_plastic=int(keys.sum()), elements=int(len(keys)),
+                             relayout_seconds=round(relayout_s, 1)),
+           "ablation": ablation, "steps": args.steps, "warmup": args.warmup}
+    return out, B, u, plain_ms
 
 
 def main():
@@ -361,10 +472,10 @@ def main():
     ap.add_argument("--settle-ms", type=float, default=200.0,
                     help="untimed back-to-back residuals before the warm-up steps, in ms of wall time")
     ap.add_argument("--mesh", default="800x800-q2",
-                    help="800x800-q2 (default), 800x800-q3, 40x40-q2, or batch64 (64 uneven instances)")
+                    help="800x800-q2 (default), 800x800-q3, 40x40-q2, or batch64 (the 64 uneven instances only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-formats", action="store_true", help="skip the compact / FP32 B format variants")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned branch-predictor measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the learned + mixed batched measurement")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the library's default B format instead of the autotuner's choice")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's timed work")
@@ -384,21 +495,32 @@ def main():
     ap.add_argument("--solve-tangent-format", default=None,
                     help="B format of the CG matvecs of the --state newton solve (default: the library's, "
                          "geometric FP64; the residual stays FP64 whatever this is)")
+    ap.add_argument("--batch-tangent-format", default="compact-fp32",
+                    help="B format of the CG matvecs of the batch's Newton solve (FP32: mixed precision)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the path runs on a single "
               "MI355X; multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
-
-    if args.mesh == "batch64":
-        return run_batch(args)
-    import numpy as np
     import torch
 
     import intpointsfem_amd as ipf
     from intpointsfem_amd import wellbore as wb
 
     dev = torch.device("cuda:0")
+    if args.mesh == "batch64":
+        lm, B, u, plain_ms = learned_mixed_batched(args, dev)
+        out = {"metric": METRIC, "value": lm["plain"]["value"], "unit": "Mint-pts/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": lm["plain"]["ms_per_step"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+               "data": "synthetic",
+               "config": {"workload": "plain FP64 batched residual, 64 uneven wellbore instances (seed-0 table), "
+                                      "converged mixed-precision Newton state", "int_points": B.npts,
+                          "neq": B.neq, "instances": B.n},
+               "roofline": None, "cpu_baseline": None, "learned_mixed_batched": lm}
+        print(json.dumps(out), flush=True)
+        return
+
     t0 = time.time()
     mesh = wb.make_mesh(args.mesh)
     info = mesh.info()
@@ -415,10 +537,8 @@ def main():
     r = torch.zeros(neq, dtype=torch.float64, device=dev)
     newton_info = None
     if args.state == "lame":
-        # displacement: linear-elastic thick-cylinder field of the final load step
         u_host = wb.lame_displacement(mesh.nodes(), wb.normal_stress(wb.NSTEPS), scale=1.0)
         u = torch.as_tensor(u_host, dtype=torch.float64, device=dev)
-        # branch mix of this state (one committed evaluation, then reset the state)
         ctx.set_update_mem(True)
         ctx.residual(u, r)
         ctx.set_update_mem(False)
@@ -443,7 +563,6 @@ def main():
         # 10 load steps of Newton + block-Jacobi PCG on the GPU; the state is committed after each
         ctx.set_dirichlet_mask(mesh.dirichlet_mask())
         fmt_default = ctx.b_format()
-        # FP64 residual; tangent products in the chosen format (FP32: mixed Newton)
         solve_tf = args.solve_tangent_format or fmt_default[1]
         ctx.set_b_format(fmt_default[0], tangent=solve_tf)
         f_unit = sm.export_indexes()["rhs_linear"] / wb.normal_stress(wb.NSTEPS)
@@ -459,7 +578,8 @@ def main():
         ctx.set_b_format(*fmt_default)
         u_host = u.cpu().numpy()
         _, _, mt, br = ctx.get_state()
-        newton_info = {"solve_tangent_format": solve_tf, "load_steps": wb.NSTEPS, "newton_iterations": [int(s["iterations"]) for s in steps],
+        newton_info = {"solve_tangent_format": solve_tf, "load_steps": wb.NSTEPS,
+                       "newton_iterations": [int(s["iterations"]) for s in steps],
                        "final_rel_residual": steps[-1]["rel_residual"],
                        "converged": all(s["converged"] == 1.0 for s in steps),
                        "cg_iterations": int(sum(s["cg_iterations"] for s in steps)),
@@ -493,49 +613,29 @@ def main():
     # per-launch durations (HIP events on the launch stream) for the roofline
     nrep = max(3, min(args.steps, 20))
     blocks = sm.export_blocks()
-    nbytes = algorithmic_bytes(blocks, npts, neq)  # plain FP64 byte count: dense B
     roof = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt0, step_ms=ms_per_step)
     # one untimed ipf_residual_tangent: its k_copy (f_lin into r, neq doubles
     # read and written) is the known-size dispatch that calibrates the PMC
     # byte counters of a profiled run (the residual itself needs no copy)
     ctx.residual_tangent(u, torch.empty_like(r))
     ctx.synchronize()
-    log(f"[bench] copy kernel {roof['copy_ms']:.4f} ms; color kernels "
-        f"{', '.join(f'{x:.4f}' for x in roof['color_ms'])} ms; {nbytes / npts:.1f} algorithmic B/pt -> "
-        f"{roof['achieved']:.0f} GB/s; {roof['stored_bytes'] / npts:.1f} stored B/pt -> {roof['achieved_stored']:.0f} GB/s")
-    # HBM bytes per launch measured by rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE calibrated on
-    # k_copy) for this workload, committed with the profile it comes from
     kname = kernel_name(fmt0, shape)
-    traffic = None
-    valu_busy = None
-    tfile = os.path.join(ROOT, "profiles", f"k_residual_{args.mesh}_traffic.json")
-    # bytes per launch hardly depend on the state (same loads and stores at every point)
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            tk = json.load(f)["kernels"]
-        if kname in tk:
-            traffic = int(tk[kname]["total_bytes"]) if "total_bytes" in tk[kname] else None
-            valu_busy = tk[kname].get("valu_busy")
-    roofline = {"bound": "hbm", "achieved": round(roof["achieved_stored"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(roof["achieved_stored"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"{kname} (one launch per element color)",
-                "algorithmic_bytes_per_launch": int(roof["stored_bytes_per_launch"]),
-                "algorithmic_bytes_per_point": round(roof["stored_bytes"] / npts, 1),
-                "avg_launch_ms": round(roof["avg_launch_ms"], 5),
-                "avg_launch_ms_isolated": round(roof["avg_launch_ms_isolated"], 5),
-                "measured_traffic_rate": None if traffic is None else round(traffic / (roof["avg_launch_ms"] / 1e3) / 1e9, 1),
-                # 4 cycles x VALU instructions / (1024 SIMDs x 2.4 GHz x duration), rocprofv3 PMC (profiles/)
-                "valu_busy": None if valu_busy is None else round(valu_busy, 3),
-                "dense_fp64_bytes_rate": round(roof["achieved"], 1)}
+    traffic, valu_busy = profile_figures(args.mesh, kname)
+    roofline = roofline_entry(roof, f"{kname} (one launch per element color)", traffic, valu_busy, npts)
+    log(f"[bench] copy interval {roof['copy_ms']:.4f} ms; color kernels "
+        f"{', '.join(f'{x:.4f}' for x in roof['color_ms'])} ms (dispatch gap {roof['dispatch_gap_ms']:.4f} ms "
+        f"removed); {roof['stored_bytes'] / npts:.1f} compulsory B/pt -> {roof['achieved_stored']:.0f} GB/s")
+
+    ctx.residual(u, r)
+    ctx.synchronize()
+    r64 = r.cpu().numpy()
+    fint = float(np.abs(r64 - sm.export_indexes()["rhs_linear"]).max())
+    paths = residual_paths(ctx, u, r, stream, args.steps, args.warmup, npts, r64, fint)
 
     # the other B formats (same state, same u): timing, roofline on their own
     # stored bytes, and the deviation from the FP64 residual
     variants = {}
     if not args.no_formats:
-        ctx.residual(u, r)
-        ctx.synchronize()
-        r64 = r.cpu().numpy()
-        fint = float(np.abs(r64 - sm.export_indexes()["rhs_linear"]).max())
         for fmt in ipf.B_FORMATS:
             if fmt == fmt0:
                 continue
@@ -557,8 +657,6 @@ def main():
                 f"{fr['achieved_stored']:.0f} GB/s on its stored bytes, "
                 f"|r - r_fp64| / max|f_int| = {err:.2e}")
         ctx.set_b_format(fmt0)
-
-    learned_info = None if args.no_learned else measure_learned(ctx, u, r, npts, stream, dev, ms_per_step)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -593,6 +691,13 @@ def main():
                                  "seconds": round(el1, 1)}}
         log(f"[bench] cpu oracle: {cpu_value:.2f} Mint-pts/s on {nthreads} threads ({len(rates)} reps), "
             f"{cpu['single_thread']['value']:.2f} on 1 thread ({len(rates1)} reps)")
+        del B
+
+    lm = None
+    if not args.no_batch:
+        del sm, ctx
+        torch.cuda.synchronize()
+        lm, _, _, _ = learned_mixed_batched(args, dev)
 
     out = {
         "metric": METRIC,
@@ -614,8 +719,9 @@ def main():
                    "state": state_desc, "branches": branch_counts, "newton": newton_info},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "residual_paths": paths,
         "b_format_variants": variants,
-        "learned": learned_info,
+        "learned_mixed_batched": lm,
     }
     print(json.dumps(out), flush=True)
 

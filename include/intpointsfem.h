@@ -129,12 +129,14 @@ int ipf_set_material_table(ipf_ctx *ctx, int nmat, const double *params);
 int ipf_set_element_materials(ipf_ctx *ctx, int64_t nblocks, const int *element_material);
 /* Order of the elements inside each (color, shape, material) run of the next
  * ipf_transfer_data / ipf_transfer_geometry: ascending keys [nblocks], then
- * element index (default: all keys 0, index order).  The order decides which
- * elements share a wavefront tile -- e.g. predicted-elastic elements
- * together, so that whole tiles take the certified elastic path -- and
- * nothing else: within a color every DoF has one element, so residuals,
- * tangent products and states are bit-identical for any keys.  No reference
- * counterpart. */
+ * element index (default: all keys 0, index order); with the patch assembly,
+ * the order inside each patch (patches themselves follow the geometry).  The
+ * order decides which elements share a wavefront tile -- e.g.
+ * predicted-elastic elements together, so that whole tiles take the
+ * certified elastic path -- and nothing else: within a color every DoF has
+ * one element, so colored residuals, tangent products and states are
+ * bit-identical for any keys (patch residuals: up to the order of each
+ * node's sum).  No reference counterpart. */
 int ipf_set_element_order_keys(ipf_ctx *ctx, int64_t nblocks, const int *keys);
 
 /* fRhsLinear / dRhsLinear (TPZElastoPlasticIntPointsStructMatrix.cpp:63-68,156-167), [neq] host array. */
@@ -241,7 +243,9 @@ int ipf_certify_elastic_host(int64_t n, const double *strain3, const double *pla
  * context's stream; waits, then writes the duration of each launch in
  * milliseconds to launch_ms (launch 0: r = f_lin copy, then one per element
  * group in launch order) and the group's point count to launch_pts (may be
- * NULL).  Returns the number of launches (>= 0) or a negative IPF_E* code.
+ * NULL).  With the patch assembly (non-committing): launch 0 the refresh of
+ * the patch copy of the plastic strain (empty when current), one launch per
+ * element shape, then the shared nodes' fix-up (0 points).  Returns the number of launches (>= 0) or a negative IPF_E* code.
  * Measurement aid; the plain entry point records no events. */
 int ipf_residual_timed(ipf_ctx *ctx, const double *u_dev, double *r_dev, double *launch_ms, int64_t *launch_pts,
                        int max_launches);
@@ -269,9 +273,12 @@ enum {
     IPF_B_COMPACT_FP64 = 1, /* (dN/dx, dN/dy) per node and point; same results */
     IPF_B_DENSE_FP32 = 2,   /* FP32 products with FP32 accumulation */
     IPF_B_COMPACT_FP32 = 3,
-    IPF_B_GEOMETRY_FP64 = 4 /* nothing stored per point: the compact FP64 values and
+    IPF_B_GEOMETRY_FP64 = 4, /* nothing stored per point: the compact FP64 values and
                                the weight recomputed in the kernel from the element's
                                corners (after ipf_transfer_geometry); compact FP64 results */
+    IPF_B_GEOMETRY_FP32 = 5  /* the geometric values rounded to FP32 (= compact FP32),
+                               FP32 products and accumulation; as the tangent format,
+                               w Dep is read from an FP32 copy too (the CG matvec in FP32) */
 };
 
 /* Selects the B representation of the fused residual (ipf_residual,
@@ -297,7 +304,7 @@ int ipf_set_b_format(ipf_ctx *ctx, int residual_format, int tangent_format);
  * (with allow_fp32, also the FP32 forms: the CG matvecs then steer an FP64
  * Newton, see above), and selects the fastest of each.  The FP64 forms give
  * bit-identical results, so the choice changes only the time.  times_ms
- * (optional) [2 * 5]: average ms per residual by format, then per tangent
+ * (optional) [2 * 6]: average ms per residual by format, then per tangent
  * product by format, -1 where not timed (format codes as above).  Frees
  * stored forms neither selection uses.  No reference counterpart. */
 int ipf_autotune(ipf_ctx *ctx, const double *u_dev, int reps, int allow_fp32, double *times_ms);
@@ -334,7 +341,6 @@ int ipf_point_features(ipf_ctx *ctx, const double *u_dev, float *feat_dev, int *
  * then sees points of one predicted branch) -- the same results. */
 int ipf_branch_order(ipf_ctx *ctx, const int *branch_dev, int *order_dev);
 int ipf_compute_sigma_ordered(ipf_ctx *ctx, const double *strain_dev, double *wsigma_dev, const int *order_dev);
-
 
 typedef struct ipf_mlp ipf_mlp;
 int ipf_mlp_create(int device, uint64_t seed, ipf_mlp **out);

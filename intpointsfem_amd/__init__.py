@@ -28,7 +28,7 @@ _LIB_PATH = os.path.join(_HERE, "libintpointsfem.so")
 
 IPF_OK = 0
 BRANCH_NAMES = ("elastic", "main", "right_edge", "left_edge", "apex")
-B_FORMATS = ("dense-fp64", "compact-fp64", "dense-fp32", "compact-fp32", "geometry-fp64")
+B_FORMATS = ("dense-fp64", "compact-fp64", "dense-fp32", "compact-fp32", "geometry-fp64", "geometry-fp32")
 RESIDUAL_MODES = ("exact", "certified-elastic")
 ASSEMBLIES = ("colored", "patch")
 

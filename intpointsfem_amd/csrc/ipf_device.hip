@@ -149,7 +149,7 @@ struct Group {
     int64_t b_off = 0;
     int64_t dof_off = 0;
     int64_t slot0 = 0;                         // first element slot of the group (geometry rows)
-    int64_t f_off[FMT_COUNT] = {0, 0, 0, 0, 0};  // first value of the group in each stored B format
+    int64_t f_off[FMT_COUNT] = {0, 0, 0, 0, 0, 0};  // first value of the group in each stored B format
     int64_t l_off = 0;                         // first CSR position index of the group (tangent pattern)
 };
 
@@ -260,7 +260,10 @@ struct ipf_ctx {
     bool have_mask = false;
 
     // alternative B formats (index FMT_*; FMT_D64 is dB, FMT_G64 stores nothing)
-    void* dBf[FMT_COUNT] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* dBf[FMT_COUNT] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // FP32 copy of w Dep for the FMT_G32 tangent product, current when dep32_ok
+    float* dDep32 = nullptr;
+    bool dep32_ok = false;
     // geometry of ipf_transfer_geometry: corners per element slot (8 doubles,
     // group slot order) and the shape tables (offset per shape, -1 if absent)
     double* dGeo = nullptr;
@@ -373,6 +376,8 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dU);
     dev_free(ctx->dR);
     dev_free(ctx->dDep);
+    dev_free(ctx->dDep32);
+    ctx->dep32_ok = false;
     dev_free(ctx->dStrain);
     dev_free(ctx->dSigw);
     dev_free(ctx->dMask);
@@ -485,7 +490,7 @@ PointState point_state(ipf_ctx* ctx) {
 }
 
 const void* group_fmt_B(const ipf_ctx* ctx, const Group& g, int fmt) {
-    if (fmt == FMT_G64) return ctx->dGeo + 8 * g.slot0;
+    if (format_geometric(fmt)) return ctx->dGeo + 8 * g.slot0;
     if (fmt == FMT_D64) return ctx->dB + g.b_off;
     return static_cast<const char*>(ctx->dBf[fmt]) + format_bytes(fmt) * (size_t)g.f_off[fmt];
 }
@@ -578,14 +583,19 @@ int ensure_patch(ipf_ctx* ctx) {
 // The non-committing residual over the patch layout: refresh the patch copy
 // of the plastic strain when the committed state changed, one launch per
 // element shape, then the shared nodes' fix-up.
-int patch_residual(ipf_ctx* ctx, const double* u, double* r) {
+// ev (optional, ranges + 3 events): recorded before the state refresh, after
+// it, after each shape's launch and after the fix-up.
+int patch_residual(ipf_ctx* ctx, const double* u, double* r, hipEvent_t* ev = nullptr) {
     int rc = ensure_patch(ctx);
     if (rc) return rc;
     const PatchHost& h = ctx->patch;
+    if (ev) IPF_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
     if (ctx->patch_state_version != ctx->state_version) {
         launch_patch_state(h.npts, ctx->dPptmap, ctx->dEp, ctx->dPep, ctx->stream);
         ctx->patch_state_version = ctx->state_version;
     }
+    if (ev) IPF_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+    int iev = 2;
     const bool fast = ctx->residual_mode == IPF_RESIDUAL_CERTIFIED_ELASTIC;
     for (const auto& R : h.ranges) {
         PatchDev p;
@@ -608,9 +618,11 @@ int patch_residual(ipf_ctx* ctx, const double* u, double* r) {
         (void)ET;
         (void)TP;
         launch_residual_patch(R.shape, fast, p, u, r, ctx->dFlin, ctx->dPpartial, ctx->stream);
+        if (ev) IPF_HIP(ctx, hipEventRecord(ev[iev++], ctx->stream));
     }
     launch_patch_fixup((int64_t)h.fix_dof.size(), ctx->dPfixDof, ctx->dPfixPtr, ctx->dPfixSlot, ctx->dPpartial,
                        (int64_t)h.orphan.size(), ctx->dPorphan, ctx->dFlin, r, ctx->stream);
+    if (ev) IPF_HIP(ctx, hipEventRecord(ev[iev], ctx->stream));
     return launch_status(ctx, "patch residual");
 }
 
@@ -633,8 +645,22 @@ void tangent_product(ipf_ctx* ctx, const double* v, double* y, hipStream_t s) {
         GroupDev d = group_dev(ctx, g);
         if (zf) d.dof = ctx->dDofR + g.dof_off;
         if (ctx->dNodeR) d.node = ctx->dNodeR + g.dof_off / 2;
-        launch_tangent_fmt(ctx->tfmt, g.shape, d, group_fmt_B(ctx, g, ctx->tfmt), v, y, ctx->dDep, ctx->npad, zf, s);
+        launch_tangent_fmt(ctx->tfmt, g.shape, d, group_fmt_B(ctx, g, ctx->tfmt), v, y, ctx->dDep, ctx->dDep32,
+                           ctx->npad, zf, s);
     }
+}
+
+// The FP32 copy of w Dep the FMT_G32 tangent product reads, made after each
+// new tangent (outside any captured graph).
+int ensure_dep32(ipf_ctx* ctx) {
+    if (ctx->tfmt != FMT_G32 || !ctx->have_tangent || ctx->dep32_ok) return IPF_OK;
+    if (!ctx->dDep32) {
+        int rc = dev_alloc(ctx, &ctx->dDep32, 9 * ctx->npad);
+        if (rc) return rc;
+    }
+    launch_to_f32(9 * ctx->npad, ctx->dDep, ctx->dDep32, ctx->stream);
+    ctx->dep32_ok = true;
+    return IPF_OK;
 }
 
 constexpr int kCgBlock = 10;  // CG iterations between convergence checks
@@ -705,7 +731,7 @@ int launch_status(ipf_ctx* ctx, const char* what) {
 // Makes format `fmt` resident (converted on the device from the dense FP64 tiles).
 int ensure_format(ipf_ctx* ctx, int fmt) {
     if (fmt < 0 || fmt >= FMT_COUNT) return fail(ctx, IPF_EINVAL, "ipf_set_b_format: unknown format %d", fmt);
-    if (fmt == FMT_G64)
+    if (format_geometric(fmt))
         return ctx->have_geometry
                    ? IPF_OK
                    : fail(ctx, IPF_EINVAL, "geometric B needs the element geometry (ipf_transfer_geometry)");
@@ -932,7 +958,7 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
             groups.push_back(std::move(g));
         }
     int64_t npad = 0, nb = 0, nd = 0;
-    int64_t nf[FMT_COUNT] = {0, 0, 0, 0};
+    int64_t nf[FMT_COUNT] = {0, 0, 0, 0, 0, 0};
     for (auto& g : groups) {
         const int TP = shape_tp(g.shape), CE = shape_ce(g.shape);
         g.ntiles = (int64_t)g.tmeta.size();
@@ -1083,6 +1109,7 @@ static int transfer_impl(ipf_ctx* ctx, int64_t nblocks, const int* row_sizes, co
                     const double* X = &hGeo[8 * (g.slot0 + k)];
                     E.cx = 0.25 * (X[0] + X[2] + X[4] + X[6]);
                     E.cy = 0.25 * (X[1] + X[3] + X[5] + X[7]);
+                    E.key = ekey[g.elems[k]];
                     for (int a = 0; a < CE / 2; a++) pin_nodes.push_back(hDof[g.dof_off + k * CE + 2 * a]);
                     pin.push_back(E);
                 }
@@ -1492,6 +1519,33 @@ int ipf_residual_timed(ipf_ctx* ctx, const double* u_dev, double* r_dev, double*
     int rc = check_ready(ctx, true);
     if (rc) return rc;
     if (!u_dev || !r_dev || !launch_ms) return fail(ctx, IPF_EINVAL, "ipf_residual_timed: NULL pointer");
+    if (ctx->assembly == IPF_ASSEMBLY_PATCH && !ctx->update_mem) {
+        IPF_HIP(ctx, hipSetDevice(ctx->device));
+        if ((rc = ensure_patch(ctx))) return rc;
+        const PatchHost& h = ctx->patch;
+        const int nr = (int)h.ranges.size(), np = nr + 2;
+        if (max_launches < np) return fail(ctx, IPF_EINVAL, "ipf_residual_timed: need room for %d launches", np);
+        std::vector<hipEvent_t> ev(np + 1, nullptr);
+        for (auto& e : ev) IPF_HIP(ctx, hipEventCreate(&e));
+        rc = patch_residual(ctx, u_dev, r_dev, ev.data());
+        if (rc == IPF_OK) {
+            IPF_HIP(ctx, hipEventSynchronize(ev[np]));
+            for (int i = 0; i < np; i++) {
+                float ms = 0.f;
+                IPF_HIP(ctx, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+                launch_ms[i] = ms;
+                if (launch_pts) {
+                    int64_t pts = 0;
+                    if (i >= 1 && i <= nr)
+                        for (const auto& E : ctx->patch_in)
+                            if (E.shape == h.ranges[i - 1].shape) pts += shape_nip(E.shape);
+                    launch_pts[i] = pts;
+                }
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        return rc == IPF_OK ? np : rc;
+    }
     const int n = 1 + (int)ctx->groups.size();
     if (max_launches < n) return fail(ctx, IPF_EINVAL, "ipf_residual_timed: need room for %d launches", n);
     IPF_HIP(ctx, hipSetDevice(ctx->device));
@@ -1643,6 +1697,7 @@ int ipf_set_b_format(ipf_ctx* ctx, int residual_format, int tangent_format) {
     ctx->rfmt = residual_format;
     ctx->tfmt = tangent_format;
     drop_cg_graph(ctx);
+    if ((rc = ensure_dep32(ctx))) return rc;
     return IPF_OK;
 }
 
@@ -1657,7 +1712,7 @@ int ipf_autotune(ipf_ctx* ctx, const double* u_dev, int reps, int allow_fp32, do
     // candidates: the exact FP64 forms the data allows; FP32 forms only for
     // the tangent product and only on request
     auto allowed = [&](int f) {
-        if (f == FMT_G64) return ctx->have_geometry;
+        if (format_geometric(f)) return ctx->have_geometry;
         if (format_compact(f)) return ctx->compact_ok;
         return true;
     };
@@ -1700,6 +1755,7 @@ int ipf_autotune(ipf_ctx* ctx, const double* u_dev, int reps, int allow_fp32, do
         }
         if (ctx->have_tangent && (!fp32 || allow_fp32)) {
             ctx->tfmt = f;
+            if ((rc = ensure_dep32(ctx))) break;
             rc = time_it(
                 [&] {
                     tangent_product(ctx, u_dev, ctx->dR, ctx->stream);
@@ -1767,6 +1823,8 @@ int ipf_residual_tangent(ipf_ctx* ctx, const double* u_dev, double* r_dev) {
     for (const auto& g : ctx->groups)
         launch_transpose(g.shape, group_dev(ctx, g), ctx->dSigw, r_dev, ctx->dOrig, ctx->stream);
     ctx->have_tangent = true;
+    ctx->dep32_ok = false;
+    if ((rc = ensure_dep32(ctx))) return rc;
     return launch_status(ctx, "residual_tangent");
 }
 
@@ -1865,6 +1923,7 @@ int ipf_cg_solve(ipf_ctx* ctx, const double* b_dev, double* x_dev, double rtol, 
     if (!(rtol > 0) || maxit < 1) return fail(ctx, IPF_EINVAL, "ipf_cg_solve: need rtol > 0 and maxit >= 1");
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_solver_buffers(ctx))) return rc;
+    if ((rc = ensure_dep32(ctx))) return rc;
     const int64_t n = ctx->neq;
     const int nb = dot_blocks(n);
     hipStream_t s = ctx->stream;

@@ -101,10 +101,14 @@ inline void dispatch_shape(int s, F&& f) {
 //            (8 doubles per element slot, tile order) and the shape table
 //            (ipf_geometry.h), bit-identical to the stored FP64 forms built
 //            from the same geometry
-enum BFormat { FMT_D64 = 0, FMT_C64 = 1, FMT_D32 = 2, FMT_C32 = 3, FMT_G64 = 4, FMT_COUNT = 5 };
-constexpr int format_bytes(int f) { return (f == FMT_D32 || f == FMT_C32) ? 4 : 8; }
-constexpr bool format_compact(int f) { return f == FMT_C64 || f == FMT_C32 || f == FMT_G64; }
-constexpr bool format_stored(int f) { return f != FMT_G64; }
+//   FMT_G32  geometric FP32: the FMT_G64 values rounded to FP32 (the compact
+//            FP32 values), FP32 products and accumulation; the tangent
+//            product reads an FP32 copy of w Dep
+enum BFormat { FMT_D64 = 0, FMT_C64 = 1, FMT_D32 = 2, FMT_C32 = 3, FMT_G64 = 4, FMT_G32 = 5, FMT_COUNT = 6 };
+constexpr int format_bytes(int f) { return (f == FMT_D32 || f == FMT_C32 || f == FMT_G32) ? 4 : 8; }
+constexpr bool format_compact(int f) { return f == FMT_C64 || f == FMT_C32 || f == FMT_G64 || f == FMT_G32; }
+constexpr bool format_geometric(int f) { return f == FMT_G64 || f == FMT_G32; }
+constexpr bool format_stored(int f) { return !format_geometric(f); }
 // values per integration point (stored formats: padded to a whole 16-byte
 // vector; geometric: the compact values the kernel computes)
 constexpr int format_values(int s, int f) {
@@ -155,7 +159,10 @@ void launch_residual_fmt(int fmt, int shape, bool commit, bool fast, const Group
 // g.dof's (g.node's) entries flagged kFirstWrite start from 0 instead of y[d],
 // so the launches of all groups overwrite y with K v and need no clear.
 void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, const double* v, double* y,
-                        const double* dep_soa, int64_t npad, bool zero_first, hipStream_t stream);
+                        const double* dep_soa, const float* dep32_soa, int64_t npad, bool zero_first,
+                        hipStream_t stream);
+// dst[i] = float(src[i]), i < n (the FP32 copy of w Dep for FMT_G32)
+void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t stream);
 // dense FP64 tiles of one group -> format fmt
 void launch_convert_B(int fmt, int shape, const double* dense, void* out, int64_t ntiles, hipStream_t stream);
 // Device setup of one group from geometry (corners: 8 doubles per element

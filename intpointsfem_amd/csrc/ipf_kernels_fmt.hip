@@ -40,6 +40,8 @@ template <>
 struct FmtTraits<FMT_C32> { using T = float; static constexpr bool COMPACT = true, GEOM = false; static constexpr int VEC = 4; };
 template <>
 struct FmtTraits<FMT_G64> { using T = double; static constexpr bool COMPACT = true, GEOM = true; static constexpr int VEC = 2; };
+template <>
+struct FmtTraits<FMT_G32> { using T = float; static constexpr bool COMPACT = true, GEOM = true; static constexpr int VEC = 4; };
 
 template <int SHAPE, int FMT>
 struct FC {
@@ -89,10 +91,11 @@ __device__ __forceinline__ void load_values(const void* __restrict__ Bg, int64_t
 
 // Geometric format: the compact values (dN_a/dx, dN_a/dy) of the lane's point
 // from its element's corners (Bg: 8 doubles per element slot of the group)
-// and the shape table; returns the weight w_q |det J|.
-template <int SHAPE>
+// and the shape table, computed in FP64 and stored as T (FP32: rounded);
+// returns the weight w_q |det J|.
+template <int SHAPE, class T>
 __device__ __forceinline__ double geom_values(const double* __restrict__ Bg, const double* __restrict__ tab,
-                                              int64_t tile, int lane, double* b) {
+                                              int64_t tile, int lane, T* b) {
     using C = FC<SHAPE, FMT_G64>;
     const int el = lane / C::NIP, ip = lane - el * C::NIP;
     const dv2* Xv = reinterpret_cast<const dv2*>(Bg) + (tile * C::ET + el) * 4;
@@ -109,7 +112,10 @@ __device__ __forceinline__ double geom_values(const double* __restrict__ Bg, con
 #pragma unroll
     for (int a = 0; a < C::CE / 2; a++) {
         const dv2 d = dphi[a];
-        quad_grad(m, d.x, d.y, b[2 * a], b[2 * a + 1]);
+        double gx, gy;
+        quad_grad(m, d.x, d.y, gx, gy);
+        b[2 * a] = T(gx);
+        b[2 * a + 1] = T(gy);
     }
     return quad_weight(m, xi[2 * C::NIP + ip]);
 }
@@ -363,7 +369,9 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
 template <int SHAPE, int FMT, bool PAIR>
 __global__ void __launch_bounds__(kWave, (FC<SHAPE, FMT>::MIN_WAVES))
 k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict__ v, double* __restrict__ y,
-              const double* __restrict__ dep, int64_t npad, bool zero_first) {
+              const double* __restrict__ dep, const float* __restrict__ dep32, int64_t npad, bool zero_first) {
+    // FMT_G32: the whole element product in FP32 (w Dep from its FP32 copy)
+    constexpr bool F32 = FMT == FMT_G32;
     using C = FC<SHAPE, FMT>;
     using T = typename C::T;
     constexpr int NSH = C::CE / 2;
@@ -438,12 +446,18 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     const bool task = lane < C::ET * (PAIR ? NSH : H) && re < nv;
     const bool first_half = !PAIR || rj < H / 2;
     double dp[9], y0 = 0.0, y1 = 0.0;
+    float dpf[9];
     auto start = [&](int de, int off) -> double {
         return (zero_first && de < 0) ? 0.0 : y[(de & kDofMask) + off];
     };
     auto request = [&]() {
 #pragma unroll
-        for (int k = 0; k < 9; k++) dp[k] = ld_stream(&dep[k * npad + P]);
+        for (int k = 0; k < 9; k++) {
+            if constexpr (F32)
+                dpf[k] = ld_stream(&dep32[k * npad + P]);
+            else
+                dp[k] = ld_stream(&dep[k * npad + P]);
+        }
         if (task) {
             if (PAIR) {
                 const int de = dsx[re * NSH + rj];
@@ -463,10 +477,17 @@ k_tangent_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict_
     if (valid) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            double a = dp[3 * k] * e3[0];
-            a = a + dp[3 * k + 1] * e3[1];
-            a = a + dp[3 * k + 2] * e3[2];
-            x[k] = -a;  // reduce_lane adds -(sum); y accumulates +K v
+            if constexpr (F32) {
+                float a = dpf[3 * k] * float(e3[0]);
+                a = a + dpf[3 * k + 1] * float(e3[1]);
+                a = a + dpf[3 * k + 2] * float(e3[2]);
+                x[k] = -double(a);
+            } else {
+                double a = dp[3 * k] * e3[0];
+                a = a + dp[3 * k + 1] * e3[1];
+                a = a + dp[3 * k + 2] * e3[2];
+                x[k] = -a;  // reduce_lane adds -(sum); y accumulates +K v
+            }
         }
     }
     if (slot) bt_cols<SHAPE, FMT, 0, H>(b, x, lds + lane * H);
@@ -555,6 +576,15 @@ __global__ void k_build_B(GroupDev g, const double* __restrict__ corners, double
     w[g.pt0 + tile * C::TP + lane] = wt;
 }
 
+// Orders the LDS accesses of one wavefront (its lanes exchange values
+// through LDS; LDS instructions of a wavefront execute in order, so only
+// the compiler must not move them across this point).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // The non-committing residual over the patch layout (ipf_patch.h): one
 // workgroup of patch_waves(SHAPE) wavefronts per patch.  Per point the
 // colored kernel's arithmetic (geometric FP64 B, compact strain sums, return
@@ -562,19 +592,26 @@ __global__ void k_build_B(GroupDev g, const double* __restrict__ corners, double
 // column sums over its points (point order) into LDS, and per patch node
 // the sum of its elements' contributions (ascending element slot, then
 // local node), written to r = f_lin + (-sum) where the patch owns the node,
-// else to the node's partial slot for the fix-up pass.
+// else to the node's partial slot for the fix-up pass.  A patch has at most
+// one node per thread: thread t owns patch node t, whose DoF, partial slot,
+// first contributions and u values it requests at the start, so that their
+// latency hides behind the points' work; the slots' local node indexes go to
+// LDS.  Two workgroup barriers: the nodes' u in LDS, the elements' sums in LDS.
 template <int SHAPE, bool FAST>
 __global__ void __launch_bounds__(kWave * patch_waves(SHAPE))
 k_residual_patch(PatchDev p, const double* __restrict__ u, double* __restrict__ r, const double* __restrict__ flin,
                  double* __restrict__ partial) {
     using C = FC<SHAPE, FMT_G64>;
     constexpr int NW = patch_waves(SHAPE), ET = C::ET, NIP = C::NIP, CE = C::CE, TP = C::TP;
-    constexpr int NSH = CE / 2, PE = NW * ET, MAXN = patch_max_nodes(SHAPE);
+    constexpr int NSH = CE / 2, PE = NW * ET, MAXN = patch_max_nodes(SHAPE), NT = NW * kWave;
     constexpr int H = CE / 2, H2 = CE - H;
+    constexpr int MAXC = 4;  // contributions requested up front; more are read at the end
     static_assert(ET * H <= kWave && ET * H2 <= kWave, "one column sum per lane and half");
+    static_assert(MAXN <= NT, "one patch node per thread");
     __shared__ double us[2 * MAXN];
     __shared__ double es[PE * CE];
     __shared__ double stg[NW * TP * H];
+    __shared__ unsigned short lns[PE * NSH];
     const int tid = threadIdx.x, w = tid / kWave, lane = tid - w * kWave;
     const int64_t pa = blockIdx.x;
     const int64_t n0 = p.node_off[pa];
@@ -585,39 +622,45 @@ k_residual_patch(PatchDev p, const double* __restrict__ u, double* __restrict__ 
     const int el = lane / NIP;
     const bool slot = lane < TP;
     const bool valid = slot && el < nv;
-    // requests first: the point's plastic strain (padding points hold zeros)
-    // and the element's local node indexes, then the patch's nodes into LDS
+    // the thread's patch node: DoF, partial slot, contributions, u
+    const bool owner = tid < nn;
+    const int64_t k = n0 + (owner ? tid : 0);
+    const int d = p.pnode[k], ps = p.pslot[k];
+    const int c0 = p.cbeg[k], c1 = owner ? p.cbeg[k + 1] : c0;
+    const double ux = u[d], uy = u[d + 1];
+    int cv[MAXC];
+#pragma unroll
+    for (int j = 0; j < MAXC; j++) cv[j] = p.contrib[min(c0 + j, max(c1 - 1, c0))];
+    // the point's plastic strain (padding points hold zeros), the slots' local nodes
     const int64_t P = tile * TP + (slot ? lane : 0);
     double ep[4] = {ld_stream(&p.ep0[P]), ld_stream(&p.ep1[P]), ld_stream(&p.ep2[P]), ld_stream(&p.ep3[P])};
-    const int64_t sl = tile * ET + (valid ? el : 0);
-    int ln[NSH];
-#pragma unroll
-    for (int a = 0; a < NSH; a++) ln[a] = p.lnode[sl * NSH + a];
-    for (int i = tid; i < nn; i += NW * kWave) {
-        const int d = p.pnode[n0 + i];
-        us[2 * i] = u[d];
-        us[2 * i + 1] = u[d + 1];
-    }
+    for (int i = tid; i < PE * NSH; i += NT) lns[i] = p.lnode[pa * (PE * NSH) + i];
     double b[C::NV];
     double wgeo = 0.0;
     if (valid) {
         wgeo = geom_values<SHAPE>(p.geo, p.tab, tile, lane, b);
     } else {
 #pragma unroll
-        for (int k = 0; k < C::NV; k++) b[k] = 0.0;
+        for (int q = 0; q < C::NV; q++) b[q] = 0.0;
+    }
+    if (owner) {
+        us[2 * tid] = ux;
+        us[2 * tid + 1] = uy;
     }
     __syncthreads();
     double ws[3] = {0.0, 0.0, 0.0};
     if (valid) {
+        const unsigned short* ln = lns + (w * ET + el) * NSH;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll
         for (int a = 0; a < NSH; a++) {
             const double dx = b[2 * a], dy = b[2 * a + 1];
-            const double ux = us[2 * ln[a]], uy = us[2 * ln[a] + 1];
-            a0 = a0 + dx * ux;
-            a1 = a1 + dy * ux;
-            a1 = a1 + dx * uy;
-            a2 = a2 + dy * uy;
+            const int l = ln[a];
+            const double vx = us[2 * l], vy = us[2 * l + 1];
+            a0 = a0 + dx * vx;
+            a1 = a1 + dy * vx;
+            a1 = a1 + dx * vy;
+            a2 = a2 + dy * vy;
         }
         const double e3[3] = {a0, a1, a2};
         const MatConst& mc = p.mats[tm >> 8];
@@ -627,21 +670,21 @@ k_residual_patch(PatchDev p, const double* __restrict__ u, double* __restrict__ 
             point_sigma<false>(e3, ep, alpha, mt, br, mc, wgeo, ws, nullptr);
         }
     }
-    // element column sums, half the columns at a time through the wave's staging
+    // element column sums, half the columns at a time through the wave's own staging
     double* L = stg + w * TP * H;
     const int re = lane / H, rj = lane - re * H;
     if (slot) bt_cols<SHAPE, FMT_G64, 0, H>(b, ws, L + lane * H);
-    __syncthreads();
+    wave_lds_sync();
     if (lane < ET * H && re < nv) {
         double f = L[(re * NIP) * H + rj];
 #pragma unroll
         for (int ip = 1; ip < NIP; ip++) f = f + L[(re * NIP + ip) * H + rj];
         es[(w * ET + re) * CE + rj] = f;
     }
-    __syncthreads();
+    wave_lds_sync();
     const int re2 = lane / H2, rj2 = lane - re2 * H2;
     if (slot) bt_cols<SHAPE, FMT_G64, H, CE>(b, ws, L + lane * H2);
-    __syncthreads();
+    wave_lds_sync();
     if (lane < ET * H2 && re2 < nv) {
         double f = L[(re2 * NIP) * H2 + rj2];
 #pragma unroll
@@ -650,17 +693,20 @@ k_residual_patch(PatchDev p, const double* __restrict__ u, double* __restrict__ 
     }
     __syncthreads();
     // node sums: owned nodes straight into r (never read), shared ones to their partial slot
-    for (int i = tid; i < nn; i += NW * kWave) {
-        const int64_t k = n0 + i;
+    if (owner) {
         double fx = 0.0, fy = 0.0;
-        const int c1 = p.cbeg[k + 1];
-        for (int c = p.cbeg[k]; c < c1; c++) {
-            const int v = p.contrib[c];
-            const int s = v / NSH, a = v - s * NSH;
-            fx = fx + es[s * CE + 2 * a];
-            fy = fy + es[s * CE + 2 * a + 1];
+#pragma unroll
+        for (int j = 0; j < MAXC; j++)
+            if (c0 + j < c1) {
+                const int v = cv[j], s_ = v / NSH, a = v - s_ * NSH;
+                fx = fx + es[s_ * CE + 2 * a];
+                fy = fy + es[s_ * CE + 2 * a + 1];
+            }
+        for (int c = c0 + MAXC; c < c1; c++) {
+            const int v = p.contrib[c], s_ = v / NSH, a = v - s_ * NSH;
+            fx = fx + es[s_ * CE + 2 * a];
+            fy = fy + es[s_ * CE + 2 * a + 1];
         }
-        const int d = p.pnode[k], ps = p.pslot[k];
         if (ps < 0) {
             r[d] = flin[d] + (-fx);
             r[d + 1] = flin[d + 1] + (-fy);
@@ -697,15 +743,20 @@ void residual_fmt(int shape, bool commit, bool fast, const GroupDev& g, const vo
 
 template <int FMT>
 void tangent_fmt(int shape, const GroupDev& g, const void* Bv, const double* v, double* y, const double* dep,
-                 int64_t npad, bool zf, hipStream_t s) {
+                 const float* dep32, int64_t npad, bool zf, hipStream_t s) {
     const dim3 grid((unsigned)g.ntiles), block(kWave);
     dispatch_shape(shape, [&](auto sc) {
         constexpr int S = decltype(sc)::value;
         if (g.node)
-            hipLaunchKernelGGL((k_tangent_fmt<S, FMT, true>), grid, block, 0, s, g, Bv, v, y, dep, npad, zf);
+            hipLaunchKernelGGL((k_tangent_fmt<S, FMT, true>), grid, block, 0, s, g, Bv, v, y, dep, dep32, npad, zf);
         else
-            hipLaunchKernelGGL((k_tangent_fmt<S, FMT, false>), grid, block, 0, s, g, Bv, v, y, dep, npad, zf);
+            hipLaunchKernelGGL((k_tangent_fmt<S, FMT, false>), grid, block, 0, s, g, Bv, v, y, dep, dep32, npad, zf);
     });
+}
+
+__global__ void k_to_f32(int64_t n, const double* __restrict__ src, float* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = float(src[i]);
 }
 
 template <int FMT>
@@ -727,6 +778,7 @@ void launch_residual_fmt(int fmt, int shape, bool commit, bool fast, const Group
         case FMT_D64: residual_fmt<FMT_D64>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
         case FMT_C64: residual_fmt<FMT_C64>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
         case FMT_G64: residual_fmt<FMT_G64>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
+        case FMT_G32: residual_fmt<FMT_G32>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
         case FMT_D32: residual_fmt<FMT_D32>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
         case FMT_C32: residual_fmt<FMT_C32>(shape, commit, f, g, Bv, u, r, flin, st, s); break;
         default: break;
@@ -734,17 +786,23 @@ void launch_residual_fmt(int fmt, int shape, bool commit, bool fast, const Group
 }
 
 void launch_tangent_fmt(int fmt, int shape, const GroupDev& g, const void* Bv, const double* v, double* y,
-                        const double* dep, int64_t npad, bool zero_first, hipStream_t s) {
+                        const double* dep, const float* dep32, int64_t npad, bool zero_first, hipStream_t s) {
     if (g.ntiles == 0) return;
     const bool zf = zero_first;
     switch (fmt) {
-        case FMT_D64: tangent_fmt<FMT_D64>(shape, g, Bv, v, y, dep, npad, zf, s); break;
-        case FMT_C64: tangent_fmt<FMT_C64>(shape, g, Bv, v, y, dep, npad, zf, s); break;
-        case FMT_G64: tangent_fmt<FMT_G64>(shape, g, Bv, v, y, dep, npad, zf, s); break;
-        case FMT_D32: tangent_fmt<FMT_D32>(shape, g, Bv, v, y, dep, npad, zf, s); break;
-        case FMT_C32: tangent_fmt<FMT_C32>(shape, g, Bv, v, y, dep, npad, zf, s); break;
+        case FMT_D64: tangent_fmt<FMT_D64>(shape, g, Bv, v, y, dep, dep32, npad, zf, s); break;
+        case FMT_C64: tangent_fmt<FMT_C64>(shape, g, Bv, v, y, dep, dep32, npad, zf, s); break;
+        case FMT_G64: tangent_fmt<FMT_G64>(shape, g, Bv, v, y, dep, dep32, npad, zf, s); break;
+        case FMT_D32: tangent_fmt<FMT_D32>(shape, g, Bv, v, y, dep, dep32, npad, zf, s); break;
+        case FMT_C32: tangent_fmt<FMT_C32>(shape, g, Bv, v, y, dep, dep32, npad, zf, s); break;
+        case FMT_G32: tangent_fmt<FMT_G32>(shape, g, Bv, v, y, dep, dep32, npad, zf, s); break;
         default: break;
     }
+}
+
+void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, src, dst);
 }
 
 void launch_residual_patch(int shape, bool fast, const PatchDev& p, const double* u, double* r, const double* flin,

@@ -47,6 +47,7 @@ struct PatchElemIn {
     int64_t P0;        // new-point index of its first point in the colored layout (points consecutive)
     int64_t node0;     // first of its shape_ce / 2 node x-DoFs in the builder's node array
     double cx, cy;     // centroid of its corners
+    int key = 0;       // order key inside its patch (ascending, stable): elements of equal keys share tiles
 };
 
 // Device view of the patches of one element shape.

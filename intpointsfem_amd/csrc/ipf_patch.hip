@@ -136,6 +136,9 @@ std::string build_patches(const std::vector<PatchElemIn>& el, const std::vector<
             int64_t j = i;
             while (j < ne && el[idx[j]].shape == shape && el[idx[j]].mat == mat && j - i < PE) j++;
             const int64_t npe = j - i;
+            // inside the patch: elements by order key (stable), so that equal keys fill whole tiles
+            std::stable_sort(idx.begin() + i, idx.begin() + j,
+                             [&](int64_t a, int64_t b) { return el[a].key < el[b].key; });
             // the patch's distinct nodes, ascending by x DoF
             local.clear();
             for (int64_t k = i; k < j; k++)

@@ -221,3 +221,48 @@ def test_autotune_selects_the_fastest_format(gpu):
     assert {"dense-fp32", "compact-fp32"} <= set(t2["tangent"])
     assert t2["tangent"][t2["selected"][1]] == min(t2["tangent"].values())
     print(f"autotune: residual {t['residual']} -> {r_sel}; tangent {t2['tangent']} -> {t2['selected'][1]}")
+
+
+def test_geometry_fp32(gpu):
+    """geometry-fp32: B recomputed from the corners in FP64 and rounded to
+    FP32 is the compact FP32 B, so residuals (FP32 products, FP64 return map)
+    equal compact FP32's bit for bit; its tangent product (FP32 w Dep too) is
+    the FP64 product to FP32 accuracy (1e-5 of max |K v|); a Newton with it in
+    the CG reaches the FP64 Newton's solution (iterative refinement)."""
+    import torch
+    m, sm, ctx, B, ix = _setup((20, 20, 2, 1.1))
+    neq = m.info()["neq"]
+    u = _t(wb.lame_displacement(m.nodes(), wb.normal_stress(10), scale=4.0), gpu)
+    ctx.set_b_format("compact-fp32")
+    r_c = _residual(ctx, u, neq, gpu)
+    ctx.set_b_format("geometry-fp32")
+    r_g = _residual(ctx, u, neq, gpu)
+    assert np.array_equal(r_c, r_g)
+    r = torch.zeros(neq, dtype=torch.float64, device=gpu)
+    ctx.residual_tangent(u, r)
+    v = _t(np.random.default_rng(0).standard_normal(neq), gpu)
+    y = {}
+    for fmt in ("geometry-fp64", "geometry-fp32"):
+        ctx.set_b_format("geometry-fp64", tangent=fmt)
+        yy = torch.full((neq,), float("nan"), dtype=torch.float64, device=gpu)
+        ctx.tangent_apply(v, yy)
+        ctx.synchronize()
+        y[fmt] = yy.cpu().numpy()
+    err = np.abs(y["geometry-fp32"] - y["geometry-fp64"]).max() / np.abs(y["geometry-fp64"]).max()
+    print(f"geometry-fp32 tangent product vs FP64: {err:.2e}")
+    assert 0 < err <= 1e-5
+    sols = {}
+    for tangent in ("geometry-fp64", "geometry-fp32"):
+        m2 = wb.make_mesh((20, 20, 2, 1.1 ** 2))
+        sm2 = ipf.StructMatrix(m2, device=0)
+        c2 = sm2.context()
+        c2.set_dirichlet_mask(m2.dirichlet_mask())
+        c2.set_b_format("geometry-fp64", tangent=tangent)
+        u2 = torch.zeros(m2.info()["neq"], dtype=torch.float64, device=gpu)
+        st = c2.newton(u2, rtol=1e-12, maxit=80, cg_rtol=1e-6, cg_maxit=20000)
+        assert st["converged"] == 1.0, (tangent, st)
+        sols[tangent] = u2.cpu().numpy()
+    ref = sols["geometry-fp64"]
+    e2 = np.abs(sols["geometry-fp32"] - ref).max() / np.abs(ref).max()
+    print(f"newton with geometry-fp32 tangent vs FP64: {e2:.2e}")
+    assert e2 <= 1e-9
