@@ -342,6 +342,19 @@ int ipf_point_features(ipf_ctx *ctx, const double *u_dev, float *feat_dev, int *
 int ipf_branch_order(ipf_ctx *ctx, const int *branch_dev, int *order_dev);
 int ipf_compute_sigma_ordered(ipf_ctx *ctx, const double *strain_dev, double *wsigma_dev, const int *order_dev);
 
+/* ipf_branch_order that allocates nothing: no hipMalloc / hipFree or
+ * allocating library call on any call, first included.  No counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  The counting
+ * sort's per-block histograms live in a fixed 4 MiB device array of the
+ * library, one per GPU, held until the process ends and shared by every
+ * call on that GPU: calls that overlap on one GPU (other streams, other
+ * threads) race and must be ordered by the caller.  Same order_dev as
+ * ipf_branch_order.  IPF_EINVAL when the context holds more points than
+ * ipf_branch_order_noalloc_max_points() (53,687,040). */
+int ipf_branch_order_noalloc(ipf_ctx *ctx, const int *branch_dev, int *order_dev);
+int64_t ipf_branch_order_noalloc_max_points(void);
+
 typedef struct ipf_mlp ipf_mlp;
 int ipf_mlp_create(int device, uint64_t seed, ipf_mlp **out);
 int ipf_mlp_destroy(ipf_mlp *mlp);

@@ -118,6 +118,8 @@ def lib():
         "ipf_point_features": (_int, [_vp, _vp, _vp, _vp, _vp]),
         "ipf_branch_order": (_int, [_vp, _vp, _vp]),
         "ipf_compute_sigma_ordered": (_int, [_vp, _vp, _vp, _vp]),
+        "ipf_branch_order_noalloc": (_int, [_vp, _vp, _vp]),
+        "ipf_branch_order_noalloc_max_points": (_i64, []),
         "ipf_mlp_create": (_int, [_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
         "ipf_mlp_destroy": (_int, [_vp]),
         "ipf_mlp_set_stream": (_int, [_vp, _vp]),
@@ -618,6 +620,9 @@ class DeviceContext:
 
     def branch_order(self, branch, order):
         self._c(lib().ipf_branch_order(self._h, _ptr(branch), _ptr(order)), "ipf_branch_order")
+
+    def branch_order_noalloc(self, branch, order):
+        self._c(lib().ipf_branch_order_noalloc(self._h, _ptr(branch), _ptr(order)), "ipf_branch_order_noalloc")
 
     def compute_sigma_ordered(self, strain, wsigma, order):
         self._c(lib().ipf_compute_sigma_ordered(self._h, _ptr(strain), _ptr(wsigma), _ptr(order)),

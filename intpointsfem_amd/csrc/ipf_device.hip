@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -764,6 +765,44 @@ int ipf_internal_set_error(const char* msg) {
     return IPF_OK;
 }
 
+// ------------------------------------------------- scratch of the no-alloc sort
+// The per-block key histograms of ipf_branch_order_noalloc live in this
+// file-scope device array instead of a hipMalloc'd buffer: 4 MiB, one copy
+// on every GPU the library runs on, held until the process ends.  Host code
+// never passes the array's name to a kernel (that name is a host-side
+// stand-in, not a device address): each device's address is looked up once,
+// with that device current, when the first context on it is created, and
+// kept in g_scratch_dev[device].
+constexpr int64_t kScratchInts = int64_t(1) << 20;
+static __device__ __align__(256) int scratch[kScratchInts];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_dev[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on `device` (current); once per device.
+int scratch_setup(int device) {
+    if (device >= kMaxScratchDevices)
+        return fail(nullptr, IPF_EUNSUPPORTED, "device %d beyond the %d devices the scratch table holds", device,
+                    kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return IPF_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(nullptr, IPF_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                    hipGetErrorString(e));
+    g_scratch_dev[device] = static_cast<int*>(p);
+    return IPF_OK;
+}
+
+int* scratch_of(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return (device >= 0 && device < kMaxScratchDevices) ? g_scratch_dev[device] : nullptr;
+}
+}  // namespace
+
 int ipf_ctx_create(int device, ipf_ctx** out) {
     if (!out) return fail(nullptr, IPF_EINVAL, "ipf_ctx_create: out is NULL");
     *out = nullptr;
@@ -780,6 +819,12 @@ int ipf_ctx_create(int device, ipf_ctx** out) {
     if (e != hipSuccess) {
         delete ctx;
         return fail(nullptr, IPF_EHIP, "context setup failed: %s", hipGetErrorString(e));
+    }
+    const int rs = scratch_setup(device);
+    if (rs != IPF_OK) {
+        (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return rs;
     }
     ctx->stream = ctx->own_stream;
     *out = ctx;
@@ -1649,6 +1694,25 @@ int ipf_branch_order(ipf_ctx* ctx, const int* branch_dev, int* order_dev) {
     launch_branch_order(branch_dev, ctx->npts, ctx->dHist, order_dev, ctx->stream);
     return launch_status(ctx, "branch_order");
 }
+
+int ipf_branch_order_noalloc(ipf_ctx* ctx, const int* branch_dev, int* order_dev) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!branch_dev || !order_dev) return fail(ctx, IPF_EINVAL, "ipf_branch_order_noalloc: NULL pointer");
+    const int64_t need = (int64_t)key_sort_blocks(ctx->npts) * IPF_NBRANCH;
+    if (need > kScratchInts)
+        return fail(ctx, IPF_EINVAL, "ipf_branch_order_noalloc: %lld points exceed its limit of %lld",
+                    (long long)ctx->npts, (long long)ipf_branch_order_noalloc_max_points());
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    int* hist = scratch_of(ctx->device);
+    if (!hist) return fail(ctx, IPF_ESTATE, "ipf_branch_order_noalloc: no scratch address for device %d", ctx->device);
+    // every histogram entry the sort reads is written by its first kernel:
+    // nothing in the scratch needs clearing
+    launch_branch_order(branch_dev, ctx->npts, hist, order_dev, ctx->stream);
+    return launch_status(ctx, "branch_order_noalloc");
+}
+
+int64_t ipf_branch_order_noalloc_max_points(void) { return (kScratchInts / IPF_NBRANCH) * IPF_SORT_BLOCK; }
 
 int ipf_compute_sigma_ordered(ipf_ctx* ctx, const double* strain_dev, double* wsigma_dev, const int* order_dev) {
     int rc = check_ready(ctx, true);
