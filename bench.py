@@ -67,16 +67,16 @@ def algorithmic_bytes(blocks, npts, neq, fmt="dense-fp64"):
     entry; compact: dN/dx, dN/dy per node and point; geometric: 8 corner
     coordinates per element), the weight per point (not for the geometric
     form, which computes it), a 4-byte DoF index per block column, 4
-    plastic-strain components per point, u read and r read-modify-written
-    once."""
+    plastic-strain components per point, and per DoF u and f_lin read once
+    and r written once (r = f_lin - sum B^T w sigma)."""
     import numpy as np
     R = blocks["row_sizes"].astype(np.int64)
     C = blocks["col_sizes"].astype(np.int64)
     if fmt == "geometry-fp64":
-        return int((64 + 4 * C).sum()) + npts * 32 + neq * (8 + 8)
+        return int((64 + 4 * C).sum()) + npts * 32 + neq * (8 + 8 + 8)
     word = 4 if fmt.endswith("fp32") else 8
     nval = (R // 3) * C if fmt.startswith("compact") else R * C
-    return int((word * nval + 4 * C).sum()) + npts * (8 + 32) + neq * (8 + 8)
+    return int((word * nval + 4 * C).sum()) + npts * (8 + 32) + neq * (8 + 8 + 8)
 
 
 _ROCTX = None
@@ -139,13 +139,14 @@ def time_residuals(ctx, u, r, stream, steps, warmup, roi=None):
     return start.elapsed_time(end)
 
 
-def kernel_name(fmt, shape, pairs=True):
+def kernel_name(fmt, shape, pairs=True, fast=False):
     """Name rocprofv3 shows for the non-committing residual kernel of a B
     format and element shape (pairs: one gathered index per node, taken when
     every element numbers its nodes' DoFs as (x, y) pairs, as the wellbore
-    meshes do)."""
+    meshes do; fast: the certified-elastic stress mode)."""
     import intpointsfem_amd as ipf
-    return f"void k_residual_fmt<{shape}, {ipf.B_FORMATS.index(fmt)}, false, {'true' if pairs else 'false'}>"
+    tf = lambda b: "true" if b else "false"
+    return f"void k_residual_fmt<{shape}, {ipf.B_FORMATS.index(fmt)}, false, {tf(pairs)}, {tf(fast)}>"
 
 
 def launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt, step_ms=None):
@@ -158,10 +159,10 @@ def launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt, step_ms=None):
     those of the timed region.  The interval before the kernels (the f_lin
     copy) holds no kernel when the launches read f_lin themselves (colored
     assembly with every DoF in some element): it is left out of the gap and
-    reported as measured."""
+    reported as measured (the library reports an empty interval as 0)."""
     avg, pts = launch_times(ctx, u, r, nrep)
     kern = pts > 0  # the intervals of the residual kernels proper
-    first_real = float(avg[0]) > 0.002  # a copy / refresh kernel ran in interval 0
+    first_real = float(avg[0]) > 0.0  # a copy / refresh kernel ran in interval 0 (else exactly 0)
     has = kern.copy()
     has[0] = first_real
     tail = np.zeros(len(avg), dtype=bool)
@@ -319,10 +320,12 @@ def learned_mixed_batched(args, dev):
     """The learned + mixed-precision batched residual on the 64 uneven
     instances (tests/golden/batch64_instances.json, drawn with seed 0):
 
-      * state: one load step to each instance's final wellbore pressure from
-        rest, solved by the segmented Newton with FP32 tangent products in the
-        CG and FP64 residuals (iterative refinement) to ||r_k|| / ||r0_k|| <=
-        newton_rtol per instance, then committed;
+      * state (--batch-state): lame (default) -- each instance's plastic
+        state committed at twice its linear-elastic (Lame) displacement, the
+        residuals at three times it; newton -- 10 load steps to each
+        instance's final wellbore pressure by the segmented Newton with FP32
+        tangent products in the CG and FP64 residuals (iterative
+        refinement), ||r_k|| / ||r0_k|| <= newton_rtol per instance, committed;
       * plain: the FP64 residual of the drop-in (colored assembly, exact
         return map), natural element order;
       * learned + mixed: the predictor (trained on this state's pairs) orders
@@ -336,6 +339,7 @@ def learned_mixed_batched(args, dev):
     import torch
 
     from intpointsfem_amd import batch
+    from intpointsfem_amd import wellbore as wb
     table = batch.draw_instances(64, seed=0)
     t0 = time.time()
     meshes = [batch.instance_mesh(i) for i in table]
@@ -346,19 +350,49 @@ def learned_mixed_batched(args, dev):
     ctx.set_stream(stream.cuda_stream)
     B.set_dirichlet_mask(B.dirichlet_mask())
     log(f"[bench] batch64: {len(B.row_sizes)} elements, {npts} points, neq {neq}; setup {time.time() - t0:.1f}s")
-    fmt0 = ctx.b_format()
-    tfmt = args.batch_tangent_format
-    ctx.set_b_format(fmt0[0], tangent=tfmt)
-    u = torch.zeros(neq, dtype=torch.float64, device=dev)
-    t1 = time.time()
-    st = B.newton(u, rtol=args.newton_rtol, maxit=80, cg_rtol=args.cg_rtol, cg_maxit=200000)
-    solve_s = time.time() - t1
-    ctx.set_b_format(*fmt0)
-    newton = {"tangent_format": tfmt, "iterations": int(st["iterations"]), "converged": bool(st["converged"] == 1.0),
-              "max_rel_residual": st["rel_residual"], "cg_iterations": int(st["cg_iterations"]),
-              "solve_seconds": round(solve_s, 1), "load": "one step from rest to each instance's final pressure"}
-    log(f"[bench] batch64 newton ({tfmt} tangent): {newton['iterations']} iterations, {newton['cg_iterations']} CG, "
-        f"worst ||r_k||/||r0_k|| {st['rel_residual']:.2e}, {solve_s:.1f} s")
+    newton = None
+    if args.batch_state == "newton":
+        # 10 load steps to each instance's final pressure (every instance's
+        # load is linear in the step), segmented Newton with the mixed tangent
+        fmt0 = ctx.b_format()
+        tfmt = args.batch_tangent_format
+        ctx.set_b_format(fmt0[0], tangent=tfmt)
+        u = torch.zeros(neq, dtype=torch.float64, device=dev)
+        t1 = time.time()
+        ctx.set_newton_monitor(lambda it, rel, rn, cg: log(f"[bench]   batch64 newton {it}: worst ||r_k||/||r0_k|| "
+                                                              f"{rel:.2e}, cg {cg}, {time.time() - t1:.1f}s"))
+        its, cgs, worst, conv = [], 0, 0.0, True
+        for k in range(1, wb.NSTEPS + 1):
+            ctx.set_rhs_linear(B.rhs_linear * (k / wb.NSTEPS))
+            st = B.newton(u, rtol=args.newton_rtol, maxit=80, cg_rtol=args.cg_rtol, cg_maxit=200000)
+            its.append(int(st["iterations"]))
+            cgs += int(st["cg_iterations"])
+            worst = st["rel_residual"]
+            conv = conv and st["converged"] == 1.0
+            log(f"[bench] batch64 load step {k}: newton {its[-1]}, worst ||r_k||/||r0_k|| {worst:.2e}, "
+                f"{time.time() - t1:.1f}s")
+        ctx.set_newton_monitor(None)
+        ctx.set_rhs_linear(B.rhs_linear)
+        ctx.set_b_format(*fmt0)
+        newton = {"tangent_format": tfmt, "newton_iterations": its, "converged": conv,
+                  "final_max_rel_residual": worst, "cg_iterations": cgs, "solve_seconds": round(time.time() - t1, 1),
+                  "load": f"{wb.NSTEPS} load steps to each instance's final pressure"}
+        state = (f"converged segmented Newton solution of load step {wb.NSTEPS} (||r_k||/||r0_k|| <= "
+                 f"{args.newton_rtol:g} per instance, FP32 tangent products in the CG, FP64 residuals), committed")
+    else:
+        # each instance's linear-elastic (Lame) displacement: 2x committed, residuals at 3x
+        us = [wb.lame_displacement(m.nodes(), wb.normal_stress(wb.NSTEPS, wb.NSTEPS, i["pw_final"], wb.SIGMA0),
+                                   E_=i["E"], nu=i["nu"]) for i, m in zip(table, meshes)]
+        u1 = torch.as_tensor(np.concatenate([2 * x for x in us]), device=dev)
+        u = torch.as_tensor(np.concatenate([3 * x for x in us]), device=dev)
+        r1 = torch.zeros(neq, dtype=torch.float64, device=dev)
+        ctx.set_update_mem(True)
+        ctx.residual(u1, r1)
+        ctx.set_update_mem(False)
+        ctx.synchronize()
+        del u1, r1
+        state = ("plastic state committed at 2x each instance's linear-elastic (Lame) displacement, residuals at "
+                 "3x (--batch-state newton: the converged 10-step Newton state)")
     _, _, mt, br = ctx.get_state()
     r = torch.zeros(neq, dtype=torch.float64, device=dev)
     settle(ctx, u, r, args.settle_ms)
@@ -447,13 +481,12 @@ def learned_mixed_batched(args, dev):
     value = npts * args.steps / (lm_total / 1e3) / 1e6
     log(f"[bench] batch64 learned+mixed ({best[0]} assembly): {lm_ms:.4f} ms, {value:.1f} Mint-pts/s, "
         f"{plain_ms / lm_ms:.3f}x plain ({plain_ms:.4f} ms)")
-    out = {"workload": "64 uneven wellbore instances (seed-0 table) at the converged mixed-precision Newton state",
+    out = {"workload": "64 uneven wellbore instances (seed-0 table)", "state": state,
            "value": round(value, 2), "unit": "Mint-pts/s", "ms_per_step": round(lm_ms, 5),
            "speedup_vs_plain": round(plain_ms / lm_ms, 3),
            "plain": {"ms_per_step": round(plain_ms, 5), "value": round(npts * args.steps / (plain_total / 1e3) / 1e6, 2),
                      "path": "colored assembly, exact return map, index order (the drop-in's residual)"},
-           "path": {"assembly": best[0], "residual_mode": "certified-elastic", "order": "predicted branch",
-                    "state_solve": "FP32 tangent products in the CG, FP64 residuals"},
+           "path": {"assembly": best[0], "residual_mode": "certified-elastic", "order": "predicted branch"},
            "max_dev_vs_plain": dev_plain, "max_dev_vs_oracle": oracle_dev, "oracle_instances_checked": checked,
            "int_points": npts, "neq": neq, "newton": newton,
            "branches": {n: int((br == i).sum()) for i, n in enumerate(("elastic", "main", "right_edge", "left_edge",
@@ -476,6 +509,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-formats", action="store_true", help="skip the compact / FP32 B format variants")
     ap.add_argument("--no-batch", action="store_true", help="skip the learned + mixed batched measurement")
+    ap.add_argument("--no-predictor", action="store_true",
+                    help="skip training / timing the branch predictor on this mesh's state")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the library's default B format instead of the autotuner's choice")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline's timed work")
@@ -495,6 +530,9 @@ def main():
     ap.add_argument("--solve-tangent-format", default=None,
                     help="B format of the CG matvecs of the --state newton solve (default: the library's, "
                          "geometric FP64; the residual stays FP64 whatever this is)")
+    ap.add_argument("--batch-state", choices=("lame", "newton"), default="lame",
+                    help="state of the batched measurement: a committed plastic state from each instance's "
+                         "linear-elastic displacement (fast), or the converged 10-step Newton state (minutes)")
     ap.add_argument("--batch-tangent-format", default="compact-fp32",
                     help="B format of the CG matvecs of the batch's Newton solve (FP32: mixed precision)")
     args = ap.parse_args()
@@ -569,12 +607,15 @@ def main():
         u = torch.zeros(neq, dtype=torch.float64, device=dev)
         t1 = time.time()
         steps = []
+        ctx.set_newton_monitor(lambda it, rel, rn, cg: log(f"[bench]   newton {it}: ||r||/||r0|| {rel:.2e}, cg {cg}, "
+                                                              f"{time.time() - t1:.1f}s") if it % 5 == 0 else None)
         for k in range(1, wb.NSTEPS + 1):
             ctx.set_rhs_linear(f_unit * wb.normal_stress(k))
             st = ctx.newton(u, rtol=args.newton_rtol, maxit=60, cg_rtol=args.cg_rtol, cg_maxit=200000)
             steps.append(st)
             log(f"[bench] load step {k}: newton {int(st['iterations'])}, |r|/|r0| {st['rel_residual']:.2e}, "
                 f"cg {int(st['cg_iterations'])}, {time.time() - t1:.1f}s")
+        ctx.set_newton_monitor(None)
         ctx.set_b_format(*fmt_default)
         u_host = u.cpu().numpy()
         _, _, mt, br = ctx.get_state()
@@ -631,6 +672,10 @@ def main():
     r64 = r.cpu().numpy()
     fint = float(np.abs(r64 - sm.export_indexes()["rhs_linear"]).max())
     paths = residual_paths(ctx, u, r, stream, args.steps, args.warmup, npts, r64, fint)
+    predictor = None
+    if not args.no_predictor:
+        # the branch predictor on this state: its inference kernel over all the points
+        _, _, predictor = train_predictor(ctx, u, npts, dev)
 
     # the other B formats (same state, same u): timing, roofline on their own
     # stored bytes, and the deviation from the FP64 residual
@@ -720,6 +765,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "residual_paths": paths,
+        "predictor": predictor,
         "b_format_variants": variants,
         "learned_mixed_batched": lm,
     }

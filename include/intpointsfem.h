@@ -241,10 +241,11 @@ int ipf_certify_elastic_host(int64_t n, const double *strain3, const double *pla
 
 /* ipf_residual with a HIP event pair around every kernel launch on the
  * context's stream; waits, then writes the duration of each launch in
- * milliseconds to launch_ms (launch 0: r = f_lin copy, then one per element
- * group in launch order) and the group's point count to launch_pts (may be
- * NULL).  With the patch assembly (non-committing): launch 0 the refresh of
- * the patch copy of the plastic strain (empty when current), one launch per
+ * milliseconds to launch_ms (launch 0: r = f_lin copy, exactly 0 when no
+ * copy runs, then one per element group in launch order) and the group's
+ * point count to launch_pts (may be NULL).  With the patch assembly
+ * (non-committing): launch 0 the refresh of the patch copy of the plastic
+ * strain (exactly 0 when it is current), one launch per
  * element shape, then the shared nodes' fix-up (0 points).  Returns the number of launches (>= 0) or a negative IPF_E* code.
  * Measurement aid; the plain entry point records no events. */
 int ipf_residual_timed(ipf_ctx *ctx, const double *u_dev, double *r_dev, double *launch_ms, int64_t *launch_pts,
@@ -427,6 +428,15 @@ int ipf_cg_solve(ipf_ctx *ctx, const double *b_dev, double *x_dev, double rtol, 
  * stats[0..7] = Newton iterations, converged (0/1), ||r||/||r0||, ||r0||,
  * ||r||, total CG iterations, worst CG relative residual, 0. */
 int ipf_newton(ipf_ctx *ctx, double *u_dev, double rtol, int maxit, double cg_rtol, int cg_maxit, double *stats);
+
+/* Progress report of ipf_newton / ipf_newton_segmented, called on the host
+ * after each residual norm (iteration 0 = the entry u), like the residual
+ * norms the reference's Newton loop prints every iteration (main.cpp:240-241):
+ * rel = ||r|| / ||r0|| (segmented: the largest ||r_k|| / ||r0_k|| over the
+ * segments), res_norm = ||r|| (segmented: over all segments), cg_iterations
+ * = CG iterations so far.  fn NULL turns it off (the default). */
+typedef void (*ipf_newton_monitor)(void *user, int iteration, double rel, double res_norm, int cg_iterations);
+int ipf_set_newton_monitor(ipf_ctx *ctx, ipf_newton_monitor fn, void *user);
 
 /* Absolute residual floor of ipf_newton / ipf_newton_segmented (per
  * segment): also converged when ||r|| <= atol.  atol >= 0; 0 turns it off. */

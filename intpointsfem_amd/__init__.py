@@ -54,6 +54,10 @@ _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _d = ctypes.c_double
 _int = ctypes.c_int
+# ipf_newton_monitor: (user, iteration, rel, res_norm, cg_iterations)
+NEWTON_MONITOR = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_int)
+_MONITORS = {}
 
 _lib = None
 
@@ -104,6 +108,7 @@ def lib():
         "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
         "ipf_set_cg_graph": (_int, [_vp, _int]),
         "ipf_set_newton_atol": (_int, [_vp, _d]),
+        "ipf_set_newton_monitor": (_int, [_vp, NEWTON_MONITOR, _vp]),
         "ipf_set_residual_mode": (_int, [_vp, _int]),
         "ipf_get_residual_mode": (_int, [_vp, _ip]),
         "ipf_certify_elastic_host": (_int, [_i64, _dp, _dp, _d, _d, _d, _d, _d, _ip, _dp]),
@@ -385,6 +390,7 @@ class DeviceContext:
     def __del__(self):
         if getattr(self, "_own", False) and self._h and _lib is not None:
             _lib.ipf_ctx_destroy(self._h)
+            _MONITORS.pop(int(self._h.value or 0), None)
             self._h = None
 
     @property
@@ -604,6 +610,19 @@ class DeviceContext:
 
     def set_newton_atol(self, atol):
         self._c(lib().ipf_set_newton_atol(self._h, float(atol)), "ipf_set_newton_atol")
+
+    def set_newton_monitor(self, fn):
+        """fn(iteration, rel, res_norm, cg_iterations) after every residual
+        norm of newton / newton_segmented; None turns it off."""
+        # the callback object lives as long as the context may call it: in a
+        # table keyed by the context handle (several wrappers may share one)
+        key = int(self._h.value or 0)
+        cb = NEWTON_MONITOR() if fn is None else NEWTON_MONITOR(lambda _u, it, rel, rn, cg: fn(it, rel, rn, cg))
+        self._c(lib().ipf_set_newton_monitor(self._h, cb, None), "ipf_set_newton_monitor")
+        if fn is None:
+            _MONITORS.pop(key, None)
+        else:
+            _MONITORS[key] = cb
 
     def newton(self, u, rtol=1e-10, maxit=50, cg_rtol=1e-10, cg_maxit=20000):
         """Newton for the current load; commits the state. Returns a stats dict."""

@@ -173,6 +173,8 @@ struct ipf_ctx {
     bool have_data = false;
     bool update_mem = false;
     double newton_atol = 0.0;  // absolute residual floor of ipf_newton[_segmented]
+    ipf_newton_monitor monitor = nullptr;  // per-iteration report of ipf_newton[_segmented]
+    void* monitor_user = nullptr;
     int residual_mode = IPF_RESIDUAL_EXACT;  // ipf_set_residual_mode
     int assembly = IPF_ASSEMBLY_COLORED;     // ipf_set_residual_assembly
     // every call that may change the committed state bumps this; the patch
@@ -1532,6 +1534,13 @@ int ipf_get_residual_mode(const ipf_ctx* ctx, int* mode) {
     return IPF_OK;
 }
 
+int ipf_set_newton_monitor(ipf_ctx* ctx, ipf_newton_monitor fn, void* user) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_newton_monitor: NULL context");
+    ctx->monitor = fn;
+    ctx->monitor_user = user;
+    return IPF_OK;
+}
+
 int ipf_set_newton_atol(ipf_ctx* ctx, double atol) {
     if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_newton_atol: NULL context");
     if (!(atol >= 0.0)) return fail(ctx, IPF_EINVAL, "ipf_set_newton_atol: atol must be >= 0");
@@ -1572,13 +1581,14 @@ int ipf_residual_timed(ipf_ctx* ctx, const double* u_dev, double* r_dev, double*
         if (max_launches < np) return fail(ctx, IPF_EINVAL, "ipf_residual_timed: need room for %d launches", np);
         std::vector<hipEvent_t> ev(np + 1, nullptr);
         for (auto& e : ev) IPF_HIP(ctx, hipEventCreate(&e));
+        const bool refresh = ctx->patch_state_version != ctx->state_version;  // launch 0 runs
         rc = patch_residual(ctx, u_dev, r_dev, ev.data());
         if (rc == IPF_OK) {
             IPF_HIP(ctx, hipEventSynchronize(ev[np]));
             for (int i = 0; i < np; i++) {
                 float ms = 0.f;
                 IPF_HIP(ctx, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-                launch_ms[i] = ms;
+                launch_ms[i] = (i == 0 && !refresh) ? 0.0 : ms;
                 if (launch_pts) {
                     int64_t pts = 0;
                     if (i >= 1 && i <= nr)
@@ -1598,6 +1608,7 @@ int ipf_residual_timed(ipf_ctx* ctx, const double* u_dev, double* r_dev, double*
     for (auto& e : ev) IPF_HIP(ctx, hipEventCreate(&e));
     const PointState st = point_state(ctx);
     IPF_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+    const bool copied = !ctx->fold_flin;
     const double* flin = residual_start(ctx, r_dev);  // launch 0: the copy, or nothing when folded
     IPF_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
     for (size_t i = 0; i < ctx->groups.size(); i++) {
@@ -1611,7 +1622,7 @@ int ipf_residual_timed(ipf_ctx* ctx, const double* u_dev, double* r_dev, double*
         for (int i = 0; i < n; i++) {
             float ms = 0.f;
             IPF_HIP(ctx, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-            launch_ms[i] = ms;
+            launch_ms[i] = (i == 0 && !copied) ? 0.0 : ms;
             if (launch_pts)
                 launch_pts[i] = i == 0 ? 0
                                        : ctx->groups[i - 1].nreal * shape_nip(ctx->groups[i - 1].shape);
@@ -2049,6 +2060,7 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
         if ((rc = ipf_residual_tangent(ctx, u_dev, ctx->dNr))) break;
         if ((rc = masked_norm(ctx, ctx->dNr, &rn))) break;
         if (it == 0) r0 = rn;
+        if (ctx->monitor) ctx->monitor(ctx->monitor_user, it, r0 > 0 ? rn / r0 : 0.0, rn, cg_total);
         if (rn <= rtol * r0 || rn <= ctx->newton_atol || rn == 0.0) {
             converged = true;
             break;
@@ -2169,6 +2181,14 @@ int ipf_newton_segmented(ipf_ctx* ctx, double* u_dev, int nseg, const int64_t* s
                 }
             }
             all = all && done[k];
+        }
+        if (!rc && ctx->monitor) {
+            double worst = 0.0, sn = 0.0;
+            for (int k = 0; k < nseg; k++) {
+                if (r0[k] > 0) worst = std::max(worst, rn[k] / r0[k]);
+                sn += rn[k] * rn[k];
+            }
+            ctx->monitor(ctx->monitor_user, it, worst, std::sqrt(sn), cg_total);
         }
         if (rc || all || it == maxit) break;
         int cgi = 0;

@@ -60,6 +60,26 @@ def roi_stats(trace_dir, prefix, roi):
              "Percentage": 100.0 * v[1] / total} for k, v in sorted(agg.items(), key=lambda kv: -kv[1][1])]
 
 
+def whole_stats(trace_dir, prefix, patterns):
+    """Per-kernel statistics over the whole kernel trace for the kernels whose
+    short name contains one of `patterns` (kernels the run launches outside
+    the timed region, e.g. the other residual paths it compares)."""
+    agg = {}
+    with open(os.path.join(trace_dir, f"{prefix}_kernel_trace.csv")) as f:
+        for r in csv.DictReader(f):
+            k = short(r["Kernel_Name"])
+            if not any(p in k for p in patterns):
+                continue
+            d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            a = agg.setdefault(r["Kernel_Name"], [0, 0, d, d])
+            a[0] += 1
+            a[1] += d
+            a[2] = min(a[2], d)
+            a[3] = max(a[3], d)
+    return [{"Name": k, "Calls": v[0], "AverageNs": v[1] / v[0], "MinNs": v[2], "MaxNs": v[3]}
+            for k, v in sorted(agg.items(), key=lambda kv: -kv[1][1])]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--name", required=True)
@@ -71,6 +91,8 @@ def main():
     ap.add_argument("--roi", default=None,
                     help="a marker range (rocprofv3 --marker-trace): kernel statistics over the dispatches that run "
                          "inside it, from the kernel trace, instead of the whole-run statistics")
+    ap.add_argument("--also", action="append", default=[],
+                    help="with --roi: also list, over the whole trace, the kernels whose name contains this text")
     ap.add_argument("--sq", default=None, help="PMC run with SQ_INSTS_VALU and SQ_WAVES (optional)")
     ap.add_argument("--simds", type=int, default=1024, help="SIMDs of the GPU (MI355X: 256 CUs x 4)")
     ap.add_argument("--clock-ghz", type=float, default=2.4, help="peak engine clock (MI355X: 2.4 GHz)")
@@ -95,6 +117,14 @@ def main():
             continue
         lines.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                      f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} % |")
+    others = whole_stats(a.trace, a.prefix, a.also) if (a.roi and a.also) else []
+    if others:
+        lines += ["", "Kernels of the same run outside the timed region (whole trace): "
+                  + ", ".join(f"`{p}`" for p in a.also) + ".", "",
+                  "| kernel | calls | average µs | min µs | max µs |", "|---|---|---|---|---|"]
+        for r in others:
+            lines.append(f"| `{short(r['Name'])}` | {r['Calls']} | {r['AverageNs'] / 1e3:.1f} | "
+                         f"{r['MinNs'] / 1e3:.1f} | {r['MaxNs'] / 1e3:.1f} |")
     out = None
     if a.fetch and a.write:
         fetch = counters(os.path.join(a.fetch, f"{a.prefix}_counter_collection.csv"), "FETCH_SIZE")
@@ -124,7 +154,8 @@ def main():
         sqf = os.path.join(a.sq, f"{a.prefix}_counter_collection.csv")
         valu = counters(sqf, "SQ_INSTS_VALU")
         waves = counters(sqf, "SQ_WAVES")
-        avg_us = {short(r["Name"]): float(r["AverageNs"]) / 1e3 for r in stats}
+        avg_us = {short(r["Name"]): float(r["AverageNs"]) / 1e3 for r in others}
+        avg_us.update({short(r["Name"]): float(r["AverageNs"]) / 1e3 for r in stats})
         lines += ["", "## VALU issue load (PMC)", "",
                   f"busy = 4 cycles x SQ_INSTS_VALU / ({a.simds} SIMDs x {a.clock_ghz} GHz x average duration).", "",
                   "| kernel | SQ_WAVES | SQ_INSTS_VALU | VALU instr / wave | average µs | VALU busy |",

@@ -230,3 +230,34 @@ def test_newton_restores_update_mem_after_an_error(gpu):
     ctx.synchronize()
     ep, _, mt, _ = ctx.get_state()
     assert (mt != 0).any() and ep.any()  # UpdateMem was still on
+
+
+def test_newton_monitor_reports_every_iteration(gpu):
+    """ipf_set_newton_monitor: one call per residual norm, iteration 0 at the
+    entry u (rel 1), the last one equal to the returned statistics -- for
+    ipf_newton and for ipf_newton_segmented over two equation blocks."""
+    import torch
+    m, sm, ctx, B, ix, mask = _setup((16, 12, 2, 1.1))
+    neq = sm.sizes()["neq"]
+    ctx.set_rhs_linear(ix["rhs_linear"] * (wb.normal_stress(3) / wb.normal_stress(wb.NSTEPS)))
+    for segmented in (False, True):
+        ctx.set_state(*[np.zeros_like(a) for a in ctx.get_state()[:2]])
+        calls = []
+        ctx.set_newton_monitor(lambda it, rel, rn, cg: calls.append((it, rel, rn, cg)))
+        u = torch.zeros(neq, dtype=torch.float64, device=gpu)
+        if segmented:
+            st = ctx.newton_segmented(u, [0, neq // 2, neq], rtol=1e-10, maxit=80, cg_rtol=1e-13, cg_maxit=40000)
+        else:
+            st = ctx.newton(u, rtol=1e-10, maxit=80, cg_rtol=1e-13, cg_maxit=40000)
+        ctx.set_newton_monitor(None)
+        its = [c[0] for c in calls]
+        print(f"segmented={segmented}: {len(calls)} reports, final rel {calls[-1][1]:.2e}, stats {st['rel_residual']:.2e}")
+        assert its == list(range(int(st["iterations"]) + 1))
+        assert calls[0][1] == 1.0 and calls[0][3] == 0
+        assert calls[-1][1] == st["rel_residual"] and calls[-1][2] == st.get("rn", st.get("r"))
+        assert calls[-1][3] == int(st["cg_iterations"])
+        assert all(a[3] <= b[3] for a, b in zip(calls, calls[1:]))
+    # off: newton runs without reports
+    calls.clear()
+    ctx.newton(u, rtol=1e-10, maxit=2, cg_rtol=1e-13, cg_maxit=40000)
+    assert not calls
