@@ -200,7 +200,6 @@ struct ipf_ctx {
     double* dPpartial = nullptr;
     int* dPfixDof = nullptr;
     int* dPfixPtr = nullptr;
-    int* dPfixSlot = nullptr;
     int* dPorphan = nullptr;
     // material table (index = element material id) and the per-element ids
     // that the next ipf_transfer_data groups by
@@ -351,7 +350,6 @@ void release_patch(ipf_ctx* ctx) {
     dev_free(ctx->dPpartial);
     dev_free(ctx->dPfixDof);
     dev_free(ctx->dPfixPtr);
-    dev_free(ctx->dPfixSlot);
     dev_free(ctx->dPorphan);
     ctx->patch = PatchHost();
     ctx->patch_built = false;
@@ -545,7 +543,6 @@ int ensure_patch(ipf_ctx* ctx) {
         (rc = dev_alloc(ctx, &ctx->dPpartial, 2 * std::max<int64_t>(1, h.npartial))) ||
         (rc = dev_alloc(ctx, &ctx->dPfixDof, (int64_t)h.fix_dof.size())) ||
         (rc = dev_alloc(ctx, &ctx->dPfixPtr, (int64_t)h.fix_ptr.size())) ||
-        (rc = dev_alloc(ctx, &ctx->dPfixSlot, (int64_t)h.fix_slot.size())) ||
         (rc = dev_alloc(ctx, &ctx->dPorphan, (int64_t)h.orphan.size())) ||
         (rc = up(ctx->dPtmeta, h.tmeta.data(), sizeof(int) * h.tmeta.size())) ||
         (rc = up(ctx->dPlnode, h.lnode.data(), sizeof(unsigned short) * h.lnode.size())) ||
@@ -557,7 +554,6 @@ int ensure_patch(ipf_ctx* ctx) {
         (rc = up(ctx->dPptmap, h.pt_map.data(), sizeof(int64_t) * h.npts)) ||
         (rc = up(ctx->dPfixDof, h.fix_dof.data(), sizeof(int) * h.fix_dof.size())) ||
         (rc = up(ctx->dPfixPtr, h.fix_ptr.data(), sizeof(int) * h.fix_ptr.size())) ||
-        (rc = up(ctx->dPfixSlot, h.fix_slot.data(), sizeof(int) * h.fix_slot.size())) ||
         (rc = up(ctx->dPorphan, h.orphan.data(), sizeof(int) * h.orphan.size()))) {
         release_patch(ctx);
         return rc;
@@ -623,7 +619,7 @@ int patch_residual(ipf_ctx* ctx, const double* u, double* r, hipEvent_t* ev = nu
         launch_residual_patch(R.shape, fast, p, u, r, ctx->dFlin, ctx->dPpartial, ctx->stream);
         if (ev) IPF_HIP(ctx, hipEventRecord(ev[iev++], ctx->stream));
     }
-    launch_patch_fixup((int64_t)h.fix_dof.size(), ctx->dPfixDof, ctx->dPfixPtr, ctx->dPfixSlot, ctx->dPpartial,
+    launch_patch_fixup((int64_t)h.fix_dof.size(), ctx->dPfixDof, ctx->dPfixPtr, ctx->dPpartial,
                        (int64_t)h.orphan.size(), ctx->dPorphan, ctx->dFlin, r, ctx->stream);
     if (ev) IPF_HIP(ctx, hipEventRecord(ev[iev], ctx->stream));
     return launch_status(ctx, "patch residual");

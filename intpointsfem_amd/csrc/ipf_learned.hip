@@ -314,20 +314,27 @@ __global__ void k_mlp_refresh(const float* __restrict__ par, short* __restrict__
 
 // GELU as x * sigmoid(2 y), y = k0 (x + k1 x^3) -- the tanh form above
 // rewritten (0.5 (1 + tanh y) = sigmoid(2 y)) -- with the hardware exp2 and
-// reciprocal: a few ulps of FP32 from gelu(), far below the BF16 rounding
-// the activations get next.
+// reciprocal and one fused multiply-add: exp(-2 y) = exp2(x (c0 + c1 x^2)),
+// c0 = -2 log2(e) k0, c1 = c0 k1.  A few ulps of FP32 from gelu(), far below
+// the BF16 rounding the activations get next.  Five full-rate and two
+// quarter-rate VALU instructions: the inference kernel's main cost.
 __device__ __forceinline__ float gelu_fast(float x) {
-    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-    const float y = k0 * x * (1.f + k1 * x * x);
-    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * y);  // exp(-2 y)
+    constexpr float c0 = -2.8853900817779268f * 0.7978845608028654f;
+    constexpr float c1 = c0 * 0.044715f;
+    const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(x * x, c1, c0));
     return x * __builtin_amdgcn_rcpf(1.f + e);
 }
 
+typedef __bf16 bf16x2_hw __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+// Four FP32 values to BF16 with two v_cvt_pk_bf16_f32 (round to nearest
+// even: f2bf's result for every finite value).
 __device__ __forceinline__ bf16x4 to_bf16x4(f32x4 v) {
-    bf16x4 b;
-#pragma unroll
-    for (int i = 0; i < 4; i++) b[i] = f2bf(v[i]);
-    return b;
+    const s16x2 lo = __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{v[0], v[1]}, bf16x2_hw));
+    const s16x2 hi = __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{v[2], v[3]}, bf16x2_hw));
+    return bf16x4{lo.x, lo.y, hi.x, hi.y};
 }
 
 // Inference over n samples, 16 per wavefront step, every layer computed

@@ -34,8 +34,11 @@
 
 namespace ipf {
 
-// Wavefront tiles per patch: one workgroup of patch_waves * 64 lanes.
-constexpr int patch_waves(int s) { return s == SHAPE_Q3 ? 6 : 8; }
+// Wavefront tiles per patch: one workgroup of patch_waves * 64 lanes.  Four
+// measured faster than 8 (Q3: 6) on MI355X at every shape: a workgroup
+// waits at its barriers for its slowest wave's return map, and smaller
+// workgroups (less LDS each) leave more of them resident per CU.
+constexpr int patch_waves(int) { return 4; }
 constexpr int patch_elems(int s) { return patch_waves(s) * shape_et(s); }
 constexpr int patch_max_nodes(int s) { return patch_elems(s) * (shape_ce(s) / 2); }
 
@@ -83,7 +86,8 @@ struct PatchHost {
     std::vector<int> pnode, pslot, cbeg;   // per patch-node (cbeg: + 1 sentinel)
     std::vector<unsigned short> contrib;
     std::vector<int64_t> pt_map;           // per patch point: colored new-point index, -1 padding
-    std::vector<int> fix_dof, fix_ptr, fix_slot;  // shared nodes: x DoF, partial slots in fixed order
+    std::vector<int> fix_dof, fix_ptr, fix_slot;  // shared nodes by x DoF: partial slots [fix_ptr[i], fix_ptr[i+1])
+                                                  // (fix_slot: the identity, slots numbered in this order)
     std::vector<int> orphan;               // DoFs of no element
     int64_t npartial = 0;
     int64_t ntiles = 0, nslots = 0, npts = 0;
@@ -97,9 +101,8 @@ std::string build_patches(const std::vector<PatchElemIn>& el, const std::vector<
 // Kernels (ipf_kernels_fmt.hip: the residual; ipf_patch.hip: the rest).
 void launch_residual_patch(int shape, bool fast, const PatchDev& p, const double* u, double* r, const double* flin,
                            double* partial, hipStream_t stream);
-void launch_patch_fixup(int64_t nshared, const int* fix_dof, const int* fix_ptr, const int* fix_slot,
-                        const double* partial, int64_t norphan, const int* orphan, const double* flin, double* r,
-                        hipStream_t stream);
+void launch_patch_fixup(int64_t nshared, const int* fix_dof, const int* fix_ptr, const double* partial,
+                        int64_t norphan, const int* orphan, const double* flin, double* r, hipStream_t stream);
 // ep_patch[k][q] = ep[k][pt_map[q]] (0 for padding), k = 0..3
 void launch_patch_state(int64_t npts, const int64_t* pt_map, const double* const* ep, double* const* ep_patch,
                         hipStream_t stream);

@@ -21,17 +21,18 @@ uint32_t spread16(uint32_t v) {
     return v;
 }
 
+// Shared node i: r = f_lin + (-(sum of its partials in patch order)); its
+// partials are the consecutive slots [fix_ptr[i], fix_ptr[i + 1]).
 __global__ void k_patch_fixup(int64_t nshared, const int* __restrict__ fix_dof, const int* __restrict__ fix_ptr,
-                              const int* __restrict__ fix_slot, const double* __restrict__ partial, int64_t norphan,
-                              const int* __restrict__ orphan, const double* __restrict__ flin,
-                              double* __restrict__ r) {
+                              const double* __restrict__ partial, int64_t norphan, const int* __restrict__ orphan,
+                              const double* __restrict__ flin, double* __restrict__ r) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nshared) {
         double fx = 0.0, fy = 0.0;
-        for (int k = fix_ptr[i]; k < fix_ptr[i + 1]; k++) {
-            const int s = fix_slot[k];
-            fx = fx + partial[2 * (int64_t)s];
-            fy = fy + partial[2 * (int64_t)s + 1];
+        const int k1 = fix_ptr[i + 1];
+        for (int k = fix_ptr[i]; k < k1; k++) {
+            fx = fx + partial[2 * (int64_t)k];
+            fy = fy + partial[2 * (int64_t)k + 1];
         }
         const int d = fix_dof[i];
         r[d] = flin[d] + (-fx);
@@ -207,16 +208,26 @@ std::string build_patches(const std::vector<PatchElemIn>& el, const std::vector<
     out.cbeg.push_back((int)out.contrib.size());
     if (out.contrib.size() >= (size_t)0x7fffffff || out.npartial >= 0x7fffffff)
         return "patch layout: too many contributions";
-    // shared nodes: partial slots in patch order
+    // shared nodes by x DoF, each node's partials in patch order; the slots
+    // are renumbered in that order, so that a shared node's partials are
+    // consecutive (fix_slot is the identity) and the fix-up reads them
+    // without an indirection
     std::sort(shared.begin(), shared.end());
+    std::vector<int> renum(out.npartial, -1);
     out.fix_ptr.push_back(0);
     for (size_t k = 0; k < shared.size();) {
         size_t m = k;
-        while (m < shared.size() && shared[m].first == shared[k].first) out.fix_slot.push_back(shared[m++].second);
+        while (m < shared.size() && shared[m].first == shared[k].first) {
+            renum[shared[m].second] = (int)out.fix_slot.size();
+            out.fix_slot.push_back((int)out.fix_slot.size());
+            m++;
+        }
         out.fix_dof.push_back(shared[k].first);
         out.fix_ptr.push_back((int)out.fix_slot.size());
         k = m;
     }
+    for (int& ps : out.pslot)
+        if (ps >= 0) ps = renum[ps];
     std::vector<unsigned char> covered(neq, 0);
     for (int64_t e = 0; e < ne; e++) {
         const int nsh = shape_ce(el[e].shape) / 2;
@@ -230,13 +241,12 @@ std::string build_patches(const std::vector<PatchElemIn>& el, const std::vector<
     return "";
 }
 
-void launch_patch_fixup(int64_t nshared, const int* fix_dof, const int* fix_ptr, const int* fix_slot,
-                        const double* partial, int64_t norphan, const int* orphan, const double* flin, double* r,
-                        hipStream_t s) {
+void launch_patch_fixup(int64_t nshared, const int* fix_dof, const int* fix_ptr, const double* partial,
+                        int64_t norphan, const int* orphan, const double* flin, double* r, hipStream_t s) {
     const int64_t n = nshared + norphan;
     if (n == 0) return;
     hipLaunchKernelGGL(k_patch_fixup, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nshared, fix_dof, fix_ptr,
-                       fix_slot, partial, norphan, orphan, flin, r);
+                       partial, norphan, orphan, flin, r);
 }
 
 void launch_patch_state(int64_t n, const int64_t* pt_map, const double* const* ep, double* const* ep_patch,

@@ -27,7 +27,7 @@ from intpointsfem_amd import wellbore as wb
 SHAPES = {(4, 8): 0, (9, 18): 1, (16, 32): 2, (3, 6): 3, (6, 12): 4, (12, 20): 5}
 NIP = [4, 9, 16, 3, 6, 12]
 CE = [8, 18, 32, 6, 12, 20]
-WAVES = [8, 8, 6, 8, 8, 8]  # patch_waves
+WAVES = [4, 4, 4, 4, 4, 4]  # patch_waves
 
 
 def et(s):
@@ -77,6 +77,7 @@ def check(meshes):
         np.add.at(r_ref, nodes[noff[e]:noff[e + 1]] + 1, force[e][1::2])
     r = np.full(neq, np.nan)
     partial = np.full((max(L["npartial"], 1), 2), np.nan)
+    writer = np.full(max(L["npartial"], 1), -1, dtype=np.int64)  # the patch that writes each partial slot
     for (sh, npch, patch0, tile0, slot0, pt0, ln0) in L["ranges"]:
         NW, ET, NSH, TP = WAVES[sh], et(sh), CE[sh] // 2, et(sh) * NIP[sh]
         for p in range(npch):
@@ -132,10 +133,15 @@ def check(meshes):
                     r[d], r[d + 1] = fx, fy
                 else:
                     partial[L["pslot"][k]] = (fx, fy)
+                    writer[L["pslot"][k]] = P
     assert (seen_elem == 1).all() and (seen_pt == 1).all()
     slots_seen = np.zeros(max(L["npartial"], 1), dtype=np.int64)
+    # a shared node's partials are consecutive slots, in patch order, nodes by DoF
+    assert np.array_equal(L["fix_slot"], np.arange(len(L["fix_slot"])))
+    assert (np.diff(L["fix_dof"]) > 0).all()
     for i, d in enumerate(L["fix_dof"]):
         sl = L["fix_slot"][L["fix_ptr"][i]:L["fix_ptr"][i + 1]]
+        assert len(sl) >= 2 and (np.diff(writer[sl]) > 0).all()
         slots_seen[sl] += 1
         assert np.isnan(r[d])
         r[d] = partial[sl, 0].sum()
