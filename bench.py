@@ -331,8 +331,9 @@ def learned_mixed_batched(args, dev):
       * learned + mixed: the predictor (trained on this state's pairs) orders
         each color's / patch's elements so that predicted-elastic elements
         share tiles, and the certified-elastic path skips the spectral
-        decomposition of every point it proves elastic; the assembly
-        (colored or patch) is the faster one at this state;
+        decomposition of every point it proves elastic; the tile shape
+        (colored tiles or patches of 2 / 4 / 8 wavefront tiles) is the one
+        ipf_autotune_assembly picks at this state;
       * deviation from the plain residual, and from the CPU oracle on whole
         instances within --cpu-seconds.
     Returns (dict for the JSON line, batch, u, plain_ms)."""
@@ -430,15 +431,12 @@ def learned_mixed_batched(args, dev):
     B.relayout(keys)
     ctx.set_stream(stream.cuda_stream)
     relayout_s = time.time() - t2
-    best = None
-    for asm in ("colored", "patch"):
-        ctx.set_residual_assembly(asm)
-        ctx.set_residual_mode("certified-elastic")
-        tms = timed(f"{asm}/certified-elastic, predicted order")
-        if best is None or tms < best[1]:
-            best = (asm, tms)
-    ctx.set_residual_assembly(best[0])
+    # the tile shape: colored tiles or patches of 2 / 4 / 8 wavefront tiles, by the autotuner
     ctx.set_residual_mode("certified-elastic")
+    tune = ctx.autotune_assembly(u, reps=max(5, min(args.steps, 50)))
+    log(f"[bench] batch64 assembly autotune (certified-elastic, predicted order): "
+        f"{', '.join(f'{k} {v:.4f} ms' for k, v in tune['times_ms'].items())} -> {tune['selected']}")
+    best = (tune["selected"], timed(f"{tune['selected']}/certified-elastic, predicted order"))
     settle(ctx, u, r, args.settle_ms)
     lm_total = time_residuals(ctx, u, r, stream, args.steps, args.warmup, roi="timed learned+mixed residuals")
     lm_ms = lm_total / args.steps
@@ -486,7 +484,8 @@ def learned_mixed_batched(args, dev):
            "speedup_vs_plain": round(plain_ms / lm_ms, 3),
            "plain": {"ms_per_step": round(plain_ms, 5), "value": round(npts * args.steps / (plain_total / 1e3) / 1e6, 2),
                      "path": "colored assembly, exact return map, index order (the drop-in's residual)"},
-           "path": {"assembly": best[0], "residual_mode": "certified-elastic", "order": "predicted branch"},
+           "path": {"assembly": best[0], "residual_mode": "certified-elastic", "order": "predicted branch",
+                    "assembly_autotune_ms": {k: round(v, 5) for k, v in tune["times_ms"].items()}},
            "max_dev_vs_plain": dev_plain, "max_dev_vs_oracle": oracle_dev, "oracle_instances_checked": checked,
            "int_points": npts, "neq": neq, "newton": newton,
            "branches": {n: int((br == i).sum()) for i, n in enumerate(("elastic", "main", "right_edge", "left_edge",

@@ -209,11 +209,29 @@ int ipf_get_residual_assembly(const ipf_ctx *ctx, int *assembly);
  * nodes, DoFs of no element, patch points (padded), element shapes. */
 int ipf_patch_info(const ipf_ctx *ctx, int64_t out[8]);
 
+/* Wavefront tiles per patch of IPF_ASSEMBLY_PATCH: 2, 4 (default) or 8
+ * (one workgroup of 64 * waves lanes per patch); a change drops the built
+ * patch layout (rebuilt at once when the patch assembly is selected).  No
+ * reference counterpart. */
+int ipf_set_patch_waves(ipf_ctx *ctx, int waves);
+int ipf_get_patch_waves(const ipf_ctx *ctx, int *waves);
+
+/* Autotuner of the non-committing residual's tile shape: times reps
+ * residuals at u_dev (current stress mode and B format, committed state
+ * untouched) with the colored assembly (one wavefront tile per workgroup)
+ * and, where the setup allows patches, with patches of 2, 4 and 8 wavefront
+ * tiles, and keeps the fastest (ipf_get_residual_assembly /
+ * ipf_get_patch_waves tell which).  times_ms (optional) [4]: colored, patch
+ * 2, 4, 8; -1 where not timed.  The assemblies differ in the order of the
+ * per-DoF sums only (a few ulps).  No reference counterpart. */
+int ipf_autotune_assembly(ipf_ctx *ctx, const double *u_dev, int reps, double *times_ms);
+
 /* The patch layout of IPF_ASSEMBLY_PATCH planned on the host (no GPU), for
  * inspection and tests: n elements with their shape (kernel shape index:
  * 0..2 Q1..Q3, 3..5 T1..T3), material, first integration point, centroid
  * and node x-DoFs (shape_ce / 2 per element, concatenated; y DoF = x + 1),
- * neq equations.  sizes[12] = ranges, patches, tiles, slots, local node
+ * neq equations, waves wavefront tiles per patch (2, 4 or 8).  sizes[12] =
+ * ranges, patches, tiles, slots, local node
  * entries, patch nodes, contributions, patch points, shared nodes, partial
  * slot entries of the shared nodes, orphan DoFs, partial slots.  Each output
  * array may be NULL (call once with NULL outputs for the sizes): ranges
@@ -222,10 +240,13 @@ int ipf_patch_info(const ipf_ctx *ctx, int64_t out[8]);
  * (element index, -1 padding); lnode [local node entries]; node_off
  * [patches + 1]; pnode, pslot [patch nodes]; cbeg [patch nodes + 1];
  * contrib [contributions]; pt_map [patch points] (the element's first point
- * + point, -1 padding); fix_dof [shared], fix_ptr [shared + 1], fix_slot;
- * orphan [orphan DoFs]. */
+ * + point, -1 padding); fix_dof [shared] (ascending), fix_ptr [shared + 1],
+ * fix_slot (the identity: shared node i owns the partial slots
+ * [fix_ptr[i], fix_ptr[i + 1]), one per patch, in patch order); orphan
+ * [orphan DoFs]. */
 int ipf_patch_plan_host(int64_t n, const int *shape, const int *mat, const int64_t *first_point, const double *cx,
-                        const double *cy, const int *node_xdof, int64_t neq, int64_t *sizes, int64_t *ranges,
+                        const double *cy, const int *node_xdof, int64_t neq, int waves, int64_t *sizes,
+                        int64_t *ranges,
                         int *tmeta, int64_t *slot_elem, unsigned short *lnode, int64_t *node_off, int *pnode,
                         int *pslot, int *cbeg, unsigned short *contrib, int64_t *pt_map, int *fix_dof, int *fix_ptr,
                         int *fix_slot, int *orphan);

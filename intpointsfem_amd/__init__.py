@@ -115,8 +115,11 @@ def lib():
         "ipf_set_residual_assembly": (_int, [_vp, _int]),
         "ipf_get_residual_assembly": (_int, [_vp, _ip]),
         "ipf_patch_info": (_int, [_vp, _lp]),
-        "ipf_patch_plan_host": (_int, [_i64, _ip, _ip, _lp, _dp, _dp, _ip, _i64, _lp, _lp, _ip, _lp, _vp, _lp,
-                                       _ip, _ip, _ip, _vp, _lp, _ip, _ip, _ip, _ip]),
+        "ipf_patch_plan_host": (_int, [_i64, _ip, _ip, _lp, _dp, _dp, _ip, _i64, _int, _lp, _lp, _ip, _lp, _vp,
+                                       _lp, _ip, _ip, _ip, _vp, _lp, _ip, _ip, _ip, _ip]),
+        "ipf_set_patch_waves": (_int, [_vp, _int]),
+        "ipf_get_patch_waves": (_int, [_vp, _ip]),
+        "ipf_autotune_assembly": (_int, [_vp, _vp, _int, _dp]),
         "ipf_newton_segmented": (_int, [_vp, _vp, _int, _lp, _d, _int, _d, _int, _dp, _dp, _ip]),
         "ipf_autotune": (_int, [_vp, _vp, _int, _int, _dp]),
         "ipf_get_state": (_int, [_vp, _dp, _dp, _ip, _ip]),
@@ -175,8 +178,9 @@ def lib():
     return L
 
 
-def patch_plan(shape, mat, first_point, cx, cy, node_xdof, neq):
-    """The patch layout planned on the host (ipf_patch_plan_host): a dict of numpy arrays."""
+def patch_plan(shape, mat, first_point, cx, cy, node_xdof, neq, waves=4):
+    """The patch layout planned on the host (ipf_patch_plan_host), patches of
+    `waves` wavefront tiles: a dict of numpy arrays."""
     n = len(shape)
     sh = np.ascontiguousarray(shape, dtype=np.int32)
     mt = np.ascontiguousarray(mat, dtype=np.int32)
@@ -185,7 +189,8 @@ def patch_plan(shape, mat, first_point, cx, cy, node_xdof, neq):
     y = np.ascontiguousarray(cy, dtype=np.float64)
     nd = np.ascontiguousarray(node_xdof, dtype=np.int32)
     sizes = np.zeros(12, dtype=np.int64)
-    args = [n, _p(sh, _ip), _p(mt, _ip), _p(fp, _lp), _p(x, _dp), _p(y, _dp), _p(nd, _ip), int(neq), _p(sizes, _lp)]
+    args = [n, _p(sh, _ip), _p(mt, _ip), _p(fp, _lp), _p(x, _dp), _p(y, _dp), _p(nd, _ip), int(neq), int(waves),
+            _p(sizes, _lp)]
     _check(lib().ipf_patch_plan_host(*args, *([None] * 14)), what="ipf_patch_plan_host")
     nr, npch, nt, ns, nl, npn, nc, npt, nsh, nfs, nor, npart = (int(v) for v in sizes)
     out = dict(ranges=np.zeros(7 * nr, np.int64), tmeta=np.zeros(nt, np.int32), slot_elem=np.zeros(ns, np.int64),
@@ -607,6 +612,26 @@ class DeviceContext:
         self._c(lib().ipf_patch_info(self._h, _p(out, _lp)), "ipf_patch_info")
         keys = ("built", "patches", "patch_nodes", "partials", "shared_nodes", "orphan_dofs", "points", "shapes")
         return dict(zip(keys, (int(v) for v in out)))
+
+    def set_patch_waves(self, waves):
+        """Wavefront tiles per patch of the patch assembly: 2, 4 or 8."""
+        self._c(lib().ipf_set_patch_waves(self._h, int(waves)), "ipf_set_patch_waves")
+
+    def patch_waves(self):
+        m = ctypes.c_int()
+        self._c(lib().ipf_get_patch_waves(self._h, ctypes.byref(m)), "ipf_get_patch_waves")
+        return m.value
+
+    def autotune_assembly(self, u, reps=10):
+        """Times the non-committing residual with the colored assembly and
+        with patches of 2, 4 and 8 wavefront tiles (where available) and
+        keeps the fastest.  Returns {candidate: ms} and the selection."""
+        t = np.zeros(4)
+        self._c(lib().ipf_autotune_assembly(self._h, _ptr(u), int(reps), _p(t, _dp)), "ipf_autotune_assembly")
+        names = ("colored", "patch-2", "patch-4", "patch-8")
+        times = {n: float(v) for n, v in zip(names, t) if v >= 0}
+        sel = self.residual_assembly()
+        return {"times_ms": times, "selected": sel if sel == "colored" else f"patch-{self.patch_waves()}"}
 
     def set_newton_atol(self, atol):
         self._c(lib().ipf_set_newton_atol(self._h, float(atol)), "ipf_set_newton_atol")

@@ -185,6 +185,7 @@ struct ipf_ctx {
     std::vector<PatchElemIn> patch_in;
     std::vector<int> patch_in_nodes;
     bool patch_built = false;
+    int patch_nw = kPatchWavesDefault;  // wavefront tiles per patch of the next build
     PatchHost patch;
     uint64_t patch_state_version = 0;
     int* dPtmeta = nullptr;
@@ -518,7 +519,7 @@ int ensure_patch(ipf_ctx* ctx) {
                     "patch assembly needs the geometric setup (ipf_transfer_geometry) with every element's DoFs "
                     "in (x, y) node pairs");
     PatchHost& h = ctx->patch;
-    const std::string err = build_patches(ctx->patch_in, ctx->patch_in_nodes, ctx->neq, h);
+    const std::string err = build_patches(ctx->patch_in, ctx->patch_in_nodes, ctx->neq, ctx->patch_nw, h);
     if (!err.empty()) {
         ctx->patch = PatchHost();
         return fail(ctx, IPF_EUNSUPPORTED, "%s", err.c_str());
@@ -616,7 +617,7 @@ int patch_residual(ipf_ctx* ctx, const double* u, double* r, hipEvent_t* ev = nu
         p.tab = ctx->dTab + ctx->tab_off[R.shape];
         (void)ET;
         (void)TP;
-        launch_residual_patch(R.shape, fast, p, u, r, ctx->dFlin, ctx->dPpartial, ctx->stream);
+        launch_residual_patch(R.shape, h.nw, fast, p, u, r, ctx->dFlin, ctx->dPpartial, ctx->stream);
         if (ev) IPF_HIP(ctx, hipEventRecord(ev[iev++], ctx->stream));
     }
     launch_patch_fixup((int64_t)h.fix_dof.size(), ctx->dPfixDof, ctx->dPfixPtr, ctx->dPpartial,
@@ -1425,6 +1426,97 @@ int ipf_set_residual_assembly(ipf_ctx* ctx, int assembly) {
     return IPF_OK;
 }
 
+int ipf_set_patch_waves(ipf_ctx* ctx, int waves) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_patch_waves: NULL context");
+    if (!patch_waves_ok(waves)) return fail(ctx, IPF_EINVAL, "ipf_set_patch_waves: %d is not 2, 4 or 8", waves);
+    if (waves == ctx->patch_nw) return IPF_OK;
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    release_patch(ctx);
+    ctx->patch_nw = waves;
+    // the selected patch assembly is rebuilt now, so that its errors show here
+    return ctx->assembly == IPF_ASSEMBLY_PATCH ? ensure_patch(ctx) : IPF_OK;
+}
+
+int ipf_get_patch_waves(const ipf_ctx* ctx, int* waves) {
+    if (!ctx || !waves) return fail(nullptr, IPF_EINVAL, "ipf_get_patch_waves: NULL argument");
+    *waves = ctx->patch_nw;
+    return IPF_OK;
+}
+
+int ipf_autotune_assembly(ipf_ctx* ctx, const double* u_dev, int reps, double* times_ms) {
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!u_dev) return fail(ctx, IPF_EINVAL, "ipf_autotune_assembly: u is NULL");
+    if (reps < 1 || reps > 10000)
+        return fail(ctx, IPF_EINVAL, "ipf_autotune_assembly: reps %d outside [1, 10000]", reps);
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    if (times_ms)
+        for (int k = 0; k < 4; k++) times_ms[k] = -1.0;
+    const bool commit = ctx->update_mem;
+    ctx->update_mem = false;  // non-committing residuals only: the committed state must not change
+    const int a0 = ctx->assembly, nw0 = ctx->patch_nw;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        ctx->update_mem = commit;
+        return fail(ctx, IPF_EHIP, "ipf_autotune_assembly: event creation failed");
+    }
+    auto time_it = [&](double* ms) -> int {
+        int r = IPF_OK;
+        for (int k = 0; k < 2 && r == IPF_OK; k++) r = ipf_residual(ctx, u_dev, ctx->dR);
+        if (r) return r;
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) return fail(ctx, IPF_EHIP, "autotune: event record");
+        for (int k = 0; k < reps && r == IPF_OK; k++) r = ipf_residual(ctx, u_dev, ctx->dR);
+        if (r) return r;
+        float t = 0.f;
+        if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+            return fail(ctx, IPF_EHIP, "autotune: event timing");
+        *ms = (double)t / reps;
+        return IPF_OK;
+    };
+    // candidates: colored (one wavefront tile per workgroup), then patches of
+    // 2, 4 and 8 wavefront tiles where the setup allows patches
+    int best_a = IPF_ASSEMBLY_COLORED, best_nw = nw0;
+    double best = 1e300, ms = 0.0;
+    ctx->assembly = IPF_ASSEMBLY_COLORED;
+    rc = time_it(&ms);
+    if (rc == IPF_OK) {
+        if (times_ms) times_ms[0] = ms;
+        best = ms;
+    }
+    const int cand[3] = {2, 4, 8};
+    for (int k = 0; k < 3 && rc == IPF_OK && !ctx->patch_in.empty(); k++) {
+        if (cand[k] != ctx->patch_nw) {
+            release_patch(ctx);
+            ctx->patch_nw = cand[k];
+        }
+        ctx->assembly = IPF_ASSEMBLY_PATCH;
+        if ((rc = ensure_patch(ctx))) break;
+        if ((rc = time_it(&ms))) break;
+        if (times_ms) times_ms[1 + k] = ms;
+        if (ms < best) best = ms, best_a = IPF_ASSEMBLY_PATCH, best_nw = cand[k];
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->update_mem = commit;
+    const int fa = rc ? a0 : best_a, fnw = rc ? nw0 : best_nw;
+    // keep the chosen layout built; a patch layout nobody selected goes
+    if (ctx->patch_nw != fnw || fa != IPF_ASSEMBLY_PATCH) {
+        (void)hipStreamSynchronize(ctx->stream);
+        release_patch(ctx);
+    }
+    ctx->patch_nw = fnw;
+    ctx->assembly = IPF_ASSEMBLY_COLORED;
+    if (fa == IPF_ASSEMBLY_PATCH) {
+        const int rb = ensure_patch(ctx);
+        if (rb == IPF_OK) ctx->assembly = IPF_ASSEMBLY_PATCH;
+        if (!rc) rc = rb;
+    }
+    return rc;
+}
+
 int ipf_get_residual_assembly(const ipf_ctx* ctx, int* assembly) {
     if (!ctx || !assembly) return fail(nullptr, IPF_EINVAL, "ipf_get_residual_assembly: NULL argument");
     *assembly = ctx->assembly;
@@ -1448,7 +1540,8 @@ int ipf_patch_info(const ipf_ctx* ctx, int64_t out[8]) {
 }
 
 int ipf_patch_plan_host(int64_t n, const int* shape, const int* mat, const int64_t* first_point, const double* cx,
-                        const double* cy, const int* node_xdof, int64_t neq, int64_t* sizes, int64_t* ranges,
+                        const double* cy, const int* node_xdof, int64_t neq, int waves, int64_t* sizes,
+                        int64_t* ranges,
                         int* tmeta, int64_t* slot_elem, unsigned short* lnode, int64_t* node_off, int* pnode,
                         int* pslot, int* cbeg, unsigned short* contrib, int64_t* pt_map, int* fix_dof, int* fix_ptr,
                         int* fix_slot, int* orphan) {
@@ -1468,7 +1561,7 @@ int ipf_patch_plan_host(int64_t n, const int* shape, const int* mat, const int64
         for (int a = 0; a < shape_ce(shape[e]) / 2; a++) nodes.push_back(node_xdof[el[e].node0 + a]);
     }
     PatchHost h;
-    const std::string err = build_patches(el, nodes, neq, h);
+    const std::string err = build_patches(el, nodes, neq, waves, h);
     if (!err.empty()) return fail(nullptr, IPF_EUNSUPPORTED, "%s", err.c_str());
     int64_t np = 0;
     for (const auto& R : h.ranges) np += R.npatch;

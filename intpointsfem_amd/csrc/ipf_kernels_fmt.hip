@@ -586,7 +586,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // The non-committing residual over the patch layout (ipf_patch.h): one
-// workgroup of patch_waves(SHAPE) wavefronts per patch.  Per point the
+// workgroup of NW wavefronts per patch.  Per point the
 // colored kernel's arithmetic (geometric FP64 B, compact strain sums, return
 // map or FAST's certified elastic path, B^T products); then per element the
 // column sums over its points (point order) into LDS, and per patch node
@@ -597,13 +597,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 // first contributions and u values it requests at the start, so that their
 // latency hides behind the points' work; the slots' local node indexes go to
 // LDS.  Two workgroup barriers: the nodes' u in LDS, the elements' sums in LDS.
-template <int SHAPE, bool FAST>
-__global__ void __launch_bounds__(kWave * patch_waves(SHAPE))
+template <int SHAPE, bool FAST, int NW>
+__global__ void __launch_bounds__(kWave * NW)
 k_residual_patch(PatchDev p, const double* __restrict__ u, double* __restrict__ r, const double* __restrict__ flin,
                  double* __restrict__ partial) {
     using C = FC<SHAPE, FMT_G64>;
-    constexpr int NW = patch_waves(SHAPE), ET = C::ET, NIP = C::NIP, CE = C::CE, TP = C::TP;
-    constexpr int NSH = CE / 2, PE = NW * ET, MAXN = patch_max_nodes(SHAPE), NT = NW * kWave;
+    constexpr int ET = C::ET, NIP = C::NIP, CE = C::CE, TP = C::TP;
+    constexpr int NSH = CE / 2, PE = NW * ET, MAXN = patch_max_nodes(SHAPE, NW), NT = NW * kWave;
     constexpr int H = CE / 2, H2 = CE - H;
     constexpr int MAXC = 4;  // contributions requested up front; more are read at the end
     static_assert(ET * H <= kWave && ET * H2 <= kWave, "one column sum per lane and half");
@@ -805,16 +805,27 @@ void launch_to_f32(int64_t n, const double* src, float* dst, hipStream_t s) {
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, src, dst);
 }
 
-void launch_residual_patch(int shape, bool fast, const PatchDev& p, const double* u, double* r, const double* flin,
-                           double* partial, hipStream_t s) {
+template <int S, int NW>
+void launch_patch_nw(bool fast, const PatchDev& p, const double* u, double* r, const double* flin, double* partial,
+                     hipStream_t s) {
+    const dim3 grid((unsigned)p.npatch), block(kWave * NW);
+    if (fast)
+        hipLaunchKernelGGL((k_residual_patch<S, true, NW>), grid, block, 0, s, p, u, r, flin, partial);
+    else
+        hipLaunchKernelGGL((k_residual_patch<S, false, NW>), grid, block, 0, s, p, u, r, flin, partial);
+}
+
+void launch_residual_patch(int shape, int nw, bool fast, const PatchDev& p, const double* u, double* r,
+                           const double* flin, double* partial, hipStream_t s) {
     if (p.npatch == 0) return;
     dispatch_shape(shape, [&](auto sc) {
         constexpr int S = decltype(sc)::value;
-        const dim3 grid((unsigned)p.npatch), block(kWave * patch_waves(S));
-        if (fast)
-            hipLaunchKernelGGL((k_residual_patch<S, true>), grid, block, 0, s, p, u, r, flin, partial);
+        if (nw == 2)
+            launch_patch_nw<S, 2>(fast, p, u, r, flin, partial, s);
+        else if (nw == 8)
+            launch_patch_nw<S, 8>(fast, p, u, r, flin, partial, s);
         else
-            hipLaunchKernelGGL((k_residual_patch<S, false>), grid, block, 0, s, p, u, r, flin, partial);
+            launch_patch_nw<S, 4>(fast, p, u, r, flin, partial, s);
     });
 }
 

@@ -6,7 +6,7 @@
 // is gathered and r read and written about four times per residual.  The
 // patch layout instead cuts the elements of each (shape, material) run into
 // compact patches -- consecutive elements of a Morton order of the element
-// centroids, patch_waves(shape) wavefront tiles each -- and runs one
+// centroids, nw wavefront tiles each -- and runs one
 // workgroup per patch in a single launch per shape:
 //   * the patch's distinct nodes are gathered once into LDS (u read once per
 //     node and patch);
@@ -34,13 +34,16 @@
 
 namespace ipf {
 
-// Wavefront tiles per patch: one workgroup of patch_waves * 64 lanes.  Four
-// measured faster than 8 (Q3: 6) on MI355X at every shape: a workgroup
-// waits at its barriers for its slowest wave's return map, and smaller
-// workgroups (less LDS each) leave more of them resident per CU.
-constexpr int patch_waves(int) { return 4; }
-constexpr int patch_elems(int s) { return patch_waves(s) * shape_et(s); }
-constexpr int patch_max_nodes(int s) { return patch_elems(s) * (shape_ce(s) / 2); }
+// Wavefront tiles per patch: one workgroup of nw * 64 lanes, nw = 2, 4 or 8
+// per context (ipf_set_patch_waves, or chosen by ipf_autotune_assembly).
+// Smaller patches share more nodes with their neighbours (more partial sums
+// for the fix-up); larger workgroups wait longer at their barriers for their
+// slowest wave's return map and hold more LDS each.  Four is the default:
+// the fastest at 800x800 Q2 and Q3 and on the 64-instance batch on MI355X.
+constexpr int kPatchWavesDefault = 4;
+constexpr bool patch_waves_ok(int nw) { return nw == 2 || nw == 4 || nw == 8; }
+constexpr int patch_elems(int s, int nw) { return nw * shape_et(s); }
+constexpr int patch_max_nodes(int s, int nw) { return patch_elems(s, nw) * (shape_ce(s) / 2); }
 
 // One real element of the colored layout, the input of the patch builder.
 struct PatchElemIn {
@@ -91,16 +94,17 @@ struct PatchHost {
     std::vector<int> orphan;               // DoFs of no element
     int64_t npartial = 0;
     int64_t ntiles = 0, nslots = 0, npts = 0;
+    int nw = kPatchWavesDefault;           // wavefront tiles per patch
 };
 
-// Builds the patches of the elements `el` (node x-DoFs in `nodes`), neq
-// equations.  Returns "" or an error text.
-std::string build_patches(const std::vector<PatchElemIn>& el, const std::vector<int>& nodes, int64_t neq,
+// Builds the patches of nw wavefront tiles of the elements `el` (node x-DoFs
+// in `nodes`), neq equations.  Returns "" or an error text.
+std::string build_patches(const std::vector<PatchElemIn>& el, const std::vector<int>& nodes, int64_t neq, int nw,
                           PatchHost& out);
 
 // Kernels (ipf_kernels_fmt.hip: the residual; ipf_patch.hip: the rest).
-void launch_residual_patch(int shape, bool fast, const PatchDev& p, const double* u, double* r, const double* flin,
-                           double* partial, hipStream_t stream);
+void launch_residual_patch(int shape, int nw, bool fast, const PatchDev& p, const double* u, double* r,
+                           const double* flin, double* partial, hipStream_t stream);
 void launch_patch_fixup(int64_t nshared, const int* fix_dof, const int* fix_ptr, const double* partial,
                         int64_t norphan, const int* orphan, const double* flin, double* r, hipStream_t stream);
 // ep_patch[k][q] = ep[k][pt_map[q]] (0 for padding), k = 0..3

@@ -67,9 +67,11 @@ __global__ void k_patch_geo(int64_t nslots, const int64_t* __restrict__ slot_geo
 
 }  // namespace
 
-std::string build_patches(const std::vector<PatchElemIn>& el, const std::vector<int>& nodes, int64_t neq,
+std::string build_patches(const std::vector<PatchElemIn>& el, const std::vector<int>& nodes, int64_t neq, int nw,
                           PatchHost& out) {
     out = PatchHost();
+    if (!patch_waves_ok(nw)) return "patch layout: wavefront tiles per patch must be 2, 4 or 8";
+    out.nw = nw;
     const int64_t ne = (int64_t)el.size();
     // uses of each node (keyed by its x DoF) over all elements
     std::vector<int> uses(neq, 0);
@@ -122,7 +124,7 @@ std::string build_patches(const std::vector<PatchElemIn>& el, const std::vector<
     int64_t i = 0;
     while (i < ne) {
         const int shape = el[idx[i]].shape;
-        const int NW = patch_waves(shape), ET = shape_et(shape), NIP = shape_nip(shape), TP = shape_tp(shape);
+        const int NW = nw, ET = shape_et(shape), NIP = shape_nip(shape), TP = shape_tp(shape);
         const int NSH = shape_ce(shape) / 2, PE = NW * ET;
         PatchHost::Range R;
         R.shape = shape;
@@ -147,7 +149,7 @@ std::string build_patches(const std::vector<PatchElemIn>& el, const std::vector<
             std::sort(local.begin(), local.end());
             local.erase(std::unique(local.begin(), local.end()), local.end());
             const int nn = (int)local.size();
-            if (nn > patch_max_nodes(shape) || nn > 65535) return "patch layout: too many nodes in a patch";
+            if (nn > patch_max_nodes(shape, nw) || nn > 65535) return "patch layout: too many nodes in a patch";
             auto lidx = [&](int d) { return (int)(std::lower_bound(local.begin(), local.end(), d) - local.begin()); };
             // tiles, slots, points, local node indexes; contributions counted per node
             cnt.assign(nn + 1, 0);

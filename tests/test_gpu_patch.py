@@ -140,3 +140,64 @@ def test_patch_assembly_needs_geometry(gpu):
     with pytest.raises(ipf.IPFError, match="geometric setup"):
         ctx.set_residual_assembly("patch")
     assert ctx.residual_assembly() == "colored"
+
+
+@pytest.mark.parametrize("waves", [2, 8])
+def test_patch_waves_give_the_same_residual(gpu, waves):
+    """Patches of 2 and 8 wavefront tiles (4 is the default, covered above):
+    the same residual as the colored assembly to the same bar, both stress
+    modes, Q2 and Q3."""
+    import torch
+    for mesh in ("40x40-q2", (20, 18, 3, 1.2)):
+        m = wb.make_mesh(mesh)
+        sm = ipf.StructMatrix(m, device=0)
+        ctx = sm.context()
+        neq = sm.sizes()["neq"]
+        flin = sm.export_indexes()["rhs_linear"]
+        base = wb.lame_displacement(m.nodes(), wb.normal_stress(10))
+        ctx.set_patch_waves(waves)
+        assert ctx.patch_waves() == waves
+        _compare(ctx, neq, flin, torch.as_tensor(2.0 * base, device=gpu), torch.as_tensor(3.0 * base, device=gpu),
+                 gpu, f"{mesh} waves={waves}")
+    with pytest.raises(ipf.IPFError, match="2, 4 or 8"):
+        ctx.set_patch_waves(6)
+
+
+def test_autotune_assembly_keeps_the_fastest(gpu):
+    """ipf_autotune_assembly times colored and patch-2/4/8, keeps the
+    fastest, leaves the committed state alone, and the residual of the
+    chosen layout matches the colored one (1e-13 of max|f_int|).  Without the
+    geometric setup only the colored assembly is timed."""
+    import torch
+    m = wb.make_mesh("40x40-q2")
+    sm = ipf.StructMatrix(m, device=0)
+    ctx = sm.context()
+    neq = sm.sizes()["neq"]
+    flin = sm.export_indexes()["rhs_linear"]
+    base = wb.lame_displacement(m.nodes(), wb.normal_stress(10))
+    u1 = torch.as_tensor(2.0 * base, device=gpu)
+    u2 = torch.as_tensor(3.0 * base, device=gpu)
+    _res(ctx, u1, neq, gpu, commit=True)
+    st = ctx.get_state()
+    rc = _res(ctx, u2, neq, gpu)
+    out = ctx.autotune_assembly(u2, reps=5)
+    print(out)
+    t = out["times_ms"]
+    assert set(t) == {"colored", "patch-2", "patch-4", "patch-8"}
+    assert out["selected"] == min(t, key=t.get)
+    assert all(np.array_equal(a, b) for a, b in zip(st, ctx.get_state()))
+    r = _res(ctx, u2, neq, gpu)
+    assert np.abs(r - rc).max() <= 1e-13 * np.abs(rc - flin).max()
+    # uploaded blocks (no geometry): only the colored assembly is timed
+    h = ipf.StructMatrix(m, device=-1)
+    b = h.export_blocks()
+    ix = h.export_indexes()
+    sz = h.sizes()
+    ctx2 = ipf.DeviceContext(device=0)
+    ctx2.transfer_data(b["row_sizes"], b["col_sizes"], b["matrix_position"], b["row_first"], b["col_first"],
+                       b["storage"], ix["dof_indexes"], ix["color_indexes"], sz["ncolor"], sz["neq"], ix["weight"])
+    mat = m.material()
+    ctx2.set_material(mat["lambda"], mat["mu"], mat["phi"], mat["psi"], mat["cohesion"])
+    ctx2.set_rhs_linear(ix["rhs_linear"])
+    out2 = ctx2.autotune_assembly(u2, reps=2)
+    assert set(out2["times_ms"]) == {"colored"} and out2["selected"] == "colored"

@@ -27,7 +27,6 @@ from intpointsfem_amd import wellbore as wb
 SHAPES = {(4, 8): 0, (9, 18): 1, (16, 32): 2, (3, 6): 3, (6, 12): 4, (12, 20): 5}
 NIP = [4, 9, 16, 3, 6, 12]
 CE = [8, 18, 32, 6, 12, 20]
-WAVES = [4, 4, 4, 4, 4, 4]  # patch_waves
 
 
 def et(s):
@@ -60,11 +59,11 @@ def elements(meshes):
             np.array(nodes, dtype=np.int64), neq0, pt0)
 
 
-def check(meshes):
+def check(meshes, waves=4):
     shape, mat, fp, cx, cy, nodes, neq, npts = elements(meshes)
     n = len(shape)
     noff = np.concatenate([[0], np.cumsum([CE[s] // 2 for s in shape])])
-    L = ipf.patch_plan(shape, mat, fp, cx, cy, nodes, neq)
+    L = ipf.patch_plan(shape, mat, fp, cx, cy, nodes, neq, waves=waves)
     uses = np.zeros(neq, dtype=np.int64)
     np.add.at(uses, nodes, 1)
     seen_elem = np.zeros(n, dtype=np.int64)
@@ -79,7 +78,7 @@ def check(meshes):
     partial = np.full((max(L["npartial"], 1), 2), np.nan)
     writer = np.full(max(L["npartial"], 1), -1, dtype=np.int64)  # the patch that writes each partial slot
     for (sh, npch, patch0, tile0, slot0, pt0, ln0) in L["ranges"]:
-        NW, ET, NSH, TP = WAVES[sh], et(sh), CE[sh] // 2, et(sh) * NIP[sh]
+        NW, ET, NSH, TP = waves, et(sh), CE[sh] // 2, et(sh) * NIP[sh]
         for p in range(npch):
             P = patch0 + p
             n0, n1 = L["node_off"][P], L["node_off"][P + 1]
@@ -158,10 +157,17 @@ def check(meshes):
     return L
 
 
+@pytest.mark.parametrize("waves", [2, 4, 8])
 @pytest.mark.parametrize("mesh", [(30, 20, 1, 1.1), (24, 22, 2, 1.1), (13, 11, 3, 1.2)],
                          ids=lambda t: f"{t[0]}x{t[1]}-q{t[2]}")
-def test_patch_layout_quarter_annulus(mesh):
-    check([wb.make_mesh(mesh)])
+def test_patch_layout_quarter_annulus(mesh, waves):
+    check([wb.make_mesh(mesh)], waves)
+
+
+def test_patch_layout_rejects_other_wave_counts():
+    m = wb.make_mesh((6, 5, 2, 1.1))
+    with pytest.raises(ipf.IPFError, match="2, 4 or 8"):
+        check([m], waves=3)
 
 
 def test_patch_layout_mixed_mesh_and_instances():
@@ -171,4 +177,5 @@ def test_patch_layout_mixed_mesh_and_instances():
     mix.set_material(5000, 0.2, 10, 0.5, 0.5)
     table = [dict(N=9, p=1, phi_deg=25.0, cohesion=3.0, E=3000.0, nu=0.2, pw_final=-20.0),
              dict(N=6, p=3, phi_deg=35.0, cohesion=2.0, E=15000.0, nu=0.28, pw_final=-28.0)]
-    check([mix] + [bt.instance_mesh(i) for i in table])
+    for waves in (2, 4, 8):
+        check([mix] + [bt.instance_mesh(i) for i in table], waves)
