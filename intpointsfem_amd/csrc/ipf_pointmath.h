@@ -120,7 +120,16 @@ IPF_HD void eigenvalues_newton(double up, double lo, const double* s, double max
 }
 
 // FunctionsSpectralDecomp.h:116-150
-IPF_HD void eigvec_simple(const double* s, double lam, double* vec) {
+// The eigenvector helpers return their vectors as values (V3) and the callers
+// store all nine components once, unconditionally: with results written
+// through &vec[0] / &vec[3] / &vec[6] under branches the compiler merged the
+// stores into one through a computed address and kept vec in private
+// (scratch) memory.  The arithmetic is unchanged.
+struct V3 {
+    double x, y, z;
+};
+
+IPF_HD V3 eigvec_simple(const double* s, double lam) {
     const double d0 = (s[XX] - lam) * (s[YY] - lam) - s[XY] * s[XY];
     const double d1 = (s[XX] - lam) * (s[ZZ] - lam) - s[XZ] * s[XZ];
     const double d2 = (s[YY] - lam) * (s[ZZ] - lam) - s[YZ] * s[YZ];
@@ -142,13 +151,11 @@ IPF_HD void eigvec_simple(const double* s, double lam, double* vec) {
         v2 = 1 / d2 * (-(s[YY] - lam) * s[XZ] + s[XY] * s[YZ]);
     }
     const double norm = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-    vec[0] = v0 / norm;
-    vec[1] = v1 / norm;
-    vec[2] = v2 / norm;
+    return V3{v0 / norm, v1 / norm, v2 / norm};
 }
 
 // FunctionsSpectralDecomp.h:155-212 (second norm keeps the v2[1]*v1[1] term of :203)
-IPF_HD void eigvec_double(const double* s, double lam, double* vec1, double* vec2) {
+IPF_HD void eigvec_double(const double* s, double lam, V3& vec1, V3& vec2) {
     const double x0 = s[XX] - lam, x1 = s[YY] - lam, x2 = s[ZZ] - lam;
     double maxx = fabs(x0);
     if (fabs(x1) > fabs(maxx)) maxx = fabs(x1);
@@ -166,8 +173,8 @@ IPF_HD void eigvec_double(const double* s, double lam, double* vec1, double* vec
     }
     const double n1 = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
     const double n2 = sqrt(b0 * b0 + b1 * a1 + b2 * b2);
-    vec1[0] = a0 / n1; vec1[1] = a1 / n1; vec1[2] = a2 / n1;
-    vec2[0] = b0 / n2; vec2[1] = b1 / n2; vec2[2] = b2 / n2;
+    vec1 = V3{a0 / n1, a1 / n1, a2 / n1};
+    vec2 = V3{b0 / n2, b1 / n2, b2 / n2};
 }
 
 // FunctionsSpectralDecomp.h:217-256 followed by the normalise/scale-back of
@@ -175,29 +182,27 @@ IPF_HD void eigvec_double(const double* s, double lam, double* vec1, double* vec
 IPF_HD void eigenvectors6(double* s, const double* ev, double* vec, double maxel) {
 #pragma unroll
     for (int i = 0; i < 6; i++) s[i] *= maxel;
-    if ((ev[0] == ev[1]) && (ev[1] == ev[2])) {
-#pragma unroll
-        for (int i = 0; i < 9; i++) vec[i] = 0.;
-        vec[0] = 1.;
-        vec[4] = 1.;
-        vec[8] = 1.;
-        return;
+    V3 a{1., 0., 0.}, b{0., 1., 0.}, c{0., 0., 1.};  // three equal eigenvalues: the identity
+    if (!((ev[0] == ev[1]) && (ev[1] == ev[2]))) {
+        if (ev[0] != ev[1] && ev[0] != ev[2]) {
+            a = eigvec_simple(s, ev[0]);
+        } else if (ev[0] == ev[1]) {
+            eigvec_double(s, ev[0], a, b);
+        } else if (ev[0] == ev[2]) {
+            eigvec_double(s, ev[0], a, c);
+        }
+        if (ev[1] != ev[0] && ev[1] != ev[2]) {
+            b = eigvec_simple(s, ev[1]);
+        } else if (ev[1] == ev[2]) {
+            eigvec_double(s, ev[1], b, c);
+        }
+        if (ev[2] != ev[0] && ev[2] != ev[1]) {
+            c = eigvec_simple(s, ev[2]);
+        }
     }
-    if (ev[0] != ev[1] && ev[0] != ev[2]) {
-        eigvec_simple(s, ev[0], &vec[0]);
-    } else if (ev[0] == ev[1]) {
-        eigvec_double(s, ev[0], &vec[0], &vec[3]);
-    } else if (ev[0] == ev[2]) {
-        eigvec_double(s, ev[0], &vec[0], &vec[6]);
-    }
-    if (ev[1] != ev[0] && ev[1] != ev[2]) {
-        eigvec_simple(s, ev[1], &vec[3]);
-    } else if (ev[1] == ev[2]) {
-        eigvec_double(s, ev[1], &vec[3], &vec[6]);
-    }
-    if (ev[2] != ev[0] && ev[2] != ev[1]) {
-        eigvec_simple(s, ev[2], &vec[6]);
-    }
+    vec[0] = a.x; vec[1] = a.y; vec[2] = a.z;
+    vec[3] = b.x; vec[4] = b.y; vec[5] = b.z;
+    vec[6] = c.x; vec[7] = c.y; vec[8] = c.z;
 }
 
 // FunctionsSpectralDecomp.h:261-268 (s is modified in place like the reference)
@@ -261,7 +266,7 @@ IPF_HD void eigenvalues_newton_ps(double up, double lo, const double* s, double 
     if (ev[1] < ev[2]) { const double a = ev[1]; ev[1] = ev[2]; ev[2] = a; }
 }
 
-IPF_HD void eigvec_simple_ps(const double* s, double lam, double* vec) {
+IPF_HD V3 eigvec_simple_ps(const double* s, double lam) {
     const double d0 = (s[XX] - lam) * (s[YY] - lam) - s[XY] * s[XY];
     const double d1 = (s[XX] - lam) * (s[ZZ] - lam) - s[XZ] * s[XZ];
     const double d2 = (s[YY] - lam) * (s[ZZ] - lam) - s[YZ] * s[YZ];
@@ -274,10 +279,7 @@ IPF_HD void eigvec_simple_ps(const double* s, double lam, double* vec) {
         const double t1 = -(s[XX] - lam) * s[YZ] + s[XY] * s[XZ];
         if ((d0 > 0.0 || d0 < 0.0) && t0 == 0.0 && t1 == 0.0) {
             // (1/d0) t: a zero with sign(d0) * sign(t); norm sqrt(0 + 0 + 1) = 1
-            vec[0] = d0 > 0.0 ? t0 : -t0;
-            vec[1] = d0 > 0.0 ? t1 : -t1;
-            vec[2] = 1.0;
-            return;
+            return V3{d0 > 0.0 ? t0 : -t0, d0 > 0.0 ? t1 : -t1, 1.0};
         }
         v0 = 1 / d0 * t0;
         v1 = 1 / d0 * t1;
@@ -292,38 +294,34 @@ IPF_HD void eigvec_simple_ps(const double* s, double lam, double* vec) {
         v2 = 1 / d2 * (-(s[YY] - lam) * s[XZ] + s[XY] * s[YZ]);
     }
     const double norm = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-    vec[0] = v0 / norm;
-    vec[1] = v1 / norm;
     // v2 / norm for a zero v2 and a finite norm (>= 1 here) is v2 itself
-    vec[2] = (v2 == 0.0 && norm >= 1.0 && norm <= 1.7976931348623157e308) ? v2 : v2 / norm;
+    return V3{v0 / norm, v1 / norm, (v2 == 0.0 && norm >= 1.0 && norm <= 1.7976931348623157e308) ? v2 : v2 / norm};
 }
 
 IPF_HD void eigenvectors6_ps(double* s, const double* ev, double* vec, double maxel) {
 #pragma unroll
     for (int i = 0; i < 6; i++) s[i] *= maxel;
-    if ((ev[0] == ev[1]) && (ev[1] == ev[2])) {
-#pragma unroll
-        for (int i = 0; i < 9; i++) vec[i] = 0.;
-        vec[0] = 1.;
-        vec[4] = 1.;
-        vec[8] = 1.;
-        return;
+    V3 a{1., 0., 0.}, b{0., 1., 0.}, c{0., 0., 1.};  // three equal eigenvalues: the identity
+    if (!((ev[0] == ev[1]) && (ev[1] == ev[2]))) {
+        if (ev[0] != ev[1] && ev[0] != ev[2]) {
+            a = eigvec_simple_ps(s, ev[0]);
+        } else if (ev[0] == ev[1]) {
+            eigvec_double(s, ev[0], a, b);
+        } else if (ev[0] == ev[2]) {
+            eigvec_double(s, ev[0], a, c);
+        }
+        if (ev[1] != ev[0] && ev[1] != ev[2]) {
+            b = eigvec_simple_ps(s, ev[1]);
+        } else if (ev[1] == ev[2]) {
+            eigvec_double(s, ev[1], b, c);
+        }
+        if (ev[2] != ev[0] && ev[2] != ev[1]) {
+            c = eigvec_simple_ps(s, ev[2]);
+        }
     }
-    if (ev[0] != ev[1] && ev[0] != ev[2]) {
-        eigvec_simple_ps(s, ev[0], &vec[0]);
-    } else if (ev[0] == ev[1]) {
-        eigvec_double(s, ev[0], &vec[0], &vec[3]);
-    } else if (ev[0] == ev[2]) {
-        eigvec_double(s, ev[0], &vec[0], &vec[6]);
-    }
-    if (ev[1] != ev[0] && ev[1] != ev[2]) {
-        eigvec_simple_ps(s, ev[1], &vec[3]);
-    } else if (ev[1] == ev[2]) {
-        eigvec_double(s, ev[1], &vec[3], &vec[6]);
-    }
-    if (ev[2] != ev[0] && ev[2] != ev[1]) {
-        eigvec_simple_ps(s, ev[2], &vec[6]);
-    }
+    vec[0] = a.x; vec[1] = a.y; vec[2] = a.z;
+    vec[3] = b.x; vec[4] = b.y; vec[5] = b.z;
+    vec[6] = c.x; vec[7] = c.y; vec[8] = c.z;
 }
 
 // s[XZ] and s[YZ] must be +0 on entry.

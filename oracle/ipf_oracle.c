@@ -9,8 +9,11 @@
  *                               reconstruction/trace invariants
  *   orc_project_sigma           pinned: closed forms per branch, yield
  *                               consistency, brute-force convex projection
- *   orc_tangent_operator        pinned: elastic Dep == D exactly, central
- *                               finite differences of orc_point_sigma
+ *   orc_tangent_operator        pinned: elastic Dep == D exactly, finite
+ *                               differences of orc_point_sigma on the normal
+ *                               columns of every plastic branch, frame
+ *                               covariance under rotations about z (the
+ *                               shear / eigenvector-rotation term)
  *   orc_point_sigma / orc_compute_sigma[_dep]  pinned through the above
  *   orc_multiply_vector / orc_multiply / orc_multiply_transpose  pinned:
  *                               dense numpy products, patch tests
