@@ -37,6 +37,12 @@ DEVICE = [
     "--offload-arch=" + ARCH,
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# Per-source device flags.  ipf_learned.hip: MFMA results in VGPRs (the
+# inference loop applies GELU to every MFMA result; with the results in AGPRs
+# each one is first copied out, 36 extra instructions per 16 samples).
+EXTRA = {
+    "ipf_learned.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
 
 
 def _hipcc():
@@ -57,7 +63,7 @@ def build(verbose=False, jobs=None):
         objs.append(obj)
         cmd = [hipcc] + COMMON
         if src.endswith(".hip"):
-            cmd += DEVICE + ["-x", "hip"]
+            cmd += DEVICE + EXTRA.get(src, []) + ["-x", "hip"]
         cmd += ["-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

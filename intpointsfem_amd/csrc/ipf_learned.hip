@@ -325,16 +325,12 @@ __device__ __forceinline__ float gelu_fast(float x) {
     return x * __builtin_amdgcn_rcpf(1.f + e);
 }
 
-typedef __bf16 bf16x2_hw __attribute__((ext_vector_type(2)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x4_hw __attribute__((ext_vector_type(4)));
 
-// Four FP32 values to BF16 with two v_cvt_pk_bf16_f32 (round to nearest
-// even: f2bf's result for every finite value).
+// Four FP32 values to BF16 with the hardware conversion (v_cvt_pk_bf16_f32:
+// round to nearest even, f2bf's result for every finite value).
 __device__ __forceinline__ bf16x4 to_bf16x4(f32x4 v) {
-    const s16x2 lo = __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{v[0], v[1]}, bf16x2_hw));
-    const s16x2 hi = __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{v[2], v[3]}, bf16x2_hw));
-    return bf16x4{lo.x, lo.y, hi.x, hi.y};
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16x4_hw));
 }
 
 // Inference over n samples, 16 per wavefront step, every layer computed
