@@ -11,8 +11,8 @@ and SQ_WAVES gives the VALU issue load: a wave64 vector instruction holds a
 average duration) at the peak engine clock (``--simds``, ``--clock-ghz``).
 
   python -m intpointsfem_amd.rocprof_summary --name k_residual_800x800-q2 \\
-      --trace gpurun_out/prof_trace --fetch gpurun_out/prof_fetch \\
-      --write gpurun_out/prof_write --neq 5126402 --workload "..." --command "..." [--command ...]
+      --trace /tmp/p_trace --fetch /tmp/p_fetch \\
+      --write /tmp/p_write --neq 5126402 --workload "..." --command "..." [--command ...]
 """
 import argparse
 import collections

@@ -42,7 +42,7 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-NT = 16  # oracle threads: the job's share of the GPU box's host
+NT = 16  # oracle threads: this job's share of the host's cores
 
 
 def _log(msg):
