@@ -207,9 +207,12 @@ __device__ __forceinline__ void reduce_lane(const typename FC<SHAPE, FMT>::T* L,
 // round trip before the gather completes, not two per gather round), and the
 // r values of the lane's column sums right after the gather (R_EARLY) or
 // both right after the return map -- the color's elements share no DoF, so
-// no other wave of the launch writes them.  R_EARLY costs 7 VGPRs; it is on
-// where that keeps the waves per SIMD (Q3: LDS already limits them to 4), off
-// where it would cost one wave (measured slower at Q2 on MI355X).
+// no other wave of the launch writes them.  R_EARLY costs 6-7 VGPRs; it is on
+// at Q3 (LDS already limits the waves per SIMD to 4) and for FAST at Q2 (94
+// VGPRs, still 5 waves per SIMD; a wave whose points are all certified
+// reaches the scatter soon after the gather: 0.184 against 0.196 ms per
+// 800x800 Q2 residual on MI355X), off for the exact Q2 kernel (no gain
+// measured).
 //
 // FAST (never with COMMIT): a point whose trial stress passes
 // certified_elastic takes its trial stress and skips the spectral
@@ -228,7 +231,7 @@ k_residual_fmt(GroupDev g, const void* __restrict__ Bv, const double* __restrict
     constexpr int H = PAIR ? 2 * ((NSH + 1) / 2) : C::CE / 2;
     constexpr int H2 = C::CE - H;
     static_assert(C::ET * NSH <= kWave, "one column sum (PAIR: node) per lane and half");
-    constexpr bool R_EARLY = SHAPE == SHAPE_Q3 && !COMMIT;
+    constexpr bool R_EARLY = (SHAPE == SHAPE_Q3 || (SHAPE == SHAPE_Q2 && FAST)) && !COMMIT;
     constexpr int NE = PAIR ? C::ET * NSH : C::ET * C::CE;  // index entries per tile
     __shared__ T lds[C::TP * H];
     __shared__ double us[C::ET * C::CE];

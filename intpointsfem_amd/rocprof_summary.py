@@ -93,6 +93,8 @@ def main():
                          "inside it, from the kernel trace, instead of the whole-run statistics")
     ap.add_argument("--also", action="append", default=[],
                     help="with --roi: also list, over the whole trace, the kernels whose name contains this text")
+    ap.add_argument("--calibrate-from", default=None,
+                    help="a traffic JSON of this tool whose k_copy calibration to use when the run has no k_copy")
     ap.add_argument("--sq", default=None, help="PMC run with SQ_INSTS_VALU and SQ_WAVES (optional)")
     ap.add_argument("--simds", type=int, default=1024, help="SIMDs of the GPU (MI355X: 256 CUs x 4)")
     ap.add_argument("--clock-ghz", type=float, default=2.4, help="peak engine clock (MI355X: 2.4 GHz)")
@@ -129,14 +131,23 @@ def main():
     if a.fetch and a.write:
         fetch = counters(os.path.join(a.fetch, f"{a.prefix}_counter_collection.csv"), "FETCH_SIZE")
         write = counters(os.path.join(a.write, f"{a.prefix}_counter_collection.csv"), "WRITE_SIZE")
-        copyk = next(k for k in fetch if "k_copy" in k)
-        true_b = a.neq * 8
-        ff = true_b / (fetch[copyk] * 1024)
-        fw = true_b / (write[copyk] * 1024)
-        lines += ["", "## HBM traffic per dispatch (PMC)", "",
-                  f"Calibration on `k_copy` (reads and writes neq x 8 = {true_b / 1e6:.2f} MB): FETCH_SIZE reads "
-                  f"{fetch[copyk] * 1024 / 1e6:.2f} MB -> factor {ff:.3f}; WRITE_SIZE reads "
-                  f"{write[copyk] * 1024 / 1e6:.2f} MB -> factor {fw:.3f}.", "",
+        copyk = next((k for k in fetch if "k_copy" in k), None)
+        if copyk is not None:
+            true_b = a.neq * 8
+            ff = true_b / (fetch[copyk] * 1024)
+            fw = true_b / (write[copyk] * 1024)
+            cal = (f"Calibration on `k_copy` (reads and writes neq x 8 = {true_b / 1e6:.2f} MB): FETCH_SIZE reads "
+                   f"{fetch[copyk] * 1024 / 1e6:.2f} MB -> factor {ff:.3f}; WRITE_SIZE reads "
+                   f"{write[copyk] * 1024 / 1e6:.2f} MB -> factor {fw:.3f}.")
+        elif a.calibrate_from:
+            with open(a.calibrate_from) as f:
+                c = json.load(f)["calibration"]
+            ff, fw, true_b = c["fetch_factor"], c["write_factor"], c["copy_kernel_bytes"]
+            cal = (f"Calibration factors from {os.path.basename(a.calibrate_from)} (its `k_copy` of "
+                   f"{true_b / 1e6:.2f} MB, same GPU model and counters): FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {fw:.3f}.")
+        else:
+            raise SystemExit("no k_copy dispatch in the PMC runs: give --calibrate-from")
+        lines += ["", "## HBM traffic per dispatch (PMC)", "", cal, "",
                   "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read MB (calibrated) | written MB (calibrated) | "
                   "total MB |", "|---|---|---|---|---|---|"]
         out = {"calibration": {"fetch_factor": ff, "write_factor": fw, "copy_kernel_bytes": true_b}, "kernels": {}}
