@@ -201,3 +201,29 @@ def test_autotune_assembly_keeps_the_fastest(gpu):
     ctx2.set_rhs_linear(ix["rhs_linear"])
     out2 = ctx2.autotune_assembly(u2, reps=2)
     assert set(out2["times_ms"]) == {"colored"} and out2["selected"] == "colored"
+
+
+def test_patch_residual_timed_reports_the_state_refresh(gpu):
+    """ipf_residual_timed over the patch layout: launch 0 (the refresh of the
+    patch copy of the plastic strain) takes a positive time after a commit
+    and reads exactly 0 once the copy is current; the fix-up is the last
+    launch (0 points); the residual equals the untimed one bit for bit."""
+    import torch
+    m = wb.make_mesh("40x40-q2")
+    sm = ipf.StructMatrix(m, device=0)
+    ctx = sm.context()
+    neq = sm.sizes()["neq"]
+    base = wb.lame_displacement(m.nodes(), wb.normal_stress(10))
+    u1 = torch.as_tensor(2.0 * base, device=gpu)
+    u2 = torch.as_tensor(3.0 * base, device=gpu)
+    ctx.set_residual_assembly("patch")
+    _res(ctx, u1, neq, gpu, commit=True)
+    r = torch.zeros(neq, dtype=torch.float64, device=gpu)
+    ms1, pts1 = ctx.residual_timed(u2, r)
+    ms2, pts2 = ctx.residual_timed(u2, r)
+    ctx.synchronize()
+    rt = r.cpu().numpy()
+    print(f"after a commit: {ms1}; current: {ms2}")
+    assert ms1[0] > 0.0 and ms2[0] == 0.0
+    assert pts2[0] == 0 and pts2[-1] == 0 and pts2.sum() == sm.sizes()["npts"]
+    assert np.array_equal(rt, _res(ctx, u2, neq, gpu))

@@ -61,6 +61,13 @@ def test_unowned_dofs_keep_f_lin(gpu, mesh, fmt):
         ctx.residual(torch.as_tensor(u[:n], device=gpu), r)
         ctx.synchronize()
         out[n] = r.cpu().numpy()
+        # ipf_residual_timed: the same residual; its launch 0 (the copy) reads
+        # exactly 0 ms when nothing runs there, a positive time when it does
+        r.fill_(float("nan"))
+        ms, pts = ctx.residual_timed(torch.as_tensor(u[:n], device=gpu), r)
+        assert np.array_equal(r.cpu().numpy(), out[n])
+        assert pts[0] == 0 and pts[1:].sum() == s["npts"] and (ms[1:] > 0).all()
+        assert (ms[0] == 0.0) if n == neq else (ms[0] > 0.0)
         ctx.set_b_format(fmt, tangent=fmt)
         ctx.residual_tangent(torch.as_tensor(u[:n], device=gpu), torch.empty_like(r))
         y = torch.full((n,), float("nan"), dtype=torch.float64, device=gpu)
