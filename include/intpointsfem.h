@@ -468,8 +468,12 @@ int ipf_set_newton_atol(ipf_ctx *ctx, double atol);
  * from 0 to neq) and converges on its own criterion ||r_k|| <= rtol ||r0_k||
  * (norms over the DoFs the Dirichlet mask leaves free); once it has, its
  * DoFs are held -- out of the CG and of the update -- while the others
- * iterate, and the mask is restored afterwards.  The commit follows as in
- * ipf_newton.  stats as ipf_newton's, with [1] = all segments converged, [2]
+ * iterate, and the mask is restored afterwards.  One CG serves all segments
+ * (the system is block diagonal): each active segment's right-hand side is
+ * scaled to unit norm and the CG tolerance is cg_rtol / sqrt(active
+ * segments), so that every segment's own relative CG residual is <= cg_rtol
+ * as in its own ipf_newton.  The commit follows as in ipf_newton.  stats as
+ * ipf_newton's, with [1] = all segments converged, [2]
  * the largest ||r_k|| / ||r0_k||, [3] / [4] the 2-norms over all segments;
  * seg_rel [nseg] (optional) the final ratio per segment, seg_iters [nseg]
  * (optional) the Newton iteration at which each converged (-1: not

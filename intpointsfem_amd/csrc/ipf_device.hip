@@ -2280,9 +2280,26 @@ int ipf_newton_segmented(ipf_ctx* ctx, double* u_dev, int nseg, const int64_t* s
             ctx->monitor(ctx->monitor_user, it, worst, std::sqrt(sn), cg_total);
         }
         if (rc || all || it == maxit) break;
+        // One CG for all segments (the system is block diagonal), each active
+        // segment's right-hand side scaled to unit norm and the tolerance
+        // divided by sqrt(active segments): then every segment's own
+        // relative residual is <= cg_rtol, as in its own Newton; with the
+        // plain right-hand side the segments with large residuals would set
+        // the stopping point and the others would get inaccurate updates.
+        int nact = 0;
+        for (int k = 0; k < nseg; k++)
+            if (!done[k] && rn[k] > 0.0) {
+                launch_scale(seg_first[k + 1] - seg_first[k], 1.0 / rn[k], ctx->dNr + seg_first[k], ctx->stream);
+                nact++;
+            }
         int cgi = 0;
         double cgr = 0.0;
-        if ((rc = ipf_cg_solve(ctx, ctx->dNr, ctx->dNd, cg_rtol, cg_maxit, &cgi, &cgr))) break;
+        if ((rc = ipf_cg_solve(ctx, ctx->dNr, ctx->dNd, cg_rtol / std::sqrt((double)std::max(1, nact)), cg_maxit,
+                               &cgi, &cgr)))
+            break;
+        for (int k = 0; k < nseg; k++)
+            if (!done[k] && rn[k] > 0.0)
+                launch_scale(seg_first[k + 1] - seg_first[k], rn[k], ctx->dNd + seg_first[k], ctx->stream);
         cg_total += cgi;
         worst_cg = std::max(worst_cg, cgr);
         launch_axpy(ctx->neq, 1.0, ctx->dNd, u_dev, ctx->stream);

@@ -196,6 +196,8 @@ void launch_cg_mask_dot(int64_t n, const signed char* mask, const double* p, dou
 void launch_cg_update_dot(int64_t n, double* scal, const double* p, const double* q, double* x, double* r,
                           const double* pa, const double* pb, double* z, double* partial, int nb, hipStream_t stream);
 void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t stream);
+// x = a x
+void launch_scale(int64_t n, double a, double* x, hipStream_t stream);
 void launch_mask_copy(int64_t n, const signed char* mask, const double* src, double* dst, hipStream_t stream);
 int dot_blocks(int64_t n);
 // out[k] = sum of x[i]^2 over the free (mask 0) i of segment k, the segment

@@ -271,6 +271,11 @@ __global__ void k_axpy(int64_t n, double a, const double* __restrict__ x, double
     if (i < n) y[i] = y[i] + a * x[i];
 }
 
+__global__ void k_scale(int64_t n, double a, double* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = a * x[i];
+}
+
 __global__ void k_mask_copy(int64_t n, const signed char* __restrict__ mask, const double* __restrict__ src,
                             double* __restrict__ dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -367,6 +372,10 @@ void launch_cg_direction(int64_t n, const double* scal, const double* z, double*
 
 void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_axpy, grid1(n), dim3(256), 0, s, n, a, x, y);
+}
+
+void launch_scale(int64_t n, double a, double* x, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_scale, grid1(n), dim3(256), 0, s, n, a, x);
 }
 
 void launch_mask_copy(int64_t n, const signed char* mask, const double* src, double* dst, hipStream_t s) {

@@ -76,7 +76,10 @@ def test_batch_newton_reaches_each_instance_solution(gpu):
     with the global criterion (ipf_newton) within 1e-7 of max |u_i|, and with
     per-instance convergence (ipf_newton_segmented: every instance to its own
     ||r_i|| <= 1e-10 ||r0_i|| at every step) within 1e-12 (measured 4e-16 to
-    9e-16 on MI355X, against 1e-13 to 2e-12 with the global criterion)."""
+    9e-16 on MI355X, against 1e-13 to 2e-12 with the global criterion); and
+    per-instance convergence with the loose CG tolerance 0.1 (every step
+    converges; within 1e-7 of the instance alone, the bar of the global
+    criterion)."""
     import torch
     table = NEWTON_TABLE
     meshes = [batch.instance_mesh(i) for i in table]
@@ -101,6 +104,18 @@ def test_batch_newton_reaches_each_instance_solution(gpu):
         assert (st["seg_rel"] <= 1e-10).all() and (st["seg_iters"] >= 0).all(), (k, st)
         print(f"segmented step {k}: newton iterations per instance {st['seg_iters'].tolist()}", flush=True)
     useg = us.cpu().numpy()
+    # the same with the loose inner tolerance of the bench's solves (0.1): the
+    # segments' right-hand sides are scaled to unit norm inside, so each one
+    # gets the CG accuracy it would get alone and every step converges
+    Bl = batch.Batch(meshes, device=0)
+    Bl.ctx.set_dirichlet_mask(np.concatenate([m.dirichlet_mask() for m in meshes]))
+    ul = torch.zeros(Bl.neq, dtype=torch.float64, device=gpu)
+    for k in range(1, wb.NSTEPS + 1):
+        Bl.ctx.set_rhs_linear(Bl.rhs_linear * (k / wb.NSTEPS))
+        st = Bl.newton(ul, rtol=1e-10, maxit=80, cg_rtol=1e-1, cg_maxit=40000)
+        print(f"loose-CG segmented step {k}: newton iterations per instance {st['seg_iters'].tolist()}", flush=True)
+        assert st["converged"] == 1.0 and (st["seg_iters"] >= 0).all(), (k, st)
+    uloose = ul.cpu().numpy()
     for i, m in enumerate(meshes):
         sm = ipf.StructMatrix(m, device=0)
         ctx = sm.context()
@@ -114,9 +129,12 @@ def test_batch_newton_reaches_each_instance_solution(gpu):
         ref = ui.cpu().numpy()
         err = np.abs(ub[B.eq_offsets[i]:B.eq_offsets[i + 1]] - ref).max() / np.abs(ref).max()
         err_s = np.abs(useg[Bs.eq_offsets[i]:Bs.eq_offsets[i + 1]] - ref).max() / np.abs(ref).max()
-        print(f"instance {i}: |u_batch - u_alone| / |u| = {err:.2e} (global criterion), {err_s:.2e} (per instance)")
+        err_l = np.abs(uloose[Bl.eq_offsets[i]:Bl.eq_offsets[i + 1]] - ref).max() / np.abs(ref).max()
+        print(f"instance {i}: |u_batch - u_alone| / |u| = {err:.2e} (global criterion), {err_s:.2e} (per instance), "
+              f"{err_l:.2e} (per instance, CG tolerance 0.1)")
         assert err <= 1e-7
         assert err_s <= 1e-12
+        assert err_l <= 1e-7
 
 
 def test_material_errors(gpu):
