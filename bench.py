@@ -556,8 +556,9 @@ def main():
                          "geometric FP64; the residual stays FP64 whatever this is)")
     ap.add_argument("--batch-state", choices=("lame", "newton"), default="lame",
                     help="state of the batched measurement: a committed plastic state from each instance's "
-                         "linear-elastic displacement (fast), or the converged 10-step Newton state of every "
-                         "instance (solved instance by instance; minutes)")
+                         "linear-elastic displacement (fast), or 10 load steps of Newton for every instance, "
+                         "instance by instance (about 10 minutes; with the CG tolerance of --cg-rtol some "
+                         "low-cohesion instances diverge, which the line reports)")
     ap.add_argument("--batch-tangent-format", default="compact-fp32",
                     help="B format of the CG matvecs of the batch's Newton solve (FP32: mixed precision)")
     args = ap.parse_args()
