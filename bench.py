@@ -373,6 +373,7 @@ def learned_mixed_batched(args, dev):
             ci.set_stream(stream.cuda_stream)
             ci.set_dirichlet_mask(m.dirichlet_mask())
             ci.set_b_format(ci.b_format()[0], tangent=tfmt)
+            ci.set_newton_line_search(args.batch_line_search)
             fi = smi.export_indexes()["rhs_linear"]
             ui = torch.zeros(smi.sizes()["neq"], dtype=torch.float64, device=dev)
             ni = 0
@@ -396,7 +397,8 @@ def learned_mixed_batched(args, dev):
         u = torch.as_tensor(np.concatenate(ue), dtype=torch.float64, device=dev)
         ctx.set_state(np.concatenate(eps), np.concatenate(als))
         ctx.synchronize()
-        newton = {"tangent_format": tfmt, "newton_iterations_per_instance": its, "converged": conv,
+        newton = {"tangent_format": tfmt, "line_search_halvings": args.batch_line_search,
+                  "newton_iterations_per_instance": its, "converged": conv,
                   "worst_final_rel_residual": worst, "cg_iterations": cgs,
                   "solve_seconds": round(time.time() - t1, 1),
                   "load": f"{wb.NSTEPS} load steps to each instance's final pressure, each instance on its own"}
@@ -556,9 +558,10 @@ def main():
                          "geometric FP64; the residual stays FP64 whatever this is)")
     ap.add_argument("--batch-state", choices=("lame", "newton"), default="lame",
                     help="state of the batched measurement: a committed plastic state from each instance's "
-                         "linear-elastic displacement (fast), or 10 load steps of Newton for every instance, "
-                         "instance by instance (about 10 minutes; with the CG tolerance of --cg-rtol some "
-                         "low-cohesion instances diverge, which the line reports)")
+                         "linear-elastic displacement (fast), or 10 load steps of Newton with a line search for "
+                         "every instance, instance by instance (about 10 minutes)")
+    ap.add_argument("--batch-line-search", type=int, default=8,
+                    help="step halvings of the Newton line search in --batch-state newton (0: full steps)")
     ap.add_argument("--batch-tangent-format", default="compact-fp32",
                     help="B format of the CG matvecs of the batch's Newton solve (FP32: mixed precision)")
     args = ap.parse_args()

@@ -447,8 +447,18 @@ int ipf_cg_solve(ipf_ctx *ctx, const double *b_dev, double *x_dev, double rtol, 
  * relative test alone could not be met).  The caller's UpdateMem switch is
  * the same after the call, whatever its status.
  * stats[0..7] = Newton iterations, converged (0/1), ||r||/||r0||, ||r0||,
- * ||r||, total CG iterations, worst CG relative residual, 0. */
+ * ||r||, total CG iterations, worst CG relative residual, step halvings of
+ * the line search (0 without one). */
 int ipf_newton(ipf_ctx *ctx, double *u_dev, double rtol, int maxit, double cg_rtol, int cg_maxit, double *stats);
+
+/* Backtracking line search for ipf_newton (0, the default: the reference's
+ * full steps): after each CG solve, u0 + t du with t = 1, 1/2, ... is taken
+ * at the first t whose free-DoF residual norm is below ||r(u0)||, or at the
+ * last of max_halvings halvings (0..30).  Each trial costs one residual.
+ * With an inexact inner solve (a loose cg_rtol) a full step can increase the
+ * residual far from the solution; the search keeps such steps from
+ * diverging.  No reference counterpart (main.cpp:160-262 takes full steps). */
+int ipf_set_newton_line_search(ipf_ctx *ctx, int max_halvings);
 
 /* Progress report of ipf_newton / ipf_newton_segmented, called on the host
  * after each residual norm (iteration 0 = the entry u), like the residual

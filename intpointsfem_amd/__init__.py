@@ -108,6 +108,7 @@ def lib():
         "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
         "ipf_set_cg_graph": (_int, [_vp, _int]),
         "ipf_set_newton_atol": (_int, [_vp, _d]),
+        "ipf_set_newton_line_search": (_int, [_vp, _int]),
         "ipf_set_newton_monitor": (_int, [_vp, NEWTON_MONITOR, _vp]),
         "ipf_set_residual_mode": (_int, [_vp, _int]),
         "ipf_get_residual_mode": (_int, [_vp, _ip]),
@@ -613,6 +614,11 @@ class DeviceContext:
         keys = ("built", "patches", "patch_nodes", "partials", "shared_nodes", "orphan_dofs", "points", "shapes")
         return dict(zip(keys, (int(v) for v in out)))
 
+    def set_newton_line_search(self, max_halvings):
+        """Backtracking line search of newton(): up to max_halvings step
+        halvings per iteration (0: full Newton steps)."""
+        self._c(lib().ipf_set_newton_line_search(self._h, int(max_halvings)), "ipf_set_newton_line_search")
+
     def set_patch_waves(self, waves):
         """Wavefront tiles per patch of the patch assembly: 2, 4 or 8."""
         self._c(lib().ipf_set_patch_waves(self._h, int(waves)), "ipf_set_patch_waves")
@@ -653,8 +659,9 @@ class DeviceContext:
         """Newton for the current load; commits the state. Returns a stats dict."""
         st = np.zeros(8)
         self._c(lib().ipf_newton(self._h, _ptr(u), rtol, maxit, cg_rtol, cg_maxit, _p(st, _dp)), "ipf_newton")
-        keys = ("iterations", "converged", "rel_residual", "r0", "r", "cg_iterations", "cg_worst_relres")
-        return dict(zip(keys, (float(v) for v in st[:7])))
+        keys = ("iterations", "converged", "rel_residual", "r0", "r", "cg_iterations", "cg_worst_relres",
+                "line_search_halvings")
+        return dict(zip(keys, (float(v) for v in st[:8])))
 
     def point_features(self, u, feat, branch, target):
         """Training pairs of the exact return map at the trial state of u

@@ -258,6 +258,8 @@ struct ipf_ctx {
     double* dCq = nullptr;
     double* dNr = nullptr;      // Newton residual
     double* dNd = nullptr;      // Newton correction
+    double* dLs = nullptr;      // u before a line-searched Newton update
+    int ls_halvings = 0;        // ipf_set_newton_line_search (0: full Newton steps)
     double* dPartial = nullptr; // dot partials
     double* dScal = nullptr;    // CG scalars
     bool have_mask = false;
@@ -393,6 +395,7 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dCq);
     dev_free(ctx->dNr);
     dev_free(ctx->dNd);
+    dev_free(ctx->dLs);
     dev_free(ctx->dPartial);
     dev_free(ctx->dScal);
     for (int f = 1; f < FMT_COUNT; f++) dev_free(ctx->dBf[f]);
@@ -1630,6 +1633,14 @@ int ipf_set_newton_monitor(ipf_ctx* ctx, ipf_newton_monitor fn, void* user) {
     return IPF_OK;
 }
 
+int ipf_set_newton_line_search(ipf_ctx* ctx, int max_halvings) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_newton_line_search: NULL context");
+    if (max_halvings < 0 || max_halvings > 30)
+        return fail(ctx, IPF_EINVAL, "ipf_set_newton_line_search: max_halvings %d outside [0, 30]", max_halvings);
+    ctx->ls_halvings = max_halvings;
+    return IPF_OK;
+}
+
 int ipf_set_newton_atol(ipf_ctx* ctx, double atol) {
     if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_newton_atol: NULL context");
     if (!(atol >= 0.0)) return fail(ctx, IPF_EINVAL, "ipf_set_newton_atol: atol must be >= 0");
@@ -2143,7 +2154,7 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
     const bool commit_flag = ctx->update_mem;
     ctx->update_mem = false;
     double r0 = 0.0, rn = 0.0, worst_cg = 0.0;
-    int it = 0, cg_total = 0;
+    int it = 0, cg_total = 0, ls_halved = 0;
     bool converged = false;
     for (it = 0; it <= maxit; it++) {
         if ((rc = ipf_residual_tangent(ctx, u_dev, ctx->dNr))) break;
@@ -2160,7 +2171,24 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
         if ((rc = ipf_cg_solve(ctx, ctx->dNr, ctx->dNd, cg_rtol, cg_maxit, &cgi, &cgr))) break;
         cg_total += cgi;
         worst_cg = std::max(worst_cg, cgr);
-        launch_axpy(ctx->neq, 1.0, ctx->dNd, u_dev, ctx->stream);
+        if (ctx->ls_halvings == 0) {
+            launch_axpy(ctx->neq, 1.0, ctx->dNd, u_dev, ctx->stream);
+            continue;
+        }
+        // backtracking: u = u0 + t du with t = 1, 1/2, ... until ||r|| drops
+        // below its value at u0 (or the halvings run out: the last t stays)
+        if (!ctx->dLs && (rc = dev_alloc(ctx, &ctx->dLs, ctx->neq))) break;
+        launch_copy(ctx->neq, u_dev, ctx->dLs, ctx->stream);
+        double t = 1.0, rt = 0.0;
+        for (int h = 0;; h++) {
+            launch_copy(ctx->neq, ctx->dLs, u_dev, ctx->stream);
+            launch_axpy(ctx->neq, t, ctx->dNd, u_dev, ctx->stream);
+            if ((rc = ipf_residual(ctx, u_dev, ctx->dNr)) || (rc = masked_norm(ctx, ctx->dNr, &rt))) break;
+            if (rt < rn || h == ctx->ls_halvings) break;  // a NaN norm halves too
+            t *= 0.5;
+            ls_halved++;
+        }
+        if (rc) break;
     }
     // AcceptPseudoTimeStepSolution: one residual with UpdateMem on
     if (!rc) {
@@ -2178,7 +2206,7 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
         stats[4] = rn;
         stats[5] = cg_total;
         stats[6] = worst_cg;
-        stats[7] = 0.0;
+        stats[7] = ls_halved;
     }
     return IPF_OK;
 }
