@@ -559,7 +559,8 @@ def main():
     ap.add_argument("--batch-state", choices=("lame", "newton"), default="lame",
                     help="state of the batched measurement: a committed plastic state from each instance's "
                          "linear-elastic displacement (fast), or 10 load steps of Newton with a line search for "
-                         "every instance, instance by instance (about 10 minutes)")
+                         "every instance, instance by instance (over 15 minutes; with the inexact inner solves of "
+                         "--cg-rtol some low-cohesion instances do not converge, which the line reports)")
     ap.add_argument("--batch-line-search", type=int, default=8,
                     help="step halvings of the Newton line search in --batch-state newton (0: full steps)")
     ap.add_argument("--batch-tangent-format", default="compact-fp32",
