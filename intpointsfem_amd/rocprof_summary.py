@@ -10,6 +10,16 @@ and SQ_WAVES gives the VALU issue load: a wave64 vector instruction holds a
 16-lane SIMD for 4 cycles, so busy = 4 * SQ_INSTS_VALU / (SIMDs * clock *
 average duration) at the peak engine clock (``--simds``, ``--clock-ghz``).
 
+Alternatively (``--dram-read`` / ``--dram-write``) the bytes come straight
+from the L2's DRAM request counters, which count in 32-byte units (a 64-byte
+request counts 2, a 128-byte request 4): TCC_EA0_RDREQ_DRAM_32B_sum and
+TCC_EA0_WRREQ_WRITE_DRAM_32B_sum, no scaling; ``k_copy`` then serves as a
+check.  On gfx950 FETCH_SIZE counts a 128-byte read request as 64 bytes
+(for a copy kernel whose reads are all 128-byte requests it reads exactly
+64 bytes per request), so the FETCH_SIZE scaling holds only for kernels
+whose reads are all 128-byte requests; the read run therefore also records
+the requests by size (TCC_EA0_RDREQ_128B/64B/32B_sum).
+
   python -m intpointsfem_amd.rocprof_summary --name k_residual_800x800-q2 \\
       --trace /tmp/p_trace --fetch /tmp/p_fetch \\
       --write /tmp/p_write --neq 5126402 --workload "..." --command "..." [--command ...]
@@ -86,6 +96,10 @@ def main():
     ap.add_argument("--trace", required=True)
     ap.add_argument("--fetch", default=None, help="FETCH_SIZE run (omit with --write for a trace-only summary)")
     ap.add_argument("--write", default=None)
+    ap.add_argument("--dram-read", default=None,
+                    help="PMC run with TCC_EA0_RDREQ_DRAM_32B_sum (and the request sizes), instead of --fetch")
+    ap.add_argument("--dram-write", default=None,
+                    help="PMC run with TCC_EA0_WRREQ_WRITE_DRAM_32B_sum, instead of --write")
     ap.add_argument("--prefix", default="q2")
     ap.add_argument("--neq", type=int, default=0)
     ap.add_argument("--roi", default=None,
@@ -161,6 +175,33 @@ def main():
             lines.append(f"| `{k}` | {fetch[k]:.0f} | {write[k]:.0f} | {rb / 1e6:.1f} | {wb / 1e6:.1f} | "
                          f"{(rb + wb) / 1e6:.1f} |")
             out["kernels"][k] = {"read_bytes": rb, "write_bytes": wb, "total_bytes": rb + wb}
+    if a.dram_read and a.dram_write:
+        rf = os.path.join(a.dram_read, f"{a.prefix}_counter_collection.csv")
+        rd = counters(rf, "TCC_EA0_RDREQ_DRAM_32B_sum")
+        wr = counters(os.path.join(a.dram_write, f"{a.prefix}_counter_collection.csv"),
+                      "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum")
+        size = {n: counters(rf, f"TCC_EA0_RDREQ_{n}B_sum") for n in (128, 64, 32)}
+        copyk = next((k for k in rd if "k_copy" in k), None)
+        chk = ("" if copyk is None or not a.neq else
+               f"Check on `k_copy` (reads and writes neq x 8 = {a.neq * 8 / 1e6:.2f} MB): the counters give "
+               f"{rd[copyk] * 32 / 1e6:.2f} MB read, {wr.get(copyk, 0) * 32 / 1e6:.2f} MB written.")
+        lines += ["", "## HBM traffic per dispatch (PMC)", "",
+                  "Bytes = 32 x TCC_EA0_RDREQ_DRAM_32B_sum (read) and 32 x TCC_EA0_WRREQ_WRITE_DRAM_32B_sum "
+                  "(written): the L2's requests to DRAM in 32-byte units, unscaled. " + chk, "",
+                  "| kernel | read MB | written MB | total MB | read requests 128 B / 64 B / 32 B |",
+                  "|---|---|---|---|---|"]
+        out = {"calibration": {"counters": "TCC_EA0_RDREQ_DRAM_32B_sum, TCC_EA0_WRREQ_WRITE_DRAM_32B_sum",
+                               "fetch_factor": 1.0, "write_factor": 1.0}, "kernels": {}}
+        for k in rd:
+            if k not in wr:
+                continue
+            rb, wb = rd[k] * 32, wr[k] * 32
+            if rb + wb < 1e6:
+                continue
+            mix = " / ".join(f"{size[n].get(k, 0.0) / 1e6:.3f} M" for n in (128, 64, 32))
+            lines.append(f"| `{k}` | {rb / 1e6:.1f} | {wb / 1e6:.1f} | {(rb + wb) / 1e6:.1f} | {mix} |")
+            out["kernels"][k] = {"read_bytes": rb, "write_bytes": wb, "total_bytes": rb + wb,
+                                 "read_requests": {f"{n}B": size[n].get(k, 0.0) for n in (128, 64, 32)}}
     if a.sq:
         sqf = os.path.join(a.sq, f"{a.prefix}_counter_collection.csv")
         valu = counters(sqf, "SQ_INSTS_VALU")
