@@ -501,6 +501,18 @@ int ipf_newton_segmented(ipf_ctx *ctx, double *u_dev, int nseg, const int64_t *s
  * main.cpp:283-285). */
 int ipf_set_cg_graph(ipf_ctx *ctx, int enable);
 
+/* ipf_newton runs each Newton iteration's device work up to its first host
+ * read -- the tangent residual (ipf_residual_tangent's launches), the
+ * residual norm over the free DoFs, the CG's preconditioner and start and
+ * its first 10 iterations -- as one HIP graph replay, captured once per
+ * ipf_newton call (enable != 0, the default; used when cg_maxit >= 10), or
+ * launches the same kernels one by one (enable == 0).  One host read per
+ * iteration instead of three; the iterates, iteration counts and committed
+ * state are the same bits either way.  The graph records no memset or copy
+ * node.  ipf_newton_segmented runs its iterations eagerly.  The reference's
+ * Newton step (main.cpp:160-262) is host-driven throughout. */
+int ipf_set_newton_graph(ipf_ctx *ctx, int enable);
+
 /* ---- integration-point state (TPZConstitutiveLawProcessor.h:53-63) ---- */
 
 /* Copies the committed state to host arrays (any may be NULL):

@@ -107,6 +107,7 @@ def lib():
         "ipf_cg_solve": (_int, [_vp, _vp, _vp, _d, _int, _ip, _dp]),
         "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
         "ipf_set_cg_graph": (_int, [_vp, _int]),
+        "ipf_set_newton_graph": (_int, [_vp, _int]),
         "ipf_set_newton_atol": (_int, [_vp, _d]),
         "ipf_set_newton_line_search": (_int, [_vp, _int]),
         "ipf_set_newton_monitor": (_int, [_vp, NEWTON_MONITOR, _vp]),
@@ -580,6 +581,12 @@ class DeviceContext:
     def set_cg_graph(self, enable):
         """CG iterations from a captured HIP graph (default) or launched one by one."""
         self._c(lib().ipf_set_cg_graph(self._h, 1 if enable else 0), "ipf_set_cg_graph")
+
+    def set_newton_graph(self, enable):
+        """Each Newton iteration up to its first host read (tangent residual,
+        residual norm, CG start and first CG block) as one captured HIP graph
+        replay (default) or launched eagerly; the same bits either way."""
+        self._c(lib().ipf_set_newton_graph(self._h, 1 if enable else 0), "ipf_set_newton_graph")
 
     def cg_solve(self, b, x, rtol=1e-10, maxit=10000):
         it = ctypes.c_int(0)

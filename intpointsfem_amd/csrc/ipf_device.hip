@@ -289,6 +289,10 @@ struct ipf_ctx {
     hipGraphExec_t cg_exec = nullptr;
     const double* cg_x = nullptr;
     int cg_tfmt = -1;
+    // ipf_newton: each iteration's device work up to its first host read
+    // (tangent residual, residual norm, CG start, first CG block) as one
+    // graph, captured per call on cap_stream
+    bool newton_graph = true;
 
     // tangent CSR pattern (ipf_set_tangent_pattern): per group element, the
     // CSR positions of its packed upper triangle
@@ -449,10 +453,22 @@ int read_scalar(ipf_ctx* ctx, int slot, double* out) {
     return IPF_OK;
 }
 
-// ||x|| over the free DoFs (uses the CG q vector as scratch)
+// scalars [0, count) in one copy
+int read_scalars(ipf_ctx* ctx, int count, double* out) {
+    IPF_HIP(ctx, hipMemcpyAsync(out, ctx->dScal, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return IPF_OK;
+}
+
+// ||x||^2 over the free DoFs into scalar 6 (uses the CG q vector as scratch)
+void masked_norm_launch(ipf_ctx* ctx, const double* x, hipStream_t s) {
+    launch_mask_copy(ctx->neq, ctx->dMask, x, ctx->dCq, s);
+    launch_dot(ctx->neq, ctx->dCq, ctx->dCq, ctx->dPartial, dot_blocks(ctx->neq), ctx->dScal + 6, s);
+}
+
+// ||x|| over the free DoFs
 int masked_norm(ipf_ctx* ctx, const double* x, double* out) {
-    launch_mask_copy(ctx->neq, ctx->dMask, x, ctx->dCq, ctx->stream);
-    launch_dot(ctx->neq, ctx->dCq, ctx->dCq, ctx->dPartial, dot_blocks(ctx->neq), ctx->dScal + 6, ctx->stream);
+    masked_norm_launch(ctx, x, ctx->stream);
     double v;
     int rc = read_scalar(ctx, 6, &v);
     if (rc) return rc;
@@ -1972,6 +1988,12 @@ int ipf_set_cg_graph(ipf_ctx* ctx, int enable) {
     return IPF_OK;
 }
 
+int ipf_set_newton_graph(ipf_ctx* ctx, int enable) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    ctx->newton_graph = enable != 0;
+    return IPF_OK;
+}
+
 int ipf_set_dirichlet_mask(ipf_ctx* ctx, const signed char* mask_host) {
     int rc = check_ready(ctx, false);
     if (rc) return rc;
@@ -1983,20 +2005,116 @@ int ipf_set_dirichlet_mask(ipf_ctx* ctx, const signed char* mask_host) {
     return IPF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The launches of ipf_residual_tangent on stream s: strains, stresses with
+// w Dep, r = f_lin - B^T (w sigma).
+void residual_tangent_launches(ipf_ctx* ctx, const double* u, double* r, hipStream_t s) {
+    for (const auto& g : ctx->groups)
+        launch_multiply(g.shape, group_dev(ctx, g), u, ctx->dStrain, ctx->dOrig, s);
+    for (const auto& g : ctx->groups)
+        launch_sigma(true, ctx->update_mem, g.shape, group_dev(ctx, g), ctx->npad, ctx->dStrain, ctx->dSigw, nullptr,
+                     ctx->dOrig, point_state(ctx), s, ctx->dDep);
+    launch_copy(ctx->neq, ctx->dFlin, r, s);
+    for (const auto& g : ctx->groups)
+        launch_transpose(g.shape, group_dev(ctx, g), ctx->dSigw, r, ctx->dOrig, s);
+}
+
+// The start of a CG solve of K x = b on stream s: the block-Jacobi
+// preconditioner of the current tangent (each node's 2x2 diagonal block
+// inverted where the DoFs pair up, else its diagonal), x = 0, r / z / p, r.z
+// into scalar 0 and ||r||^2 into scalar bb_slot.
+void cg_start(ipf_ctx* ctx, const double* b, double* x, int bb_slot, hipStream_t s) {
+    const int64_t n = ctx->neq;
+    const int nb = dot_blocks(n);
+    double* offd = ctx->pairs_aligned ? ctx->dOffd : nullptr;
+    launch_fill(n, ctx->dDiag, 0.0, s);
+    if (offd) launch_fill(n, offd, 0.0, s);
+    for (const auto& g : ctx->groups)
+        launch_tangent_diag(g.shape, group_dev(ctx, g), ctx->dDiag, offd, ctx->dDep, ctx->npad, s);
+    launch_block_jacobi(n, ctx->dMask, ctx->dDiag, offd, ctx->dPa, ctx->dPb, s);
+    launch_fill(n, x, 0.0, s);
+    launch_cg_init(n, ctx->dMask, b, ctx->dCr, ctx->dPa, ctx->dPb, ctx->dCz, ctx->dCp, s);
+    launch_dot(n, ctx->dCr, ctx->dCz, ctx->dPartial, nb, ctx->dScal + 0, s);
+    launch_dot(n, ctx->dCr, ctx->dCr, ctx->dPartial, nb, ctx->dScal + bb_slot, s);
+}
+
+// CG blocks from iteration *it (relative residual *rel there) on the
+// context's stream: kCgBlock iterations per graph replay (a shorter last
+// block up to maxit launched one by one), ||r|| read after each block, until
+// it drops to rtol * bnorm (or turns NaN) or maxit is reached.
+int cg_blocks(ipf_ctx* ctx, double* x, double rtol, int maxit, double bnorm, int* it, double* rel) {
+    const int64_t n = ctx->neq;
+    const int nb = dot_blocks(n);
+    hipStream_t s = ctx->stream;
+    int rc;
+    while (*it < maxit) {
+        const int blk = std::min(kCgBlock, maxit - *it);
+        hipGraphExec_t exec = blk == kCgBlock ? cg_graph(ctx, x) : nullptr;
+        if (exec)
+            IPF_HIP(ctx, hipGraphLaunch(exec, s));
+        else
+            cg_iterations(ctx, x, blk, s);
+        *it += blk;
+        launch_dot(n, ctx->dCr, ctx->dCr, ctx->dPartial, nb, ctx->dScal + 3, s);
+        double rr;
+        if ((rc = launch_status(ctx, "cg")) || (rc = read_scalar(ctx, 3, &rr))) return rc;
+        *rel = std::sqrt(rr) / bnorm;
+        if (!(*rel > rtol)) break;  // also stops on NaN
+    }
+    return IPF_OK;
+}
+
+// One Newton iteration's device work at iterate u up to its first host
+// read, captured on the context's capture stream: the launches of
+// ipf_residual_tangent into r = dNr (and the FP32 copy of w Dep for the
+// FP32 geometric tangent), ||r||^2 over the free DoFs (scalar 6), the CG
+// start for K dNd = dNr with ||r||^2 in scalar 7, its first kCgBlock
+// iterations and ||r_cg||^2 (scalar 3).  The same kernels in the same order
+// as the eager iteration, so the iterates are the same bits.  nullptr when
+// capture fails (the caller then runs the iteration eagerly).  Needs the
+// solver buffers (and dDep32 for FMT_G32) allocated.
+hipGraphExec_t newton_step_graph(ipf_ctx* ctx, const double* u) {
+    if (!ctx->cap_stream && hipStreamCreate(&ctx->cap_stream) != hipSuccess) {
+        ctx->cap_stream = nullptr;
+        return nullptr;
+    }
+    hipStream_t s = ctx->cap_stream;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    residual_tangent_launches(ctx, u, ctx->dNr, s);
+    if (ctx->tfmt == FMT_G32) launch_to_f32(9 * ctx->npad, ctx->dDep, ctx->dDep32, s);
+    masked_norm_launch(ctx, ctx->dNr, s);
+    cg_start(ctx, ctx->dNr, ctx->dNd, 7, s);
+    cg_iterations(ctx, ctx->dNd, kCgBlock, s);
+    launch_dot(ctx->neq, ctx->dCr, ctx->dCr, ctx->dPartial, dot_blocks(ctx->neq), ctx->dScal + 3, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t e1 = hipStreamEndCapture(s, &graph);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 = (e1 == hipSuccess) ? hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) : e1;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return exec;
+}
+
+}  // namespace
+
+extern "C" {
+
 int ipf_residual_tangent(ipf_ctx* ctx, const double* u_dev, double* r_dev) {
     int rc = check_ready(ctx, true);
     if (rc) return rc;
     if (!u_dev || !r_dev) return fail(ctx, IPF_EINVAL, "ipf_residual_tangent: NULL pointer");
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_solver_buffers(ctx))) return rc;
-    for (const auto& g : ctx->groups)
-        launch_multiply(g.shape, group_dev(ctx, g), u_dev, ctx->dStrain, ctx->dOrig, ctx->stream);
-    for (const auto& g : ctx->groups)
-        launch_sigma(true, ctx->update_mem, g.shape, group_dev(ctx, g), ctx->npad, ctx->dStrain, ctx->dSigw, nullptr,
-                     ctx->dOrig, point_state(ctx), ctx->stream, ctx->dDep);
-    launch_copy(ctx->neq, ctx->dFlin, r_dev, ctx->stream);
-    for (const auto& g : ctx->groups)
-        launch_transpose(g.shape, group_dev(ctx, g), ctx->dSigw, r_dev, ctx->dOrig, ctx->stream);
+    residual_tangent_launches(ctx, u_dev, r_dev, ctx->stream);
     ctx->have_tangent = true;
     ctx->dep32_ok = false;
     if ((rc = ensure_dep32(ctx))) return rc;
@@ -2099,45 +2217,14 @@ int ipf_cg_solve(ipf_ctx* ctx, const double* b_dev, double* x_dev, double rtol, 
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_solver_buffers(ctx))) return rc;
     if ((rc = ensure_dep32(ctx))) return rc;
-    const int64_t n = ctx->neq;
-    const int nb = dot_blocks(n);
-    hipStream_t s = ctx->stream;
-    // block-Jacobi preconditioner of the current tangent: each node's 2x2
-    // diagonal block inverted where the DoFs pair up, else its diagonal
-    double* offd = ctx->pairs_aligned ? ctx->dOffd : nullptr;
-    launch_fill(n, ctx->dDiag, 0.0, s);
-    if (offd) launch_fill(n, offd, 0.0, s);
-    for (const auto& g : ctx->groups)
-        launch_tangent_diag(g.shape, group_dev(ctx, g), ctx->dDiag, offd, ctx->dDep, ctx->npad, s);
-    launch_block_jacobi(n, ctx->dMask, ctx->dDiag, offd, ctx->dPa, ctx->dPb, s);
-    launch_fill(n, x_dev, 0.0, s);
-    launch_cg_init(n, ctx->dMask, b_dev, ctx->dCr, ctx->dPa, ctx->dPb, ctx->dCz, ctx->dCp, s);
-    launch_dot(n, ctx->dCr, ctx->dCz, ctx->dPartial, nb, ctx->dScal + 0, s);
-    launch_dot(n, ctx->dCr, ctx->dCr, ctx->dPartial, nb, ctx->dScal + 3, s);
+    cg_start(ctx, b_dev, x_dev, 3, ctx->stream);
     double bb;
     if ((rc = read_scalar(ctx, 3, &bb))) return rc;
     const double bnorm = std::sqrt(bb);
     int it = 0;
     double rel = 0.0;
-    if (bnorm > 0.0) {
-        // q = K p on the free DoFs.  Blocks of kCgBlock iterations (one
-        // graph launch) between convergence checks; a shorter last block up
-        // to maxit.
-        while (it < maxit) {
-            const int blk = std::min(kCgBlock, maxit - it);
-            hipGraphExec_t exec = blk == kCgBlock ? cg_graph(ctx, x_dev) : nullptr;
-            if (exec)
-                IPF_HIP(ctx, hipGraphLaunch(exec, s));
-            else
-                cg_iterations(ctx, x_dev, blk, s);
-            it += blk;
-            launch_dot(n, ctx->dCr, ctx->dCr, ctx->dPartial, nb, ctx->dScal + 3, s);
-            double rr;
-            if ((rc = launch_status(ctx, "cg")) || (rc = read_scalar(ctx, 3, &rr))) return rc;
-            rel = std::sqrt(rr) / bnorm;
-            if (!(rel > rtol)) break;  // also stops on NaN
-        }
-    }
+    // q = K p on the free DoFs, in blocks between convergence checks
+    if (bnorm > 0.0 && (rc = cg_blocks(ctx, x_dev, rtol, maxit, bnorm, &it, &rel))) return rc;
     if (iters) *iters = it;
     if (relres) *relres = rel;
     return launch_status(ctx, "cg");
@@ -2156,9 +2243,32 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
     double r0 = 0.0, rn = 0.0, worst_cg = 0.0;
     int it = 0, cg_total = 0, ls_halved = 0;
     bool converged = false;
-    for (it = 0; it <= maxit; it++) {
-        if ((rc = ipf_residual_tangent(ctx, u_dev, ctx->dNr))) break;
-        if ((rc = masked_norm(ctx, ctx->dNr, &rn))) break;
+    // each iteration up to its first host read as one graph replay (when
+    // the first CG block fits in cg_maxit and the capture succeeds)
+    hipGraphExec_t step = nullptr;
+    if (ctx->newton_graph && cg_maxit >= kCgBlock && cg_rtol > 0) {
+        if (ctx->tfmt == FMT_G32 && !ctx->dDep32) rc = dev_alloc(ctx, &ctx->dDep32, 9 * ctx->npad);
+        if (!rc) step = newton_step_graph(ctx, u_dev);
+    }
+    for (it = 0; !rc && it <= maxit; it++) {
+        double bb = 0.0, rr = 0.0;
+        if (step) {
+            const hipError_t e = hipGraphLaunch(step, ctx->stream);
+            if (e != hipSuccess) {
+                rc = fail(ctx, IPF_EHIP, "newton step graph launch failed: %s", hipGetErrorString(e));
+                break;
+            }
+            ctx->have_tangent = true;
+            ctx->dep32_ok = ctx->tfmt == FMT_G32;
+            double v[8];
+            if ((rc = launch_status(ctx, "newton step")) || (rc = read_scalars(ctx, 8, v))) break;
+            rn = std::sqrt(v[6]);
+            bb = v[7];
+            rr = v[3];
+        } else {
+            if ((rc = ipf_residual_tangent(ctx, u_dev, ctx->dNr))) break;
+            if ((rc = masked_norm(ctx, ctx->dNr, &rn))) break;
+        }
         if (it == 0) r0 = rn;
         if (ctx->monitor) ctx->monitor(ctx->monitor_user, it, r0 > 0 ? rn / r0 : 0.0, rn, cg_total);
         if (rn <= rtol * r0 || rn <= ctx->newton_atol || rn == 0.0) {
@@ -2168,7 +2278,19 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
         if (it == maxit) break;
         int cgi = 0;
         double cgr = 0.0;
-        if ((rc = ipf_cg_solve(ctx, ctx->dNr, ctx->dNd, cg_rtol, cg_maxit, &cgi, &cgr))) break;
+        if (step) {
+            // the CG's start and first block ran in the graph: continue as
+            // ipf_cg_solve does after its first block
+            const double bnorm = std::sqrt(bb);
+            if (bnorm > 0.0) {
+                cgi = kCgBlock;
+                cgr = std::sqrt(rr) / bnorm;
+                if (cgr > cg_rtol && (rc = cg_blocks(ctx, ctx->dNd, cg_rtol, cg_maxit, bnorm, &cgi, &cgr))) break;
+            }
+            if ((rc = launch_status(ctx, "cg"))) break;
+        } else if ((rc = ipf_cg_solve(ctx, ctx->dNr, ctx->dNd, cg_rtol, cg_maxit, &cgi, &cgr))) {
+            break;
+        }
         cg_total += cgi;
         worst_cg = std::max(worst_cg, cgr);
         if (ctx->ls_halvings == 0) {
@@ -2190,6 +2312,7 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
         }
         if (rc) break;
     }
+    if (step) (void)hipGraphExecDestroy(step);
     // AcceptPseudoTimeStepSolution: one residual with UpdateMem on
     if (!rc) {
         ctx->update_mem = true;

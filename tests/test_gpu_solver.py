@@ -148,19 +148,24 @@ def test_elastic_limit_matches_thick_cylinder(gpu):
 
 
 def test_cg_graph_matches_single_launches(gpu):
-    """The CG iterations replayed from the captured HIP graph give the same
-    Newton iterates, bit for bit, as the same kernels launched one by one: 10
-    load steps of config 1 with the graph on and off (every step compared;
-    the graph is replayed hundreds of times with eager residual / tangent
-    work and host reads between the replays), then a B-format change (the
-    graph is rebuilt) and a further step."""
+    """The Newton iterations replayed from captured HIP graphs give the same
+    iterates, bit for bit, as the same kernels launched one by one: 10 load
+    steps of config 1 with (a) the Newton step graph (tangent residual,
+    residual norm, CG start and first CG block per iteration) and the CG
+    block graph, (b) the CG block graph only, (c) everything eager (every
+    step compared; the graphs are replayed hundreds of times with eager
+    residual / tangent work and host reads between the replays), then a
+    B-format change (the graphs are rebuilt) and a further step each with the
+    compact FP64 and the FP32 geometric tangent (its FP32 copy of w Dep is
+    made inside the step graph)."""
     import time
 
     import torch
     res = {}
-    for use_graph in (True, False):
+    for mode in ("step+cg", "cg", "eager"):
         m, sm, ctx, B, ix, mask = _setup("40x40-q2")
-        ctx.set_cg_graph(use_graph)
+        ctx.set_cg_graph(mode != "eager")
+        ctx.set_newton_graph(mode == "step+cg")
         neq = sm.sizes()["neq"]
         f_unit = ix["rhs_linear"] / wb.normal_stress(wb.NSTEPS)
         u = torch.zeros(neq, dtype=torch.float64, device=gpu)
@@ -169,19 +174,20 @@ def test_cg_graph_matches_single_launches(gpu):
         for k in range(1, wb.NSTEPS + 1):
             ctx.set_rhs_linear(f_unit * wb.normal_stress(k))
             st = ctx.newton(u, rtol=1e-12, maxit=40, cg_rtol=1e-13, cg_maxit=40000)
-            assert st["converged"] == 1.0, (use_graph, k, st)
-            steps.append((u.cpu().numpy().copy(), int(st["iterations"]), int(st["cg_iterations"])))
+            assert st["converged"] == 1.0, (mode, k, st)
+            steps.append((u.cpu().numpy().copy(), int(st["iterations"]), int(st["cg_iterations"]), st["r"]))
         elapsed = time.time() - t0
-        ctx.set_b_format(ctx.b_format()[0], tangent="compact-fp64")
-        ctx.set_rhs_linear(f_unit * wb.normal_stress(wb.NSTEPS) * 1.01)
-        st = ctx.newton(u, rtol=1e-12, maxit=40, cg_rtol=1e-13, cg_maxit=40000)
-        steps.append((u.cpu().numpy().copy(), int(st["iterations"]), int(st["cg_iterations"])))
-        res[use_graph] = (steps, elapsed, ctx.get_state())
-        print(f"graph {use_graph}: 10 load steps {elapsed:.2f} s, cg iterations {sum(s[2] for s in steps[:10])}",
-              flush=True)
-    for (ug, ig, cg), (ue, ie, ce) in zip(res[True][0], res[False][0]):
-        assert np.array_equal(ug, ue) and ig == ie and cg == ce
-    assert all(np.array_equal(a, b) for a, b in zip(res[True][2], res[False][2]))
+        for tangent, scale in (("compact-fp64", 1.01), ("geometry-fp32", 1.02)):
+            ctx.set_b_format(ctx.b_format()[0], tangent=tangent)
+            ctx.set_rhs_linear(f_unit * wb.normal_stress(wb.NSTEPS) * scale)
+            st = ctx.newton(u, rtol=1e-10, maxit=40, cg_rtol=1e-8, cg_maxit=40000)
+            steps.append((u.cpu().numpy().copy(), int(st["iterations"]), int(st["cg_iterations"]), st["r"]))
+        res[mode] = (steps, elapsed, ctx.get_state())
+        print(f"{mode}: 10 load steps {elapsed:.2f} s, cg iterations {sum(s[2] for s in steps[:10])}", flush=True)
+    for mode in ("step+cg", "cg"):
+        for (ug, ig, cg, rg), (ue, ie, ce, re) in zip(res[mode][0], res["eager"][0]):
+            assert np.array_equal(ug, ue) and ig == ie and cg == ce and rg == re, mode
+        assert all(np.array_equal(a, b) for a, b in zip(res[mode][2], res["eager"][2])), mode
 
 
 def test_newton_at_a_converged_u_stops_on_the_absolute_floor(gpu):
