@@ -213,10 +213,10 @@ def event_ms(fn, stream, reps=10):
 
 
 def profile_figures(workload, kname):
-    """HBM bytes per launch (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE, calibrated
-    on a copy kernel of known size) and VALU issue fraction of kernel `kname`
-    for `workload`, from the committed profile summary (profiles/); (None,
-    None) when the profile does not cover this kernel."""
+    """HBM bytes per launch (rocprofv3 PMC: the L2's DRAM read and write
+    requests, checked on a copy kernel of known size) and VALU issue fraction
+    of kernel `kname` for `workload`, from the committed profile summary
+    (profiles/); (None, None) when the profile does not cover this kernel."""
     tfile = os.path.join(ROOT, "profiles", f"k_residual_{workload}_traffic.json")
     if not os.path.exists(tfile):
         return None, None
