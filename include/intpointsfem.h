@@ -473,6 +473,18 @@ int ipf_set_newton_monitor(ipf_ctx *ctx, ipf_newton_monitor fn, void *user);
  * segment): also converged when ||r|| <= atol.  atol >= 0; 0 turns it off. */
 int ipf_set_newton_atol(ipf_ctx *ctx, double atol);
 
+/* The reference's stop test for ipf_newton (tolerance > 0; 0, the default,
+ * restores the relative test above): converged at the first iterate, after
+ * at least one update, with ||r|| < tolerance and ||du|| < tolerance, du the
+ * update that led to it -- main.cpp:239 (`norm_res < tolerance &
+ * norm_delta_du < tolerance`, tolerance 1e-7 at main.cpp:139), the
+ * conventional (non-accelerated) update of main.cpp:226-229.  ||r|| is taken
+ * over the free DoFs, ||du|| over all DoFs (du is zero on the others); with
+ * the line search du is the step taken.  rtol and ipf_set_newton_atol are
+ * then unused.  Not used by ipf_newton_segmented.  EINVAL unless tolerance
+ * is finite and >= 0. */
+int ipf_set_newton_reference_stop(ipf_ctx *ctx, double tolerance);
+
 /* ipf_newton for independent blocks of equations (batched instances): segment
  * k holds the equations [seg_first[k], seg_first[k+1]) (seg_first [nseg+1],
  * from 0 to neq) and converges on its own criterion ||r_k|| <= rtol ||r0_k||

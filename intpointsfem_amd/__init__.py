@@ -109,6 +109,7 @@ def lib():
         "ipf_set_cg_graph": (_int, [_vp, _int]),
         "ipf_set_newton_graph": (_int, [_vp, _int]),
         "ipf_set_newton_atol": (_int, [_vp, _d]),
+        "ipf_set_newton_reference_stop": (_int, [_vp, _d]),
         "ipf_set_newton_line_search": (_int, [_vp, _int]),
         "ipf_set_newton_monitor": (_int, [_vp, NEWTON_MONITOR, _vp]),
         "ipf_set_residual_mode": (_int, [_vp, _int]),
@@ -648,6 +649,12 @@ class DeviceContext:
 
     def set_newton_atol(self, atol):
         self._c(lib().ipf_set_newton_atol(self._h, float(atol)), "ipf_set_newton_atol")
+
+    def set_newton_reference_stop(self, tolerance):
+        """Newton stops as the reference's loop does (main.cpp:239): after an
+        update, ||r|| < tolerance and ||du|| < tolerance; 0 restores the
+        relative test."""
+        self._c(lib().ipf_set_newton_reference_stop(self._h, float(tolerance)), "ipf_set_newton_reference_stop")
 
     def set_newton_monitor(self, fn):
         """fn(iteration, rel, res_norm, cg_iterations) after every residual

@@ -173,6 +173,9 @@ struct ipf_ctx {
     bool have_data = false;
     bool update_mem = false;
     double newton_atol = 0.0;  // absolute residual floor of ipf_newton[_segmented]
+    // ipf_newton's stop test of the reference (main.cpp:239) when > 0:
+    // ||r|| < tol and ||du|| < tol after an update
+    double newton_ref_tol = 0.0;
     ipf_newton_monitor monitor = nullptr;  // per-iteration report of ipf_newton[_segmented]
     void* monitor_user = nullptr;
     int residual_mode = IPF_RESIDUAL_EXACT;  // ipf_set_residual_mode
@@ -1664,6 +1667,14 @@ int ipf_set_newton_atol(ipf_ctx* ctx, double atol) {
     return IPF_OK;
 }
 
+int ipf_set_newton_reference_stop(ipf_ctx* ctx, double tolerance) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_newton_reference_stop: NULL context");
+    if (!(tolerance >= 0.0) || tolerance > 1.7976931348623157e308)
+        return fail(ctx, IPF_EINVAL, "ipf_set_newton_reference_stop: tolerance must be finite and >= 0");
+    ctx->newton_ref_tol = tolerance;
+    return IPF_OK;
+}
+
 int ipf_set_update_mem(ipf_ctx* ctx, int update_mem) {
     if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
     ctx->update_mem = update_mem != 0;
@@ -2240,7 +2251,7 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
     // residuals must not commit, only the final AcceptPseudoTimeStepSolution.
     const bool commit_flag = ctx->update_mem;
     ctx->update_mem = false;
-    double r0 = 0.0, rn = 0.0, worst_cg = 0.0;
+    double r0 = 0.0, rn = 0.0, worst_cg = 0.0, du_norm = 0.0;
     int it = 0, cg_total = 0, ls_halved = 0;
     bool converged = false;
     // each iteration up to its first host read as one graph replay (when
@@ -2271,7 +2282,9 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
         }
         if (it == 0) r0 = rn;
         if (ctx->monitor) ctx->monitor(ctx->monitor_user, it, r0 > 0 ? rn / r0 : 0.0, rn, cg_total);
-        if (rn <= rtol * r0 || rn <= ctx->newton_atol || rn == 0.0) {
+        const double ref_tol = ctx->newton_ref_tol;
+        if (ref_tol > 0.0 ? (it > 0 && rn < ref_tol && du_norm < ref_tol)
+                          : (rn <= rtol * r0 || rn <= ctx->newton_atol || rn == 0.0)) {
             converged = true;
             break;
         }
@@ -2293,6 +2306,13 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
         }
         cg_total += cgi;
         worst_cg = std::max(worst_cg, cgr);
+        if (ctx->newton_ref_tol > 0.0) {  // ||du|| of the full step (all DoFs, as main.cpp:227)
+            launch_dot(ctx->neq, ctx->dNd, ctx->dNd, ctx->dPartial, dot_blocks(ctx->neq), ctx->dScal + 8,
+                       ctx->stream);
+            double dd;
+            if ((rc = read_scalar(ctx, 8, &dd))) break;
+            du_norm = std::sqrt(dd);
+        }
         if (ctx->ls_halvings == 0) {
             launch_axpy(ctx->neq, 1.0, ctx->dNd, u_dev, ctx->stream);
             continue;
@@ -2311,6 +2331,7 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
             ls_halved++;
         }
         if (rc) break;
+        du_norm *= t;  // the step taken is t du (t a power of 2: exact)
     }
     if (step) (void)hipGraphExecDestroy(step);
     // AcceptPseudoTimeStepSolution: one residual with UpdateMem on

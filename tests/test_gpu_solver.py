@@ -311,3 +311,39 @@ def test_newton_line_search_converges_with_a_loose_inner_tolerance(gpu):
     err = np.abs(sols[8] - sols[0]).max() / np.abs(sols[0]).max()
     print(f"|u_ls - u_tight| / max|u| = {err:.2e}")
     assert err <= 1e-5
+
+
+def test_newton_reference_stop_rule(gpu):
+    """The reference's stop test (main.cpp:239, tolerance 1e-7 at :139):
+    with ipf_set_newton_reference_stop(1e-7), ipf_newton on load step 1 of
+    config 1 stops after N updates at an iterate with ||r|| < 1e-7; a second
+    context run with maxit N - 1 reaches iterate N - 1 (the same kernels, the
+    same bits) and reports it not converged -- the rule failed there -- and the
+    last update u_N - u_(N-1) has norm < 1e-7.  The setter rejects negative
+    and non-finite tolerances; 0 restores the relative test."""
+    import torch
+    tol = 1e-7
+    runs = []
+    for maxit in (40, None):
+        m, sm, ctx, B, ix, mask = _setup("40x40-q2")
+        ctx.set_newton_reference_stop(tol)
+        ctx.set_rhs_linear(ix["rhs_linear"] / wb.normal_stress(wb.NSTEPS) * wb.normal_stress(1))
+        u = torch.zeros(sm.sizes()["neq"], dtype=torch.float64, device=gpu)
+        n = maxit if maxit is not None else int(runs[0][1]["iterations"]) - 1
+        st = ctx.newton(u, rtol=1e-30, maxit=n, cg_rtol=1e-13, cg_maxit=40000)
+        runs.append((u.cpu().numpy(), st, sm, ctx))
+        print(f"maxit {n}: {st}", flush=True)
+    (ua, sa, sm, ctx), (ub, sb, _, _) = runs
+    assert sa["converged"] == 1.0 and sa["iterations"] >= 2 and sa["r"] < tol
+    assert sb["converged"] == 0.0 and sb["iterations"] == sa["iterations"] - 1
+    assert np.linalg.norm(ua - ub) < tol
+    for bad in (-1.0, float("nan"), float("inf")):
+        with pytest.raises(ipf.IPFError):
+            ctx.set_newton_reference_stop(bad)
+    # 0 restores the relative test (with its absolute floor): converged at
+    # once at the converged u of the first run
+    ctx.set_newton_reference_stop(0.0)
+    ctx.set_newton_atol(1e-6)
+    u = torch.as_tensor(ua, device=gpu)
+    st = ctx.newton(u, rtol=1e-12, maxit=5, cg_rtol=1e-13, cg_maxit=40000)
+    assert st["converged"] == 1.0 and st["iterations"] == 0, st
