@@ -442,7 +442,7 @@ int ipf_cg_solve(ipf_ctx *ctx, const double *b_dev, double *x_dev, double rtol, 
  * converged u (AcceptPseudoTimeStepSolution, main.cpp:487-522).
  * Solution / Newton loop of main.cpp:160-262 with CG in place of LDL^T; the
  * reference stops on absolute norms, ||r|| < tol and ||du|| < tol
- * (main.cpp:235) -- the relative test here, the absolute floor for a call
+ * (main.cpp:239; ipf_set_newton_reference_stop) -- the relative test here, the absolute floor for a call
  * that starts at an already converged u (its r0 is at rounding level, so the
  * relative test alone could not be met).  The caller's UpdateMem switch is
  * the same after the call, whatever its status.
