@@ -485,6 +485,17 @@ int ipf_set_newton_atol(ipf_ctx *ctx, double atol);
  * is finite and >= 0. */
 int ipf_set_newton_reference_stop(ipf_ctx *ctx, double tolerance);
 
+/* The reference's accelerated update for ipf_newton (enable != 0; off by
+ * default, as modified_thomas_accel_Q at main.cpp:76): after the Newton step
+ * du of an iteration, u += alpha du, the residual there with the tangent left
+ * unchanged, its correction du~ solved with that same tangent,
+ * ||du|| := ||alpha du + du~|| (for ipf_set_newton_reference_stop),
+ * alpha += (alpha du . du~) / (alpha du . alpha du), u += du~ -- main.cpp:
+ * 194-224, with alpha = 1 at the start of each call (main.cpp:171).  Two CG
+ * solves per iteration (both counted in stats[5]).  EINVAL from ipf_newton
+ * when the line search is also on.  Not used by ipf_newton_segmented. */
+int ipf_set_newton_thomas(ipf_ctx *ctx, int enable);
+
 /* ipf_newton for independent blocks of equations (batched instances): segment
  * k holds the equations [seg_first[k], seg_first[k+1]) (seg_first [nseg+1],
  * from 0 to neq) and converges on its own criterion ||r_k|| <= rtol ||r0_k||

@@ -110,6 +110,7 @@ def lib():
         "ipf_set_newton_graph": (_int, [_vp, _int]),
         "ipf_set_newton_atol": (_int, [_vp, _d]),
         "ipf_set_newton_reference_stop": (_int, [_vp, _d]),
+        "ipf_set_newton_thomas": (_int, [_vp, _int]),
         "ipf_set_newton_line_search": (_int, [_vp, _int]),
         "ipf_set_newton_monitor": (_int, [_vp, NEWTON_MONITOR, _vp]),
         "ipf_set_residual_mode": (_int, [_vp, _int]),
@@ -649,6 +650,11 @@ class DeviceContext:
 
     def set_newton_atol(self, atol):
         self._c(lib().ipf_set_newton_atol(self._h, float(atol)), "ipf_set_newton_atol")
+
+    def set_newton_thomas(self, enable):
+        """The reference's modified Thomas acceleration of the Newton update
+        (main.cpp:194-224); off by default."""
+        self._c(lib().ipf_set_newton_thomas(self._h, 1 if enable else 0), "ipf_set_newton_thomas")
 
     def set_newton_reference_stop(self, tolerance):
         """Newton stops as the reference's loop does (main.cpp:239): after an

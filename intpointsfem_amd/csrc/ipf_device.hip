@@ -263,6 +263,8 @@ struct ipf_ctx {
     double* dNd = nullptr;      // Newton correction
     double* dLs = nullptr;      // u before a line-searched Newton update
     int ls_halvings = 0;        // ipf_set_newton_line_search (0: full Newton steps)
+    double* dNt = nullptr;      // the Newton step while the Thomas correction is solved
+    bool newton_thomas = false; // ipf_set_newton_thomas
     double* dPartial = nullptr; // dot partials
     double* dScal = nullptr;    // CG scalars
     bool have_mask = false;
@@ -403,6 +405,7 @@ void release_data(ipf_ctx* ctx) {
     dev_free(ctx->dNr);
     dev_free(ctx->dNd);
     dev_free(ctx->dLs);
+    dev_free(ctx->dNt);
     dev_free(ctx->dPartial);
     dev_free(ctx->dScal);
     for (int f = 1; f < FMT_COUNT; f++) dev_free(ctx->dBf[f]);
@@ -1667,6 +1670,12 @@ int ipf_set_newton_atol(ipf_ctx* ctx, double atol) {
     return IPF_OK;
 }
 
+int ipf_set_newton_thomas(ipf_ctx* ctx, int enable) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_newton_thomas: NULL context");
+    ctx->newton_thomas = enable != 0;
+    return IPF_OK;
+}
+
 int ipf_set_newton_reference_stop(ipf_ctx* ctx, double tolerance) {
     if (!ctx) return fail(nullptr, IPF_EINVAL, "ipf_set_newton_reference_stop: NULL context");
     if (!(tolerance >= 0.0) || tolerance > 1.7976931348623157e308)
@@ -2245,13 +2254,17 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
     int rc = check_ready(ctx, true);
     if (rc) return rc;
     if (!u_dev) return fail(ctx, IPF_EINVAL, "ipf_newton: NULL pointer");
+    if (ctx->newton_thomas && ctx->ls_halvings > 0)
+        return fail(ctx, IPF_EINVAL, "ipf_newton: the Thomas acceleration and the line search exclude each other");
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_solver_buffers(ctx))) return rc;
+    if (ctx->newton_thomas && !ctx->dNt && (rc = dev_alloc(ctx, &ctx->dNt, ctx->neq))) return rc;
     // The caller's UpdateMem switch is restored on every exit: the Newton
     // residuals must not commit, only the final AcceptPseudoTimeStepSolution.
     const bool commit_flag = ctx->update_mem;
     ctx->update_mem = false;
     double r0 = 0.0, rn = 0.0, worst_cg = 0.0, du_norm = 0.0;
+    double mt_alpha = 1.0;  // the Thomas acceleration factor, main.cpp:171
     int it = 0, cg_total = 0, ls_halved = 0;
     bool converged = false;
     // each iteration up to its first host read as one graph replay (when
@@ -2306,6 +2319,34 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
         }
         cg_total += cgi;
         worst_cg = std::max(worst_cg, cgr);
+        if (ctx->newton_thomas) {
+            // main.cpp:194-224: u += alpha du; the residual there with the
+            // tangent unchanged; its correction du~ from the same tangent;
+            // ||alpha du + du~||; alpha += (alpha du . du~) / |alpha du|^2;
+            // u += du~.  du is kept in dNt while du~ is solved into dNd (the
+            // CG graph stays keyed on dNd).
+            hipStream_t s = ctx->stream;
+            const int64_t n = ctx->neq;
+            const int nb = dot_blocks(n);
+            launch_axpy(n, mt_alpha, ctx->dNd, u_dev, s);
+            launch_copy(n, ctx->dNd, ctx->dNt, s);
+            if ((rc = ipf_residual(ctx, u_dev, ctx->dNr))) break;
+            if ((rc = ipf_cg_solve(ctx, ctx->dNr, ctx->dNd, cg_rtol, cg_maxit, &cgi, &cgr))) break;
+            cg_total += cgi;
+            worst_cg = std::max(worst_cg, cgr);
+            launch_copy(n, ctx->dNd, ctx->dCq, s);  // alpha du + du~ (q is free after the CG)
+            launch_axpy(n, mt_alpha, ctx->dNt, ctx->dCq, s);
+            launch_dot(n, ctx->dCq, ctx->dCq, ctx->dPartial, nb, ctx->dScal + 8, s);
+            launch_dot(n, ctx->dNt, ctx->dNd, ctx->dPartial, nb, ctx->dScal + 9, s);
+            launch_dot(n, ctx->dNt, ctx->dNt, ctx->dPartial, nb, ctx->dScal + 10, s);
+            launch_axpy(n, 1.0, ctx->dNd, u_dev, s);
+            double v[11];
+            if ((rc = launch_status(ctx, "newton thomas")) || (rc = read_scalars(ctx, 11, v))) break;
+            du_norm = std::sqrt(v[8]);
+            const double num = mt_alpha * v[9], dem = mt_alpha * mt_alpha * v[10];
+            if (dem > 0.0) mt_alpha += num / dem;  // (du = 0 only at r = 0, which stops first)
+            continue;
+        }
         if (ctx->newton_ref_tol > 0.0) {  // ||du|| of the full step (all DoFs, as main.cpp:227)
             launch_dot(ctx->neq, ctx->dNd, ctx->dNd, ctx->dPartial, dot_blocks(ctx->neq), ctx->dScal + 8,
                        ctx->stream);

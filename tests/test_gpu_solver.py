@@ -347,3 +347,75 @@ def test_newton_reference_stop_rule(gpu):
     u = torch.as_tensor(ua, device=gpu)
     st = ctx.newton(u, rtol=1e-12, maxit=5, cg_rtol=1e-13, cg_maxit=40000)
     assert st["converged"] == 1.0 and st["iterations"] == 0, st
+
+
+def test_newton_thomas_acceleration_matches_the_reference_loop(gpu):
+    """ipf_set_newton_thomas against the reference's accelerated loop
+    (main.cpp:160-262 with modified_thomas_accel_Q, :194-224) restated on the
+    host with the library's primitives -- Assemble = ipf_residual_tangent,
+    AssembleResidual = ipf_residual (the tangent stays), Solve =
+    ipf_cg_solve -- and the reference's stop test (:239, tolerance 1e-7):
+    load steps 1-4 of config 1, the same Newton iteration counts and the same
+    u to 1e-10 relative (the host loop's vector updates and dots round in
+    torch; measured: the same bits); both end within 1e-7 relative of the
+    plain Newton solution (both stop at |r| < 1e-7, |du| < 1e-7; measured
+    1.9e-8).  The
+    line search and the acceleration together are rejected."""
+    import torch
+    tol, cg_rtol = 1e-7, 1e-13
+    sols = {}
+    for mode in ("library", "host", "plain"):
+        m, sm, ctx, B, ix, mask = _setup("40x40-q2")
+        free = torch.as_tensor(~mask, device=gpu)
+        f_unit = ix["rhs_linear"] / wb.normal_stress(wb.NSTEPS)
+        neq = sm.sizes()["neq"]
+        u = torch.zeros(neq, dtype=torch.float64, device=gpu)
+        ctx.set_newton_reference_stop(tol)
+        ctx.set_newton_thomas(mode == "library")
+        its = []
+        for k in range(1, 5):
+            ctx.set_rhs_linear(f_unit * wb.normal_stress(k))
+            if mode != "host":
+                st = ctx.newton(u, rtol=1e-30, maxit=40, cg_rtol=cg_rtol, cg_maxit=40000)
+                assert st["converged"] == 1.0, (mode, k, st)
+                its.append(int(st["iterations"]))
+                continue
+            r = torch.empty_like(u)
+            d = torch.empty_like(u)
+            t = torch.empty_like(u)
+            ctx.residual_tangent(u, r)
+            alpha, n = 1.0, 0
+            for i in range(40):
+                ctx.cg_solve(r, d, rtol=cg_rtol, maxit=40000)
+                u += alpha * d
+                ctx.residual(u, r)
+                ctx.cg_solve(r, t, rtol=cg_rtol, maxit=40000)
+                du_norm = float(torch.linalg.norm(alpha * d + t))
+                num = float(torch.dot(alpha * d, t))
+                dem = float(torch.dot(alpha * d, alpha * d))
+                alpha += num / dem
+                u += t
+                ctx.residual_tangent(u, r)
+                n = i + 1
+                if float(torch.linalg.norm(r[free])) < tol and du_norm < tol:
+                    break
+            else:
+                raise AssertionError(f"host loop did not converge at load step {k}")
+            its.append(n)
+            ctx.set_update_mem(True)
+            ctx.residual(u, r)
+            ctx.set_update_mem(False)
+        sols[mode] = (u.cpu().numpy(), its)
+        print(f"{mode}: Newton iterations per load step {its}", flush=True)
+        if mode == "library":
+            ctx.set_newton_line_search(4)
+            with pytest.raises(ipf.IPFError):
+                ctx.newton(u, rtol=1e-30, maxit=2, cg_rtol=cg_rtol, cg_maxit=40000)
+    ul, il = sols["library"]
+    uh, ih = sols["host"]
+    up, _ = sols["plain"]
+    print(f"max |u| {np.abs(up).max():.3e}; library - host {np.abs(ul - uh).max():.3e}, "
+          f"library - plain {np.abs(ul - up).max():.3e}", flush=True)
+    assert il == ih
+    assert np.abs(ul - uh).max() <= 1e-10 * np.abs(uh).max()
+    assert np.abs(ul - up).max() <= 1e-7 * np.abs(up).max()
