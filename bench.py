@@ -686,8 +686,8 @@ def main():
     blocks = sm.export_blocks()
     roof = launch_roofline(ctx, u, r, nrep, blocks, npts, neq, fmt0, step_ms=ms_per_step)
     # one untimed ipf_residual_tangent: its k_copy (f_lin into r, neq doubles
-    # read and written) is the known-size dispatch that calibrates the PMC
-    # byte counters of a profiled run (the residual itself needs no copy)
+    # read and written) is the known-size dispatch that checks the PMC byte
+    # counters of a profiled run (the residual itself needs no copy)
     ctx.residual_tangent(u, torch.empty_like(r))
     ctx.synchronize()
     kname = kernel_name(fmt0, shape)
