@@ -346,7 +346,7 @@ int ipf_get_b_format(const ipf_ctx *ctx, int *residual_format, int *tangent_form
  * [npts] 1e3 * the alpha increment of that return. */
 int ipf_point_features(ipf_ctx *ctx, const double *u_dev, float *feat_dev, int *branch_dev, float *target_dev);
 
-/* A 7 -> 64 -> 64 -> 6 MLP (GELU; BF16 weights and activations on the
+/* A 7 -> 64 -> 64 -> 6 MLP (ReLU; BF16 weights and activations on the
  * matrix cores, FP32 accumulation): 5 branch logits + the alpha increment.
  * Parameters are NPAR FP32 values (ipf_mlp_num_params; padded entries stay
  * zero), Glorot-uniform from `seed`.  Samples are device arrays in the

@@ -716,7 +716,7 @@ class DeviceContext:
 
 class MLP:
     """The learned return-branch predictor (``ipf_mlp``): 7 -> 64 -> 64 -> 6,
-    GELU, BF16 matrix-core operands, FP32 accumulation, Adam."""
+    ReLU, BF16 matrix-core operands, FP32 accumulation, Adam."""
 
     def __init__(self, device=0, seed=0):
         h = _vp()

@@ -38,8 +38,8 @@ DEVICE = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 # Per-source device flags.  ipf_learned.hip: MFMA results in VGPRs (the
-# inference loop applies GELU to every MFMA result; with the results in AGPRs
-# each one is first copied out, 36 extra instructions per 16 samples).
+# inference loop applies the activation to every MFMA result; with the
+# results in AGPRs each one is first copied out).
 EXTRA = {
     "ipf_learned.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
 }

@@ -1,5 +1,5 @@
 // ipf_learned.hip -- the learned return-branch predictor: a 7 -> 64 -> 64 -> 6
-// MLP (GELU, BF16 weights and activations as MFMA operands, FP32
+// MLP (ReLU, BF16 weights and activations as MFMA operands, FP32
 // accumulation) trained on the GPU on pairs the exact return map emits.
 //
 //   inputs  x = (s1/c, s2/c, s3/c, ln c, phi, psi, alpha): trial principal
@@ -43,18 +43,10 @@ __device__ __forceinline__ short f2bf(float x) {
 }
 __device__ __forceinline__ float bf2f(short h) { return __uint_as_float(((uint32_t)(unsigned short)h) << 16); }
 
-// GELU, tanh form (Hendrycks & Gimpel 2016), and its derivative
-__device__ __forceinline__ float gelu_t(float x) {
-    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-    const float y = k0 * (x + k1 * x * x * x);
-    return tanhf(y);
-}
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + gelu_t(x)); }
-__device__ __forceinline__ float gelu_grad(float x) {
-    const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-    const float t = gelu_t(x);
-    return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * k0 * (1.f + 3.f * k1 * x * x);
-}
+// ReLU (one v_max_f32; a NaN input gives 0) and its derivative (0 at and
+// below 0)
+__device__ __forceinline__ float relu(float x) { return __builtin_fmaxf(x, 0.f); }
+__device__ __forceinline__ float relu_grad(float x) { return x > 0.f ? 1.f : 0.f; }
 
 // C += A (16 x K) B (K x 16); a(m, k), b(k, n) return BF16 bits.
 template <class FA, class FB>
@@ -123,7 +115,7 @@ __device__ void forward16(const WeightsLDS& w, WaveLDS& s, const float* __restri
         for (int i = 0; i < 4; i++) {
             const float z = c[i] + w.b1[nt * 16 + r];
             s.Z1[kq + i][nt * 16 + r] = z;
-            s.H1[kq + i][nt * 16 + r] = f2bf(gelu(z));
+            s.H1[kq + i][nt * 16 + r] = f2bf(relu(z));
         }
     }
     __syncthreads();
@@ -135,7 +127,7 @@ __device__ void forward16(const WeightsLDS& w, WaveLDS& s, const float* __restri
         for (int i = 0; i < 4; i++) {
             const float z = c[i] + w.b2[nt * 16 + r];
             s.Z2[kq + i][nt * 16 + r] = z;
-            s.H2[kq + i][nt * 16 + r] = f2bf(gelu(z));
+            s.H2[kq + i][nt * 16 + r] = f2bf(relu(z));
         }
     }
     __syncthreads();
@@ -215,7 +207,7 @@ k_mlp_grad(const short* __restrict__ wbf, const float* __restrict__ par, const f
         __syncthreads();
         if (lane == 0)
             for (int m = 0; m < 16; m++) loss += s.L[m];
-        // layer 3: dW3 += H2^T D3, db3, dZ2 = (D3 W3^T) gelu'(Z2)
+        // layer 3: dW3 += H2^T D3, db3, dZ2 = (D3 W3^T) relu'(Z2)
 #pragma unroll
         for (int mt = 0; mt < H / 16; mt++)
             g3[mt] = mma([&](int m, int k) { return s.H2[k][mt * 16 + m]; }, [&](int k, int j) { return s.D3[k][j]; }, 16,
@@ -228,10 +220,10 @@ k_mlp_grad(const short* __restrict__ wbf, const float* __restrict__ par, const f
             c = mma([&](int m, int k) { return s.D3[m][k]; }, [&](int k, int j) { return w.W3[nt * 16 + j][k]; }, OUTP,
                     c);
 #pragma unroll
-            for (int i = 0; i < 4; i++) s.D2[kq + i][nt * 16 + r] = f2bf(c[i] * gelu_grad(s.Z2[kq + i][nt * 16 + r]));
+            for (int i = 0; i < 4; i++) s.D2[kq + i][nt * 16 + r] = f2bf(c[i] * relu_grad(s.Z2[kq + i][nt * 16 + r]));
         }
         __syncthreads();
-        // layer 2: dW2 += H1^T D2, db2, dZ1 = (D2 W2^T) gelu'(Z1)
+        // layer 2: dW2 += H1^T D2, db2, dZ1 = (D2 W2^T) relu'(Z1)
 #pragma unroll
         for (int mt = 0; mt < H / 16; mt++)
 #pragma unroll
@@ -244,7 +236,7 @@ k_mlp_grad(const short* __restrict__ wbf, const float* __restrict__ par, const f
             f32x4 c = {0.f, 0.f, 0.f, 0.f};
             c = mma([&](int m, int k) { return s.D2[m][k]; }, [&](int k, int j) { return w.W2[nt * 16 + j][k]; }, H, c);
 #pragma unroll
-            for (int i = 0; i < 4; i++) s.D1[kq + i][nt * 16 + r] = f2bf(c[i] * gelu_grad(s.Z1[kq + i][nt * 16 + r]));
+            for (int i = 0; i < 4; i++) s.D1[kq + i][nt * 16 + r] = f2bf(c[i] * relu_grad(s.Z1[kq + i][nt * 16 + r]));
         }
         __syncthreads();
         // layer 1: dW1 += X^T D1, db1
@@ -312,25 +304,32 @@ __global__ void k_mlp_refresh(const float* __restrict__ par, short* __restrict__
     if (p < NPAR) wbf[p] = f2bf(par[p]);
 }
 
-// GELU as x * sigmoid(2 y), y = k0 (x + k1 x^3) -- the tanh form above
-// rewritten (0.5 (1 + tanh y) = sigmoid(2 y)) -- with the hardware exp2 and
-// reciprocal and one fused multiply-add: exp(-2 y) = exp2(x (c0 + c1 x^2)),
-// c0 = -2 log2(e) k0, c1 = c0 k1.  A few ulps of FP32 from gelu(), far below
-// the BF16 rounding the activations get next.  Five full-rate and two
-// quarter-rate VALU instructions: the inference kernel's main cost.
-__device__ __forceinline__ float gelu_fast(float x) {
-    constexpr float c0 = -2.8853900817779268f * 0.7978845608028654f;
-    constexpr float c1 = c0 * 0.044715f;
-    const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(x * x, c1, c0));
-    return x * __builtin_amdgcn_rcpf(1.f + e);
-}
-
 typedef __bf16 bf16x4_hw __attribute__((ext_vector_type(4)));
 
 // Four FP32 values to BF16 with the hardware conversion (v_cvt_pk_bf16_f32:
 // round to nearest even, f2bf's result for every finite value).
 __device__ __forceinline__ bf16x4 to_bf16x4(f32x4 v) {
     return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16x4_hw));
+}
+
+// C += A B over 32 k with gfx950's 16 x 16 x 32 BF16 MFMA, the operands two
+// 16 x 16 x 16 fragments each: A = (a0 | a1), B = (b0 | b1) per lane.  A lane
+// group's element e of A and of B address the same k, so the pairs (a0, b0)
+// and (a1, b1) -- k steps t and t + 1 of the K = 16 form -- are summed
+// together in one instruction instead of two.
+__device__ __forceinline__ f32x4 mfma_k32(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 b1, f32x4 c) {
+    typedef short s16x8 __attribute__((ext_vector_type(8)));
+    typedef __bf16 bf16x8_hw __attribute__((ext_vector_type(8)));
+    const s16x8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+    const s16x8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_hw, a), __builtin_bit_cast(bf16x8_hw, b), c,
+                                                   0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4 load_feat(const float* __restrict__ feat, int64_t q, int64_t n, int g) {
+    f32x4 f = {0.f, 0.f, 0.f, 0.f};
+    if (g < 2 && q < n) f = *reinterpret_cast<const f32x4*>(feat + q * FEAT_STRIDE + 4 * g);
+    return f;
 }
 
 // Inference over n samples, 16 per wavefront step, every layer computed
@@ -340,16 +339,20 @@ __device__ __forceinline__ bf16x4 to_bf16x4(f32x4 v) {
 // units, so activations stay in registers from input to output, and the
 // weights sit in registers as A fragments for the whole grid-stride loop
 // (W1^T 4 tiles, W2^T 16, W3^T 4: 48 VGPRs).  Biases start the
-// accumulators.  Inputs are read as one 16-byte vector per lane (features
-// 4g..4g+3 of the lane's sample, g < 2), outputs written by the lanes that
-// hold them.
+// accumulators; the hidden layers' biases are read from LDS per group (32
+// VGPRs fewer; 3 waves per SIMD without spills -- a bound of 4 spilled and
+// ran slower).  The hidden layers' K = 64 products run as two K = 32 MFMAs
+// each (mfma_k32).  Inputs are read as one 16-byte vector per lane
+// (features 4g..4g+3 of the lane's sample, g < 2; reading them three groups
+// ahead measured 5 % slower), outputs written by the lanes that hold them.
 __global__ void __launch_bounds__(4 * kWave)
 k_mlp_predict(const short* __restrict__ wbf, const float* __restrict__ par, const float* __restrict__ feat, int64_t n,
               int* __restrict__ branch, float* __restrict__ out) {
+    __shared__ f32x4 bias[2 * H / 4];  // b1, b2 in groups of 4 units
     const int lane = threadIdx.x & 63;
     const int s = lane & 15, g = lane >> 4;
     bf16x4 a1[H / 16], a2[H / 16][H / 16], a3[H / 16];
-    f32x4 c1[H / 16], c2[H / 16], c3;
+    f32x4 c3;
 #pragma unroll
     for (int t = 0; t < H / 16; t++) {
 #pragma unroll
@@ -357,43 +360,43 @@ k_mlp_predict(const short* __restrict__ wbf, const float* __restrict__ par, cons
             const int k = 4 * g + i;
             a1[t][i] = wbf[OFF_W1 + k * H + 16 * t + s];
             a3[t][i] = wbf[OFF_W3 + (16 * t + k) * OUTP + s];
-            c1[t][i] = par[OFF_B1 + 16 * t + k];
-            c2[t][i] = par[OFF_B2 + 16 * t + k];
 #pragma unroll
             for (int t2 = 0; t2 < H / 16; t2++) a2[t2][t][i] = wbf[OFF_W2 + (16 * t + k) * H + 16 * t2 + s];
         }
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) c3[i] = par[OFF_B3 + 4 * g + i];
-    const int64_t nw = (int64_t)gridDim.x * 4;
-    for (int64_t s0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16; s0 < n; s0 += nw * 16) {
+    for (int j = threadIdx.x; j < 2 * H / 4; j += blockDim.x) {
+        const int o = j < H / 4 ? OFF_B1 + 4 * j : OFF_B2 + 4 * (j - H / 4);
+        bias[j] = f32x4{par[o], par[o + 1], par[o + 2], par[o + 3]};
+    }
+    __syncthreads();
+    const int64_t step = (int64_t)gridDim.x * 4 * 16;
+    int64_t s0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    for (; s0 < n; s0 += step) {
         const int64_t q = s0 + s;
-        bf16x4 xb = {0, 0, 0, 0};
-        if (g < 2 && q < n) {
-            const f32x4 f = *reinterpret_cast<const f32x4*>(feat + q * FEAT_STRIDE + 4 * g);
-            xb = to_bf16x4(f);
-        }
+        const bf16x4 xb = to_bf16x4(load_feat(feat, q, n, g));
         bf16x4 h1[H / 16];
 #pragma unroll
         for (int t = 0; t < H / 16; t++) {
-            f32x4 z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[t], xb, c1[t], 0, 0, 0);
+            f32x4 z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[t], xb, bias[4 * t + g], 0, 0, 0);
 #pragma unroll
-            for (int i = 0; i < 4; i++) z[i] = gelu_fast(z[i]);
+            for (int i = 0; i < 4; i++) z[i] = relu(z[i]);
             h1[t] = to_bf16x4(z);
         }
         bf16x4 h2[H / 16];
 #pragma unroll
         for (int t2 = 0; t2 < H / 16; t2++) {
-            f32x4 z = c2[t2];
+            f32x4 z = bias[H / 4 + 4 * t2 + g];
 #pragma unroll
-            for (int t = 0; t < H / 16; t++) z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t2][t], h1[t], z, 0, 0, 0);
+            for (int t = 0; t < H / 16; t += 2) z = mfma_k32(a2[t2][t], a2[t2][t + 1], h1[t], h1[t + 1], z);
 #pragma unroll
-            for (int i = 0; i < 4; i++) z[i] = gelu_fast(z[i]);
+            for (int i = 0; i < 4; i++) z[i] = relu(z[i]);
             h2[t2] = to_bf16x4(z);
         }
         f32x4 o = c3;
 #pragma unroll
-        for (int t = 0; t < H / 16; t++) o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[t], h2[t], o, 0, 0, 0);
+        for (int t = 0; t < H / 16; t += 2) o = mfma_k32(a3[t], a3[t + 1], h2[t], h2[t + 1], o);
         // lane (s, g) holds outputs 4g..4g+3 of sample s; logit 4 sits in lane s + 16
         const float o4 = __shfl(o[0], s + 16, kWave);
         if (q < n) {

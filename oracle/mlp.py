@@ -2,7 +2,7 @@
 the checker of the GPU kernels in intpointsfem_amd/csrc/ipf_learned.hip,
 never part of the product path).
 
-7 -> 64 -> 64 -> 6 MLP with tanh-form GELU (Hendrycks & Gimpel 2016).
+7 -> 64 -> 64 -> 6 MLP with ReLU activations.
 `forward` / `gradient` round the matrix-core operands to BF16 (round to
 nearest even) where the GPU does -- inputs, weights, activations and the
 back-propagated dL/dZ -- and compute in float32; `loss64` / `gradient64`
@@ -23,7 +23,6 @@ OFF_B2 = OFF_W2 + H * H
 OFF_W3 = OFF_B2 + H
 OFF_B3 = OFF_W3 + H * OUTP
 NPAR = OFF_B3 + OUTP
-K0, K1 = 0.7978845608028654, 0.044715
 
 
 def bf16(x):
@@ -50,13 +49,14 @@ def trainable_mask():
     return m
 
 
-def _gelu(x, tanh=np.tanh):
-    return 0.5 * x * (1 + tanh(K0 * (x + K1 * x ** 3)))
+def _relu(x):
+    """max(x, 0); a NaN gives 0, as the GPU's v_max_f32 does."""
+    return np.fmax(x, 0)
 
 
-def _gelu_grad(x):
-    t = np.tanh(K0 * (x + K1 * x ** 3))
-    return 0.5 * (1 + t) + 0.5 * x * (1 - t * t) * K0 * (1 + 3 * K1 * x * x)
+def _relu_grad(x):
+    """1 above 0, else 0 (the GPU kernels' convention at 0)."""
+    return (x > 0).astype(x.dtype)
 
 
 def _inputs(feat):
@@ -70,9 +70,9 @@ def forward(p, feat):
     W1, b1, W2, b2, W3, b3 = [np.asarray(a, dtype=np.float32) for a in unpack(p)]
     X = bf16(_inputs(feat))
     Z1 = X @ bf16(W1) + b1
-    H1 = bf16(_gelu(Z1))
+    H1 = bf16(_relu(Z1))
     Z2 = H1 @ bf16(W2) + b2
-    H2 = bf16(_gelu(Z2))
+    H2 = bf16(_relu(Z2))
     Z3 = H2 @ bf16(W3) + b3
     return Z1, H1, Z2, H2, Z3, X
 
@@ -103,10 +103,10 @@ def gradient(p, feat, label, target):
     g = np.zeros(NPAR, dtype=np.float64)
     g[OFF_W3:OFF_B3] = (H2.T @ D3).reshape(-1)
     g[OFF_B3:NPAR] = D3.sum(axis=0)
-    D2 = bf16((D3 @ bf16(W3).T) * _gelu_grad(Z2))
+    D2 = bf16((D3 @ bf16(W3).T) * _relu_grad(Z2))
     g[OFF_W2:OFF_B2] = (H1.T @ D2).reshape(-1)
     g[OFF_B2:OFF_W3] = D2.sum(axis=0)
-    D1 = bf16((D2 @ bf16(W2).T) * _gelu_grad(Z1))
+    D1 = bf16((D2 @ bf16(W2).T) * _relu_grad(Z1))
     g[OFF_W1:OFF_B1] = (X.T @ D1).reshape(-1)
     g[OFF_B1:OFF_W2] = D1.sum(axis=0)
     g[~trainable_mask()] = 0.0
@@ -116,7 +116,7 @@ def gradient(p, feat, label, target):
 def loss64(p, feat, label, target):
     W1, b1, W2, b2, W3, b3 = unpack(np.asarray(p, dtype=np.float64))
     X = _inputs(feat).astype(np.float64)
-    Z3 = _gelu(_gelu(X @ W1 + b1) @ W2 + b2) @ W3 + b3
+    Z3 = _relu(_relu(X @ W1 + b1) @ W2 + b2) @ W3 + b3
     return _loss_grad_out(Z3, np.asarray(label), np.asarray(target, dtype=np.float64), len(label))[0]
 
 
@@ -124,18 +124,18 @@ def gradient64(p, feat, label, target):
     W1, b1, W2, b2, W3, b3 = unpack(np.asarray(p, dtype=np.float64))
     X = _inputs(feat).astype(np.float64)
     Z1 = X @ W1 + b1
-    H1 = _gelu(Z1)
+    H1 = _relu(Z1)
     Z2 = H1 @ W2 + b2
-    H2 = _gelu(Z2)
+    H2 = _relu(Z2)
     Z3 = H2 @ W3 + b3
     loss, D3 = _loss_grad_out(Z3, np.asarray(label), np.asarray(target, dtype=np.float64), len(label))
     g = np.zeros(NPAR)
     g[OFF_W3:OFF_B3] = (H2.T @ D3).reshape(-1)
     g[OFF_B3:NPAR] = D3.sum(axis=0)
-    D2 = (D3 @ W3.T) * _gelu_grad(Z2)
+    D2 = (D3 @ W3.T) * _relu_grad(Z2)
     g[OFF_W2:OFF_B2] = (H1.T @ D2).reshape(-1)
     g[OFF_B2:OFF_W3] = D2.sum(axis=0)
-    D1 = (D2 @ W2.T) * _gelu_grad(Z1)
+    D1 = (D2 @ W2.T) * _relu_grad(Z1)
     g[OFF_W1:OFF_B1] = (X.T @ D1).reshape(-1)
     g[OFF_B1:OFF_W2] = D1.sum(axis=0)
     g[~trainable_mask()] = 0.0
