@@ -2267,10 +2267,11 @@ int ipf_newton(ipf_ctx* ctx, double* u_dev, double rtol, int maxit, double cg_rt
     double mt_alpha = 1.0;  // the Thomas acceleration factor, main.cpp:171
     int it = 0, cg_total = 0, ls_halved = 0;
     bool converged = false;
-    // each iteration up to its first host read as one graph replay (when
-    // the first CG block fits in cg_maxit and the capture succeeds)
+    // each iteration up to its first host read as one graph replay (when an
+    // update can follow, the first CG block fits in cg_maxit and the capture
+    // succeeds)
     hipGraphExec_t step = nullptr;
-    if (ctx->newton_graph && cg_maxit >= kCgBlock && cg_rtol > 0) {
+    if (ctx->newton_graph && maxit > 0 && cg_maxit >= kCgBlock && cg_rtol > 0) {
         if (ctx->tfmt == FMT_G32 && !ctx->dDep32) rc = dev_alloc(ctx, &ctx->dDep32, 9 * ctx->npad);
         if (!rc) step = newton_step_graph(ctx, u_dev);
     }
