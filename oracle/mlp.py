@@ -50,7 +50,8 @@ def trainable_mask():
 
 
 def _relu(x):
-    """max(x, 0); a NaN gives 0, as the GPU's v_max_f32 does."""
+    """max(x, 0) (the GPU takes it on the bit pattern: the same for every
+    non-NaN value, -0 to +0)."""
     return np.fmax(x, 0)
 
 
