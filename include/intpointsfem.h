@@ -416,6 +416,18 @@ int ipf_residual_tangent(ipf_ctx *ctx, const double *u_dev, double *r_dev);
  * and color order. */
 int ipf_set_tangent_pattern(ipf_ctx *ctx, const int *el_color_index, const int64_t *color_l_sequence, int64_t nnz);
 
+/* The same positions, with the pattern itself, built on the device from the
+ * uploaded DoF indexes -- the symmetric upper-triangle CSR of the element
+ * graph that TPZElastoPlasticIntPointsStructMatrix::Create() allocates (every
+ * (i, j), i <= j, of two DoFs of one element; columns ascending in each row)
+ * and the positions FillLIndexes / me() look up (TPZNumericalIntegrator.cpp:
+ * 538-606) -- by a sort of the element entries' (row, column) keys: the same
+ * integers as the host build.  *nnz = the pattern's entries; replaces an
+ * ipf_set_tangent_pattern.  ipf_get_tangent_pattern copies the pattern to
+ * host arrays ia [neq + 1] and ja [nnz] (either may be NULL). */
+int ipf_build_tangent_pattern(ipf_ctx *ctx, int64_t *nnz);
+int ipf_get_tangent_pattern(ipf_ctx *ctx, int64_t *ia_host, int64_t *ja_host);
+
 /* TPZNumericalIntegrator::KAssembly (TPZNumericalIntegrator.cpp:131-214, GPU
  * variant :184-214 with MatrixAssembleKernel, KernelsMatrixAssemble.h:24-56):
  * ipf_residual_tangent, then kg_dev[nnz] = 0 and, color by color, every

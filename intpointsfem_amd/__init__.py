@@ -70,6 +70,14 @@ def lib():
     if not os.path.exists(_LIB_PATH):
         raise IPFError(f"{_LIB_PATH} is missing: build it with `python -m intpointsfem_amd.buildlib` "
                        "(there is no CPU fallback)")
+    # torch brings its own HIP runtime: loaded after this library's (the
+    # system one), the two do not share the GPU -- the library's contexts
+    # then find no device once torch has initialised.  Loading torch first
+    # makes the library bind to torch's runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(_LIB_PATH)
     sig = {
         # device layer
@@ -103,6 +111,8 @@ def lib():
         "ipf_residual_tangent": (_int, [_vp, _vp, _vp]),
         "ipf_tangent_apply": (_int, [_vp, _vp, _vp]),
         "ipf_set_tangent_pattern": (_int, [_vp, _ip, _lp, _i64]),
+        "ipf_build_tangent_pattern": (_int, [_vp, _lp]),
+        "ipf_get_tangent_pattern": (_int, [_vp, _lp, _lp]),
         "ipf_assemble_tangent": (_int, [_vp, _vp, _vp, _vp]),
         "ipf_cg_solve": (_int, [_vp, _vp, _vp, _d, _int, _ip, _dp]),
         "ipf_newton": (_int, [_vp, _vp, _d, _int, _d, _int, _dp]),
@@ -542,6 +552,20 @@ class DeviceContext:
         e = np.ascontiguousarray(el_color_index, dtype=np.int32)
         l = np.ascontiguousarray(color_l_sequence, dtype=np.int64)
         self._c(lib().ipf_set_tangent_pattern(self._h, _p(e, _ip), _p(l, _lp), int(nnz)), "ipf_set_tangent_pattern")
+
+    def build_tangent_pattern(self):
+        """The tangent's CSR pattern and the element entries' positions built
+        on the device; returns the number of pattern entries."""
+        n = np.zeros(1, dtype=np.int64)
+        self._c(lib().ipf_build_tangent_pattern(self._h, _p(n, _lp)), "ipf_build_tangent_pattern")
+        return int(n[0])
+
+    def get_tangent_pattern(self, neq, nnz):
+        """(ia [neq + 1], ja [nnz]) of the pattern build_tangent_pattern built."""
+        ia = np.zeros(neq + 1, dtype=np.int64)
+        ja = np.zeros(nnz, dtype=np.int64)
+        self._c(lib().ipf_get_tangent_pattern(self._h, _p(ia, _lp), _p(ja, _lp)), "ipf_get_tangent_pattern")
+        return ia, ja
 
     def assemble_tangent(self, u, r, kg):
         """KAssembly: residual into r and the CSR values of K into kg (device tensors)."""

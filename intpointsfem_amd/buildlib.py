@@ -18,6 +18,7 @@ SOURCES = [
     "ipf_solver.hip",
     "ipf_kernels_fmt.hip",
     "ipf_assemble.hip",
+    "ipf_pattern.hip",
     "ipf_learned.hip",
     "ipf_patch.hip",
     "ipf_mlp.hip",

@@ -46,6 +46,17 @@ public:
     void SetUpColoredIndexes(TPZCompMesh* cmesh);
     void FillLIndexes(TPZVec<int64_t>& IA, TPZVec<int64_t>& JA);
     int64_t me(TPZVec<int64_t>& IA, TPZVec<int64_t>& JA, int64_t& i_dest, int64_t& j_dest);
+    // FillLIndexes in two parts: the per-color offsets (fMatrixStride,
+    // fFirstColorLIndex; cheap) and the positions fColorLSequence (one lookup
+    // per element entry)
+    void FillLOffsets();
+    void FillColorLSequence(TPZVec<int64_t>& IA, TPZVec<int64_t>& JA);
+    // Create()'s pattern and the entries' positions built on the device
+    // (ipf_build_tangent_pattern): IA / JA filled from there, the positions
+    // kept on the device; fColorLSequence is then filled only on request
+    // (ColorLSequenceFor).
+    void BuildPatternOnDevice(TPZVec<int64_t>& IA, TPZVec<int64_t>& JA);
+    TPZVec<int64_t>& ColorLSequenceFor(TPZVec<int64_t>& IA, TPZVec<int64_t>& JA);
     bool isBuilt() { return fBlockMatrix.Rows() != 0; }
 
     // Creates the device context (if needed) and sets up the data there

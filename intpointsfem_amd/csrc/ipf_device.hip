@@ -304,6 +304,9 @@ struct ipf_ctx {
     int* dLidx = nullptr;
     int64_t nnz_k = 0;
     bool have_pattern = false;
+    // the pattern itself when ipf_build_tangent_pattern built it on the device
+    int64_t* dPatIA = nullptr;  // [neq + 1]
+    int64_t* dPatJA = nullptr;  // [nnz_k]
 };
 
 namespace {
@@ -414,6 +417,8 @@ void release_data(ipf_ctx* ctx) {
     for (auto& o : ctx->tab_off) o = -1;
     ctx->have_geometry = false;
     dev_free(ctx->dLidx);
+    dev_free(ctx->dPatIA);
+    dev_free(ctx->dPatJA);
     dev_free(ctx->dTmeta);
     dev_free(ctx->dNewOf);
     dev_free(ctx->dPmat);
@@ -2196,11 +2201,90 @@ int ipf_set_tangent_pattern(ipf_ctx* ctx, const int* el_color_index, const int64
     IPF_HIP(ctx, hipSetDevice(ctx->device));
     IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     dev_free(ctx->dLidx);
+    dev_free(ctx->dPatIA);
+    dev_free(ctx->dPatJA);
     ctx->have_pattern = false;
     if ((rc = dev_alloc(ctx, &ctx->dLidx, total))) return rc;
     if (total) IPF_HIP(ctx, hipMemcpy(ctx->dLidx, L.data(), sizeof(int) * (size_t)total, hipMemcpyHostToDevice));
     ctx->nnz_k = nnz;
     ctx->have_pattern = true;
+    return IPF_OK;
+}
+
+int ipf_build_tangent_pattern(ipf_ctx* ctx, int64_t* nnz) {
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (!nnz) return fail(ctx, IPF_EINVAL, "ipf_build_tangent_pattern: NULL pointer");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipStream_t s = ctx->stream;
+    int64_t total = 0;
+    for (auto& g : ctx->groups) {
+        const int64_t ce = shape_ce(g.shape);
+        g.l_off = total;
+        total += (int64_t)g.elems.size() * (ce * (ce + 1) / 2);
+    }
+    dev_free(ctx->dLidx);
+    dev_free(ctx->dPatIA);
+    dev_free(ctx->dPatJA);
+    ctx->have_pattern = false;
+    ctx->nnz_k = 0;
+    if (total == 0) return fail(ctx, IPF_ESTATE, "ipf_build_tangent_pattern: no elements");
+    uint64_t *keys = nullptr, *sorted = nullptr;
+    unsigned int* count = nullptr;
+    if ((rc = dev_alloc(ctx, &keys, total)) || (rc = dev_alloc(ctx, &sorted, total)) || (rc = dev_alloc(ctx, &count, 1))) {
+        dev_free(keys);
+        dev_free(sorted);
+        return rc;
+    }
+    for (const auto& g : ctx->groups) launch_pattern_keys(g.shape, group_dev(ctx, g), ctx->neq, keys + g.l_off, s);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = pattern_sort_unique(keys, sorted, total, count, s);
+    unsigned int nuniq = 0;
+    if (e == hipSuccess) e = hipMemcpy(&nuniq, count, sizeof(nuniq), hipMemcpyDeviceToHost);
+    dev_free(sorted);
+    dev_free(count);
+    uint64_t last = 0;
+    if (e == hipSuccess && nuniq > 0) e = hipMemcpy(&last, keys + nuniq - 1, sizeof(last), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        dev_free(keys);
+        return fail(ctx, IPF_EHIP, "ipf_build_tangent_pattern: %s", hipGetErrorString(e));
+    }
+    // the padding slots' key, if any, sorts last
+    const int64_t n = (int64_t)nuniq - (nuniq > 0 && last == ~0ull ? 1 : 0);
+    if (n < 1 || n > INT32_MAX) {
+        dev_free(keys);
+        return fail(ctx, IPF_EINVAL, "ipf_build_tangent_pattern: %lld pattern entries outside [1, 2^31 - 1]",
+                    (long long)n);
+    }
+    if ((rc = dev_alloc(ctx, &ctx->dPatIA, ctx->neq + 1)) || (rc = dev_alloc(ctx, &ctx->dPatJA, n)) ||
+        (rc = dev_alloc(ctx, &ctx->dLidx, total))) {
+        dev_free(keys);
+        dev_free(ctx->dPatIA);
+        dev_free(ctx->dPatJA);
+        return rc;
+    }
+    launch_pattern_rows(keys, n, ctx->neq, ctx->dPatIA, ctx->dPatJA, s);
+    for (const auto& g : ctx->groups)
+        launch_pattern_positions(g.shape, group_dev(ctx, g), ctx->dPatIA, ctx->dPatJA, ctx->dLidx + g.l_off, s);
+    e = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(s);
+    dev_free(keys);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return fail(ctx, IPF_EHIP, "ipf_build_tangent_pattern: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    ctx->nnz_k = n;
+    ctx->have_pattern = true;
+    *nnz = n;
+    return IPF_OK;
+}
+
+int ipf_get_tangent_pattern(ipf_ctx* ctx, int64_t* ia_host, int64_t* ja_host) {
+    if (!ctx) return fail(nullptr, IPF_EINVAL, "null context");
+    if (!ctx->dPatIA) return fail(ctx, IPF_ESTATE, "ipf_get_tangent_pattern: no pattern (ipf_build_tangent_pattern)");
+    IPF_HIP(ctx, hipSetDevice(ctx->device));
+    IPF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ia_host) IPF_HIP(ctx, hipMemcpy(ia_host, ctx->dPatIA, sizeof(int64_t) * (ctx->neq + 1), hipMemcpyDeviceToHost));
+    if (ja_host) IPF_HIP(ctx, hipMemcpy(ja_host, ctx->dPatJA, sizeof(int64_t) * ctx->nnz_k, hipMemcpyDeviceToHost));
     return IPF_OK;
 }
 

@@ -319,8 +319,11 @@ int ipf_sm_export_tangent_pattern(const ipf_sm* sm, int64_t* ia, int64_t* ja, in
     auto& in = sm->sm->Integrator();
     if (ia) std::memcpy(ia, sm->K->IA().data(), sizeof(int64_t) * sm->K->IA().size());
     if (ja) std::memcpy(ja, sm->K->JA().data(), sizeof(int64_t) * sm->K->JA().size());
-    if (color_l_sequence)
-        std::memcpy(color_l_sequence, in.ColorLSequence().data(), sizeof(int64_t) * in.ColorLSequence().size());
+    if (color_l_sequence) {
+        // filled on the host on first request when the device built the pattern
+        auto& cl = in.ColorLSequenceFor(sm->K->IA(), sm->K->JA());
+        std::memcpy(color_l_sequence, cl.data(), sizeof(int64_t) * cl.size());
+    }
     if (first_color_l_index)
         std::memcpy(first_color_l_index, in.FirstColorLIndex().data(),
                     sizeof(int64_t) * in.FirstColorLIndex().size());

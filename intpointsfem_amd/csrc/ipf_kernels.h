@@ -144,6 +144,19 @@ void launch_tangent_diag(int shape, const GroupDev& g, double* diag, double* off
 void launch_assemble_K(int shape, const GroupDev& g, const double* dep_soa, int64_t npad, const int* lidx,
                        double* kg, hipStream_t stream);
 
+// The tangent's CSR pattern on the device (ipf_pattern.hip): keys of a
+// group's element entries (packed upper triangles by element slot, like
+// lidx), sort + unique of n keys in place (keys: the distinct keys after the
+// call, *count_dev their number; sorted: n scratch keys; synchronises s),
+// ia / ja from the distinct keys, and the entries' positions.
+void launch_pattern_keys(int shape, const GroupDev& g, int64_t neq, uint64_t* keys, hipStream_t stream);
+hipError_t pattern_sort_unique(uint64_t* keys, uint64_t* sorted, int64_t n, unsigned int* count_dev,
+                               hipStream_t stream);
+void launch_pattern_rows(const uint64_t* uniq, int64_t nnz, int64_t neq, int64_t* ia, int64_t* ja,
+                         hipStream_t stream);
+void launch_pattern_positions(int shape, const GroupDev& g, const int64_t* ia, const int64_t* ja, int* lidx,
+                              hipStream_t stream);
+
 // The fused residual and the tangent product (ipf_kernels_fmt.hip): Bv points
 // at the group's first tile in format fmt (FMT_D64: the dense tiles g.B;
 // FMT_C64, FMT_D32, FMT_C32: the converted copies), or for FMT_G64 at the
